@@ -1,0 +1,243 @@
+/*
+ * gbhip.h -- C-ABI of the MI355X generalized-Born path of nabc_amd.
+ *
+ * This is the drop-in boundary for two file-static functions of libsff:
+ *
+ *   egb(lpears, upears, pearlist, lpearsnp, upearsnp, pearlistnp, x, f, fs,
+ *       rborn, q, kappa, diel_ext, enb, eelt, esurf, enp, freevectors)
+ *       src/sff/eff.c:1363-1367 (Born radii, GB + Coulomb + LJ pair terms,
+ *       sumdeijda chain rule; returns epol, accumulates the gradient into f)
+ *   nblist(lpears, upears, pearlist, x, context_PxQ, derivs, cutoff, natom,
+ *          dim, frozen)
+ *       src/sff/nblist.c:813-815 (pair list: r^2 < cut^2 at build time,
+ *       not both frozen; returns the number of list entries, 2 per pair)
+ *
+ * mme34() (eff.c:2442) is their only caller on the energy path; INTEGRATION.md
+ * shows the lines that replace the two calls with the functions below.
+ * Everything here is plain C: pointers, sizes and int status codes
+ * (0 = success, negative = error, message via gbhip_last_error()).
+ * No function calls exit().
+ *
+ * State lives in an opaque context (one per molecule) instead of the
+ * file-scope statics that egb()/nblist() keep (eff.c:1375-1376,
+ * sff.c:68-234); a batch handle scores many molecules in one launch.
+ */
+#ifndef NABC_AMD_GBHIP_H
+#define NABC_AMD_GBHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBHIP_OK 0
+#define GBHIP_EINVAL (-1)   /* bad argument or option combination */
+#define GBHIP_ENOMEM (-2)   /* device or host allocation failed */
+#define GBHIP_EHIP (-3)     /* a HIP runtime call failed */
+#define GBHIP_ESTATE (-4)   /* call out of order (e.g. egb before nblist) */
+#define GBHIP_ENODEV (-5)   /* no usable GPU */
+
+#define GBHIP_FP64 0        /* all pair arithmetic in FP64 */
+#define GBHIP_MIXED 1       /* FP32 pair arithmetic, FP64 sums and scaling */
+
+typedef struct gbhip_ctx gbhip_ctx;
+typedef struct gbhip_batch gbhip_batch;
+
+/*
+ * Per-atom topology the GB path reads (the fields egb() takes from prm and
+ * the sff.c statics, eff.c:1363-2405).  All arrays have natom entries unless
+ * stated.  Values are copied at context creation.
+ */
+typedef struct {
+    int natom;
+    int ntypes;
+    const double *charge;    /* prmtop unit: e * 18.2223 (prm.c:777-792) */
+    const double *rborn;     /* intrinsic radii, prm->Rborn */
+    const double *fs;        /* screening factors after mme_init (gb 7/8 set them) */
+    const int *iac;          /* 1-based Lennard-Jones type, prm->Iac */
+    const int *cno;          /* ntypes*ntypes, 1-based index into cn1/cn2; <= 0: none */
+    const double *cn1;       /* ntypes*(ntypes+1)/2 r^-12 coefficients */
+    const double *cn2;       /* ntypes*(ntypes+1)/2 r^-6 coefficients */
+    const int *iblo;         /* number of excluded-atom entries per atom, prm->Iblo */
+    const int *exclat;       /* sum(iblo) entries, 1-based, 0 = none, prm->ExclAt */
+    const int *frozen;       /* per-atom flag, may be NULL (none frozen) */
+    const double *gbalpha;   /* OBC / neck alpha, beta, gamma (gb 2,5,7,8), */
+    const double *gbbeta;    /*   may be NULL for gb 1                       */
+    const double *gbgamma;
+    const int *neckidx;      /* gb 7/8: (int)((rborn-1)*20+0.5), else NULL */
+} gbhip_topology;
+
+typedef struct {
+    int gb;              /* 1, 2, 5, 7 or 8 */
+    int precision;       /* GBHIP_FP64 or GBHIP_MIXED */
+    double cut;          /* pair-list cutoff (A) */
+    double rgbmax;       /* Born-integral cutoff (A), must be <= cut */
+    double fsmax;        /* prm->Fsmax as read from the prmtop */
+    double gboffset;     /* 0.09; 0.195141 for gb 8 */
+    double gbneckscale;  /* gb 7: 0.361825, gb 8: 0.826836 */
+    double kappa;        /* Debye-Hueckel inverse length (1/A) */
+    double epsext;       /* solvent dielectric */
+} gbhip_options;
+
+/* Last error message of the calling thread's most recent failed call. */
+const char *gbhip_last_error(void);
+
+/* Number of visible GPUs (0 when none); does not create a context. */
+int gbhip_device_count(void);
+
+/*
+ * Create a context on the current HIP device.  x0 (3*natom, AoS, may be
+ * NULL) fixes the spatial ordering of atoms into 64-atom tiles.
+ * Replaces the first-call allocations of egb() (eff.c:1519-1543) and the
+ * pair-list heads of mme_init_sff (sff.c:1451-1474).
+ */
+int gbhip_ctx_create(const gbhip_topology *top, const gbhip_options *opt,
+                     const double *x0, gbhip_ctx **out);
+
+/* Release everything (egb(..., freevectors=1), eff.c:1463-1491). */
+int gbhip_ctx_destroy(gbhip_ctx *ctx);
+
+/* Host coordinates, AoS x[3*natom] (copied to the device). */
+int gbhip_set_coords(gbhip_ctx *ctx, const double *x);
+
+/*
+ * Rebuild the pair list from the current coordinates (nblist(),
+ * nblist.c:813).  *nentries (may be NULL) receives lower+upper entries,
+ * i.e. twice the number of pairs, which is what nblist() returns.
+ */
+int gbhip_nblist(gbhip_ctx *ctx, long long *nentries);
+
+/*
+ * One GB energy + gradient evaluation over the current list (egb() with
+ * freevectors=0).  grad (3*natom AoS, host, may be NULL) is ACCUMULATED
+ * into, like f in egb(); epol is egb()'s return value, evdw and eelec its
+ * enb/eelt outputs.  Synchronous.
+ */
+int gbhip_egb(gbhip_ctx *ctx, double *grad, double *epol, double *evdw,
+              double *eelec);
+
+/* Effective Born radii of the last evaluation, original atom order. */
+int gbhip_get_reff(gbhip_ctx *ctx, double *reff);
+
+/* sumdeijda (dE/dR_i * R_i^2) of the last evaluation, original order. */
+int gbhip_get_sumdeijda(gbhip_ctx *ctx, double *sumdeijda);
+
+/* ---- device-resident use (no host copies; asynchronous on the stream) ---- */
+
+/* Stream for all work of this context (a hipStream_t; NULL = default). */
+int gbhip_set_stream(gbhip_ctx *ctx, void *stream);
+
+/* Coordinates from device memory d_x[3*natom] (AoS, FP64). */
+int gbhip_set_coords_device(gbhip_ctx *ctx, const double *d_x);
+
+/* Pair-list rebuild without reading the count back to the host. */
+int gbhip_nblist_async(gbhip_ctx *ctx);
+
+/*
+ * Evaluation writing to device memory: d_grad[3*natom] is accumulated into
+ * (may be NULL), d_ene[3] receives {epol, evdw, eelec} (overwritten).
+ */
+int gbhip_egb_device(gbhip_ctx *ctx, double *d_grad, double *d_ene);
+
+/* Wait for the context's stream. */
+int gbhip_sync(gbhip_ctx *ctx);
+
+/* Pair count of the current list (unordered pairs); synchronous. */
+int gbhip_npairs(gbhip_ctx *ctx, long long *npairs);
+
+/*
+ * The current list as unordered pairs (original atom indices, i in pi, j in
+ * pj, unspecified order); at most cap are written, *npairs gets the total.
+ * Read-back for checks; allocates temporary device memory.
+ */
+int gbhip_get_pairs(gbhip_ctx *ctx, int *pi, int *pj, long long cap,
+                    long long *npairs);
+
+/* Tile statistics: 64-atom tiles and stored tile pairs of the current list. */
+int gbhip_tile_stats(gbhip_ctx *ctx, int *ntiles, int *ntilepairs);
+
+/* ---- the rest of mme(): bonded and 1-4 terms on the device ---- */
+
+/*
+ * Bonded topology of one molecule, 0-based atom indices (not the prmtop's
+ * 3*(atom-1) offsets), one entry per term: a dihedral series (Pn < 0 in the
+ * prmtop, eff.c:619-623) is passed as one entry per Fourier term.
+ * 1-4 pairs carry 1/SCEE and 1/SCNB and their LJ coefficients (p14_hb = 1
+ * selects the 10-12 form, eff.c:1224-1234).  dield: 1 = diel=C, 0 = diel=R.
+ * Replaces ebond/eangl/ephi (eff.c:82-627) and nbond(N14=1)
+ * (eff.c:896-1331) as called from mme34 (eff.c:2617-2653).
+ */
+typedef struct {
+    int nbond;
+    const int *bond_i, *bond_j;
+    const double *bond_rk, *bond_req;
+    int nangle;
+    const int *angle_i, *angle_j, *angle_k;
+    const double *angle_tk, *angle_teq;
+    int ndihed;
+    const int *dihed_i, *dihed_j, *dihed_k, *dihed_l;
+    const double *dihed_pk, *dihed_pn, *dihed_phase;
+    int npair14;
+    const int *p14_i, *p14_j, *p14_hb;
+    const double *p14_scee, *p14_scnb, *p14_a, *p14_b;
+    const double *charge;    /* natom, same as the topology's */
+    int dield;
+} gbhip_bonded;
+
+int gbhip_set_bonded(gbhip_ctx *ctx, const gbhip_bonded *b);
+
+/*
+ * Whole mme34 evaluation (bonded + 1-4 + GB) over the current list and
+ * coordinates.  grad (3*natom, host) is OVERWRITTEN with the gradient,
+ * frozen atoms zeroed (eff.c:2826-2834); ene14[0..12] follows mme34's ene[]
+ * (eff.c:2617-2874, [0] = total) and ene14[13] is the rms gradient.
+ */
+int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14);
+/* Same, device pointers, asynchronous. */
+int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14);
+
+/* ---- batched scoring: many molecules in one launch per pass ---- */
+
+/*
+ * A batch of nmol independent molecules sharing one option set.  x0[m]
+ * holds molecule m's coordinates (3*natom_m AoS).  The pair lists are built
+ * at creation from x0.
+ */
+int gbhip_batch_create(int nmol, const gbhip_topology *tops,
+                       const gbhip_options *opt, const double *const *x0,
+                       gbhip_batch **out);
+int gbhip_batch_destroy(gbhip_batch *b);
+
+/* Replace molecule m's coordinates (host AoS). */
+int gbhip_batch_set_coords(gbhip_batch *b, int m, const double *x);
+
+/* Rebuild every molecule's pair list from its current coordinates. */
+int gbhip_batch_nblist(gbhip_batch *b);
+
+/*
+ * Evaluate all molecules: ene[3*nmol] = {epol, evdw, eelec} per molecule;
+ * grads[m] (3*natom_m, may be NULL or hold NULL entries) accumulated into.
+ */
+int gbhip_batch_eval(gbhip_batch *b, double *ene, double *const *grads);
+
+/* Evaluation only, results left on the device (for timing). */
+int gbhip_batch_eval_device(gbhip_batch *b);
+
+/* Total unordered pairs and atoms of the batch. */
+int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms,
+                      int *ntilepairs);
+
+/* ---- synthetic systems (deterministic generator) ---- */
+
+struct parm;
+/* Build an N-atom synthetic polymer; x receives 3N coordinates. */
+int gbgen_build(int natom, unsigned long long seed, struct parm **prm,
+                double *x);
+/* Same, written as a prmtop and an rst7 file. */
+int gbgen_write(int natom, unsigned long long seed, const char *prmtop_path,
+                const char *rst7_path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

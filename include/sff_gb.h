@@ -1,0 +1,115 @@
+/*
+ * sff_gb.h -- host-side C API of nabc_amd, mirroring the libsff entry points
+ * that sit around the generalized-Born energy path.
+ *
+ * Every function keeps the name, argument order and argument meaning of the
+ * libsff routine it replaces (src/sff/sff.h:124-180 in dacase/nabc):
+ *
+ *   rdparm        prm.c:584          read an Amber prmtop (new %FLAG format)
+ *   mm_options    mm_options.l:276   keyword=value option string
+ *   mme_init_sff  sff.c:858          GB parameter set-up, exclusion lists
+ *   mme           sff.c:1854         energy + gradient, returns the total
+ *   md            sff.c:2071         velocity-Verlet / Langevin dynamics
+ *   conjgrad      conjgrad.c:34      Polak-Ribiere CG with Powell restarts
+ *   getxv, putxv  sff.c:651, :733    rst7 coordinate / velocity files
+ *
+ * The generalized-Born pair work (egb() and nblist() of eff.c/nblist.c) runs
+ * on the GPU through the C-ABI in gbhip.h; mme() here orchestrates it.
+ *
+ * One documented deviation: where libsff prints and calls exit(1), these
+ * functions print the same message to stderr and return a negative code
+ * (mme returns NaN).  mm_get_error() gives the last message.
+ */
+#ifndef NABC_AMD_SFF_GB_H
+#define NABC_AMD_SFF_GB_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NABC_NAMESIZE 6
+
+/*
+ * Topology as read from a prmtop.  Field names follow PARMSTRUCT_T
+ * (src/sff/sff.h:40-62).  Atom indices in the bond/angle/dihedral arrays are
+ * stored as in the file (3*(atom-1), negative third/fourth index flags as in
+ * Amber); ExclAt is 1-based with 0 meaning "none".  Charges are in the
+ * prmtop's internal unit (electron charge * 18.2223).
+ */
+typedef struct parm {
+    char ititl[81];
+    int IfBox, Nmxrs, IfCap;
+    int Natom, Ntypes, Nbonh, Mbona, Ntheth, Mtheta, Nphih, Mphia,
+        Nhparm, Nparm, Nnb, Nres, Nbona, Ntheta, Nphia, Numbnd, Numang,
+        Nptra, Natyp, Nphb, Ntype2d, Nttyp;
+    char (*AtomNames)[NABC_NAMESIZE];
+    double *Charges, *Masses, *Rk, *Req, *Tk, *Teq, *Pk, *Pn, *Phase,
+           *Cn1, *Cn2, *HB12, *HB10, *Rborn, *Fs, *Scee, *Scnb;
+    double Fsmax;
+    int *Iac, *Iblo, *Cno, *Ipres, *ExclAt;
+    int *BondHAt1, *BondHAt2, *BondHNum, *BondAt1, *BondAt2, *BondNum;
+    int *AngleHAt1, *AngleHAt2, *AngleHAt3, *AngleHNum;
+    int *AngleAt1, *AngleAt2, *AngleAt3, *AngleNum;
+    int *DihHAt1, *DihHAt2, *DihHAt3, *DihHAt4, *DihHNum;
+    int *DihAt1, *DihAt2, *DihAt3, *DihAt4, *DihNum;
+    int *AtomicNum;
+    int *N14pairs, *N14pairlist;
+    double *N14sceelist, *N14scnblist;
+} PARMSTRUCT_T;
+
+/* prmtop I/O (prm.c:584 rdparm, prm.c:1997 wrparm). */
+PARMSTRUCT_T *rdparm(char *name);
+int wrparm(PARMSTRUCT_T *prm, char *name);
+void freeparm(PARMSTRUCT_T *prm);
+
+/* rst7 I/O (sff.c:651 getxv, sff.c:733 putxv). */
+int getxv(char *fname, int natom, double start_time, double *x, double *v);
+int putxv(char *fname, char *title, int natom, double start_time,
+          double *x, double *v);
+
+/* Options (mm_options.l:28-97) and the defaults of sff.c:68-234. */
+int mm_options(char *opts);
+void mm_reset_options(void);
+const char *mm_get_error(void);
+
+/*
+ * Initialisation (sff.c:858).  frozen[] and constrained[] are per-atom
+ * flags (may be NULL = none).  x0 is kept for restraints (unused by GB).
+ * ncname must be NULL (netCDF trajectories are not built).
+ */
+int mme_init_sff(PARMSTRUCT_T *prm, int *frozen, int *constrained,
+                 double *x0, char *ncname);
+
+/*
+ * Energy and gradient (sff.c:1854 -> eff.c:2442).  x and f are 3*Natom,
+ * f receives the gradient (not the force).  iter semantics: -1 prints the
+ * per-term table, -3 releases the device state, -4 forces a pair-list
+ * rebuild, other negatives run silently without a rebuild.
+ */
+double mme(double *x, double *f, int *iter);
+
+/*
+ * Per-term energies of the last mme() call, indexed as ene[] in mme34
+ * (eff.c:2617-2874): [0] total, [1] enb, [2] eel, [3] bond, [4] angle,
+ * [5] dihedral, [7] enb14, [8] eel14, [9] constraint, [10] egb,
+ * [11] esurf, [12] evdwnp.  Also the rms gradient of that call.
+ */
+void mme_get_energies(double ene[13], double *frms);
+
+/* Dynamics (sff.c:2071) and conjugate gradients (conjgrad.c:34). */
+int md(int n, int maxstep, double *x, double *f, double *v,
+       double (*func)(double *, double *, int *));
+int conjgrad(double x[], int *n, double *f,
+             double (*func)(double *, double *, int *),
+             double *rmsgrad, double *dfpred, int *maxiter);
+
+/* Output stream used for the ff:/md: lines (libsff's nabout). */
+void mm_set_output(FILE *fp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,347 @@
+"""nabc_amd -- MI355X generalized-Born energy path (Python bindings).
+
+Thin ctypes bindings over libnabc_amd.so, the C-ABI declared in
+include/gbhip.h (GPU engine) and include/sff_gb.h (libsff-compatible host
+API: rdparm, mm_options, mme_init_sff, mme, md, conjgrad, getxv, putxv).
+The library is built in-tree by ``make -C nabc_amd``; importing this module
+raises if it is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnabc_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libnabc_amd.so is not built (run `make -C nabc_amd` or __graft_entry__.build())")
+
+lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+GBHIP_FP64 = 0
+GBHIP_MIXED = 1
+
+# mme34 energy-vector layout (eff.c:2617-2874); index 13 is the rms gradient
+ENE_NAMES = {0: "total", 1: "enb", 2: "eel", 3: "bond", 4: "angle", 5: "dihedral",
+             7: "enb14", 8: "eel14", 9: "econs", 10: "egb", 11: "esurf", 12: "evdwnp",
+             13: "frms"}
+
+
+class Topology(C.Structure):
+    _fields_ = [("natom", C.c_int), ("ntypes", C.c_int), ("charge", dp), ("rborn", dp),
+                ("fs", dp), ("iac", ip), ("cno", ip), ("cn1", dp), ("cn2", dp),
+                ("iblo", ip), ("exclat", ip), ("frozen", ip), ("gbalpha", dp),
+                ("gbbeta", dp), ("gbgamma", dp), ("neckidx", ip)]
+
+
+class Options(C.Structure):
+    _fields_ = [("gb", C.c_int), ("precision", C.c_int), ("cut", C.c_double),
+                ("rgbmax", C.c_double), ("fsmax", C.c_double), ("gboffset", C.c_double),
+                ("gbneckscale", C.c_double), ("kappa", C.c_double), ("epsext", C.c_double)]
+
+
+class Parm(C.Structure):
+    """Prefix of PARMSTRUCT_T (include/sff_gb.h); only leading fields are read."""
+    _fields_ = [("ititl", C.c_char * 81), ("IfBox", C.c_int), ("Nmxrs", C.c_int),
+                ("IfCap", C.c_int), ("Natom", C.c_int), ("Ntypes", C.c_int),
+                ("Nbonh", C.c_int), ("Mbona", C.c_int), ("Ntheth", C.c_int),
+                ("Mtheta", C.c_int), ("Nphih", C.c_int), ("Mphia", C.c_int),
+                ("Nhparm", C.c_int), ("Nparm", C.c_int), ("Nnb", C.c_int),
+                ("Nres", C.c_int), ("Nbona", C.c_int), ("Ntheta", C.c_int),
+                ("Nphia", C.c_int), ("Numbnd", C.c_int), ("Numang", C.c_int),
+                ("Nptra", C.c_int), ("Natyp", C.c_int), ("Nphb", C.c_int),
+                ("Ntype2d", C.c_int), ("Nttyp", C.c_int), ("AtomNames", C.c_void_p),
+                ("Charges", dp), ("Masses", dp), ("Rk", dp), ("Req", dp), ("Tk", dp),
+                ("Teq", dp), ("Pk", dp), ("Pn", dp), ("Phase", dp), ("Cn1", dp),
+                ("Cn2", dp), ("HB12", dp), ("HB10", dp), ("Rborn", dp), ("Fs", dp),
+                ("Scee", dp), ("Scnb", dp), ("Fsmax", C.c_double), ("Iac", ip),
+                ("Iblo", ip), ("Cno", ip), ("Ipres", ip), ("ExclAt", ip),
+                ("BondHAt1", ip), ("BondHAt2", ip), ("BondHNum", ip), ("BondAt1", ip),
+                ("BondAt2", ip), ("BondNum", ip), ("AngleHAt1", ip), ("AngleHAt2", ip),
+                ("AngleHAt3", ip), ("AngleHNum", ip), ("AngleAt1", ip), ("AngleAt2", ip),
+                ("AngleAt3", ip), ("AngleNum", ip), ("DihHAt1", ip), ("DihHAt2", ip),
+                ("DihHAt3", ip), ("DihHAt4", ip), ("DihHNum", ip), ("DihAt1", ip),
+                ("DihAt2", ip), ("DihAt3", ip), ("DihAt4", ip), ("DihNum", ip),
+                ("AtomicNum", ip), ("N14pairs", ip), ("N14pairlist", ip),
+                ("N14sceelist", dp), ("N14scnblist", dp)]
+
+
+ParmP = C.POINTER(Parm)
+MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("gbhip_last_error", C.c_char_p, [])
+_sig("gbhip_device_count", C.c_int, [])
+_sig("gbhip_ctx_create", C.c_int, [C.POINTER(Topology), C.POINTER(Options), dp, C.POINTER(vp)])
+_sig("gbhip_ctx_destroy", C.c_int, [vp])
+_sig("gbhip_set_coords", C.c_int, [vp, dp])
+_sig("gbhip_nblist", C.c_int, [vp, C.POINTER(C.c_longlong)])
+_sig("gbhip_nblist_async", C.c_int, [vp])
+_sig("gbhip_egb", C.c_int, [vp, dp, dp, dp, dp])
+_sig("gbhip_get_reff", C.c_int, [vp, dp])
+_sig("gbhip_get_sumdeijda", C.c_int, [vp, dp])
+_sig("gbhip_set_stream", C.c_int, [vp, vp])
+_sig("gbhip_set_coords_device", C.c_int, [vp, dp])
+_sig("gbhip_egb_device", C.c_int, [vp, dp, dp])
+_sig("gbhip_sync", C.c_int, [vp])
+_sig("gbhip_npairs", C.c_int, [vp, C.POINTER(C.c_longlong)])
+_sig("gbhip_tile_stats", C.c_int, [vp, ip, ip])
+_sig("gbhip_get_pairs", C.c_int, [vp, ip, ip, C.c_longlong, C.POINTER(C.c_longlong)])
+_sig("gbhip_set_bonded", C.c_int, [vp, vp])
+_sig("gbhip_mme", C.c_int, [vp, dp, dp])
+_sig("gbhip_mme_device", C.c_int, [vp, dp, dp])
+_sig("gbhip_batch_create", C.c_int, [C.c_int, C.POINTER(Topology), C.POINTER(Options),
+                                     C.POINTER(dp), C.POINTER(vp)])
+_sig("gbhip_batch_destroy", C.c_int, [vp])
+_sig("gbhip_batch_set_coords", C.c_int, [vp, C.c_int, dp])
+_sig("gbhip_batch_nblist", C.c_int, [vp])
+_sig("gbhip_batch_eval", C.c_int, [vp, dp, C.POINTER(dp)])
+_sig("gbhip_batch_eval_device", C.c_int, [vp])
+_sig("gbhip_batch_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
+_sig("gbgen_build", C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(ParmP), dp])
+_sig("gbgen_write", C.c_int, [C.c_int, C.c_ulonglong, C.c_char_p, C.c_char_p])
+_sig("rdparm", ParmP, [C.c_char_p])
+_sig("wrparm", C.c_int, [ParmP, C.c_char_p])
+_sig("freeparm", None, [ParmP])
+_sig("getxv", C.c_int, [C.c_char_p, C.c_int, C.c_double, dp, dp])
+_sig("putxv", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_double, dp, dp])
+_sig("mm_options", C.c_int, [C.c_char_p])
+_sig("mm_reset_options", None, [])
+_sig("mm_get_error", C.c_char_p, [])
+_sig("mme_init_sff", C.c_int, [ParmP, ip, ip, dp, C.c_char_p])
+_sig("mme", C.c_double, [dp, dp, ip])
+_sig("mme_get_energies", None, [dp, dp])
+_sig("md", C.c_int, [C.c_int, C.c_int, dp, dp, dp, MMEFUNC])
+_sig("conjgrad", C.c_int, [dp, ip, dp, MMEFUNC, dp, dp, ip])
+_sig("nabc_md_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])
+
+MME_C = MMEFUNC(("mme", lib))
+
+
+class GBHipError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise GBHipError(lib.gbhip_last_error().decode(errors="replace") + f" (code {rc})")
+
+
+def _p(a, t=dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def device_count():
+    return lib.gbhip_device_count()
+
+
+# ---------------------------------------------------------------- prmtop / rst7
+
+def read_parm(path):
+    p = lib.rdparm(path.encode())
+    if not p:
+        raise IOError(lib.mm_get_error().decode(errors="replace"))
+    return p
+
+
+def parm_arrays(p):
+    """Copy the GB-relevant topology arrays of a PARMSTRUCT_T into numpy."""
+    s = p.contents
+    n, nt = s.Natom, s.Ntypes
+    nttyp = nt * (nt + 1) // 2
+    g = lambda ptr, k, dt: np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt).copy()
+    return dict(natom=n, ntypes=nt, charge=g(s.Charges, n, np.float64),
+                rborn=g(s.Rborn, n, np.float64), fs=g(s.Fs, n, np.float64),
+                mass=g(s.Masses, n, np.float64), iac=g(s.Iac, n, np.int32),
+                cno=g(s.Cno, nt * nt, np.int32), cn1=g(s.Cn1, nttyp, np.float64),
+                cn2=g(s.Cn2, nttyp, np.float64), iblo=g(s.Iblo, n, np.int32),
+                exclat=g(s.ExclAt, max(1, s.Nnb), np.int32), fsmax=s.Fsmax,
+                atomic_num=g(s.AtomicNum, n, np.int32))
+
+
+def read_rst7(path, natom):
+    x = np.zeros(3 * natom)
+    v = np.zeros(3 * natom)
+    rc = lib.getxv(path.encode(), natom, 0.0, _p(x), _p(v))
+    if rc != 0:
+        raise IOError(lib.mm_get_error().decode(errors="replace"))
+    return x, v
+
+
+def generate(natom, seed, prmtop_path, rst7_path):
+    rc = lib.gbgen_write(natom, seed, prmtop_path.encode(), rst7_path.encode())
+    if rc != 0:
+        raise RuntimeError(lib.mm_get_error().decode(errors="replace"))
+
+
+# ------------------------------------------------------- GB parameter set-up
+
+GB8 = {1: (0.788440, 0.798699, 0.437334, 1.425952), 6: (0.733756, 0.506378, 0.205844, 1.058554),
+       7: (0.503364, 0.316828, 0.192915, 0.733599), 8: (0.867814, 0.876635, 0.387882, 1.061039),
+       15: (1.0, 0.8, 4.85, 0.5), 16: (0.867814, 0.876635, 0.387882, -0.703469)}
+GB7_FS = {1: 1.09085413633, 6: 0.484353823306, 7: 0.700147318409, 8: 1.06557401132,
+          16: 0.602256336067}
+
+
+def gb_params(gb, rborn, fs, atomic_num):
+    """Per-atom GB parameters as mme_init_sff sets them (sff.c:1187-1333).
+
+    Returns (fs, alpha, beta, gamma, neckidx, gboffset, neckscale)."""
+    n = len(rborn)
+    fs = np.array(fs, dtype=np.float64)
+    a = np.zeros(n); b = np.zeros(n); g = np.zeros(n)
+    gboffset, neckscale = 0.09, 0.0
+    if gb == 2:
+        a[:], b[:], g[:] = 0.8, 0.0, 2.909125
+    elif gb == 5:
+        a[:], b[:], g[:] = 1.0, 0.8, 4.85
+    elif gb == 7:
+        neckscale = 0.361825
+        a[:], b[:], g[:] = 1.09511284, 1.90792938, 2.50798245
+        for i, z in enumerate(atomic_num):
+            fs[i] = GB7_FS.get(int(z), 0.5)
+    elif gb == 8:
+        gboffset, neckscale = 0.195141, 0.826836
+        for i, z in enumerate(atomic_num):
+            a[i], b[i], g[i], fs[i] = GB8.get(int(z), (1.0, 0.8, 4.85, 0.5))
+    neck = ((np.asarray(rborn) - 1.0) * 20.0 + 0.5).astype(np.int32)
+    return fs, a, b, g, neck, gboffset, neckscale
+
+
+def kappa_from_salt(conc_molar):
+    """Debye-Hueckel kappa (1/A) for a 1:1 salt at 300 K: sqrt(0.10806 c).
+
+    Reproduces the values the reference's inputs use for 0.1, 0.15 and 0.2 M
+    (txmin.c:45, benchmarks/p450.nab:8, benchmarks/mb.nab:10)."""
+    return float(np.sqrt(0.10806 * conc_molar))
+
+
+# --------------------------------------------------------------- GB context
+
+class _Keep:
+    """Holds numpy arrays alive while C holds pointers into them."""
+
+    def __init__(self):
+        self.refs = []
+
+    def arr(self, a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        self.refs.append(a)
+        return a
+
+
+def make_topology(keep, top, gb, frozen=None, atomic_num=None):
+    """Build a gbhip_topology from a dict of arrays (see parm_arrays)."""
+    fs, a, b, g, neck, gboffset, neckscale = gb_params(
+        gb, top["rborn"], top["fs"], atomic_num if atomic_num is not None else top.get("atomic_num", np.zeros(top["natom"])))
+    t = Topology()
+    t.natom = top["natom"]
+    t.ntypes = top["ntypes"]
+    t.charge = _p(keep.arr(top["charge"], np.float64))
+    t.rborn = _p(keep.arr(top["rborn"], np.float64))
+    t.fs = _p(keep.arr(fs, np.float64))
+    t.iac = _p(keep.arr(top["iac"], np.int32), ip)
+    t.cno = _p(keep.arr(top["cno"], np.int32), ip)
+    t.cn1 = _p(keep.arr(top["cn1"], np.float64))
+    t.cn2 = _p(keep.arr(top["cn2"], np.float64))
+    t.iblo = _p(keep.arr(top["iblo"], np.int32), ip)
+    t.exclat = _p(keep.arr(top["exclat"], np.int32), ip)
+    t.frozen = _p(keep.arr(frozen, np.int32), ip) if frozen is not None else None
+    t.gbalpha = _p(keep.arr(a, np.float64))
+    t.gbbeta = _p(keep.arr(b, np.float64))
+    t.gbgamma = _p(keep.arr(g, np.float64))
+    t.neckidx = _p(keep.arr(neck, np.int32), ip)
+    return t, gboffset, neckscale
+
+
+def make_options(gb, cut, rgbmax, kappa, epsext, fsmax, gboffset, neckscale, precision=GBHIP_FP64):
+    o = Options()
+    o.gb, o.precision, o.cut, o.rgbmax = gb, precision, cut, rgbmax
+    o.fsmax, o.gboffset, o.gbneckscale = fsmax, gboffset, neckscale
+    o.kappa, o.epsext = kappa, epsext
+    return o
+
+
+class GBContext:
+    """One molecule on the GPU: pair list + egb() evaluation (gbhip.h)."""
+
+    def __init__(self, top, x, gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, epsext=78.5,
+                 precision=GBHIP_FP64, frozen=None, atomic_num=None):
+        self._keep = _Keep()
+        t, gboffset, neckscale = make_topology(self._keep, top, gb, frozen, atomic_num)
+        o = make_options(gb, cut, rgbmax, kappa, epsext, top["fsmax"], gboffset, neckscale, precision)
+        self.natom = top["natom"]
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h = vp()
+        _check(lib.gbhip_ctx_create(C.byref(t), C.byref(o), _p(x), C.byref(h)))
+        self.h = h
+        self.set_coords(x)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gbhip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_coords(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib.gbhip_set_coords(self.h, _p(x)))
+        _check(lib.gbhip_sync(self.h))
+
+    def nblist(self):
+        n = C.c_longlong()
+        _check(lib.gbhip_nblist(self.h, C.byref(n)))
+        return n.value
+
+    def egb(self):
+        """Returns (epol, evdw, eelec, grad[3N])."""
+        g = np.zeros(3 * self.natom)
+        e = [C.c_double() for _ in range(3)]
+        _check(lib.gbhip_egb(self.h, _p(g), *[C.byref(v) for v in e]))
+        return e[0].value, e[1].value, e[2].value, g
+
+    def reff(self):
+        r = np.zeros(self.natom)
+        _check(lib.gbhip_get_reff(self.h, _p(r)))
+        return r
+
+    def sumdeijda(self):
+        r = np.zeros(self.natom)
+        _check(lib.gbhip_get_sumdeijda(self.h, _p(r)))
+        return r
+
+    def tile_stats(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib.gbhip_tile_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def npairs(self):
+        n = C.c_longlong()
+        _check(lib.gbhip_npairs(self.h, C.byref(n)))
+        return n.value
+
+    def pairs(self):
+        """Current list as a sorted (P, 2) array of (min, max) atom indices."""
+        cap = self.npairs()
+        pi = np.zeros(max(cap, 1), np.int32)
+        pj = np.zeros(max(cap, 1), np.int32)
+        n = C.c_longlong()
+        _check(lib.gbhip_get_pairs(self.h, _p(pi, ip), _p(pj, ip), cap, C.byref(n)))
+        assert n.value == cap
+        p = np.stack([np.minimum(pi[:cap], pj[:cap]), np.maximum(pi[:cap], pj[:cap])], axis=1)
+        return p[np.lexsort((p[:, 1], p[:, 0]))]

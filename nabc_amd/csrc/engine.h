@@ -1,0 +1,150 @@
+/*
+ * engine.h -- internal declarations shared by the HIP translation units of
+ * nabc_amd (GB kernels in gbhip.hip, bonded terms and dynamics in
+ * mm_device.hip).  Not part of the public C-ABI.
+ */
+#ifndef NABC_AMD_ENGINE_H
+#define NABC_AMD_ENGINE_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "gbhip.h"
+
+#define TILE 64
+
+int seterr(int code, const char *fmt, ...);
+
+#define HIPCHK(call)                                                                \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return seterr(GBHIP_EHIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK()                                                                    \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess) return seterr(GBHIP_EHIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+/* ------------------------------------------------------------ device data */
+
+struct DevPtrs {
+    /* per slot */
+    const int *perm;      /* slot -> global atom index, -1 = padding */
+    const int *slot_mol;  /* slot -> molecule */
+    const double *q, *rb, *ri0, *sj, *ga, *gbb, *gg;
+    const float *q_f, *ri0_f, *sj_f, *rb_f;
+    const int *neck, *frz, *ljrow, *ljcol;
+    double *xs, *ys, *zs;
+    double *sumi, *reff, *psi, *dfac, *sumdeijda, *gx, *gy, *gz;
+    double *ene;          /* 3 per molecule: epol, evdw, eelec */
+    /* tiles and tile pairs */
+    int ntiles;
+    const int *tile_mol;
+    const int *tile_end;  /* one past the molecule's last tile, per tile */
+    const long long *rowbase;
+    int *cand, *rowcnt, *rowstart;
+    int *ntp;             /* device count of tile pairs */
+    int2 *tp;
+    unsigned long long *mask, *excl;
+    unsigned long long *npairs;
+    double *bblo, *bbhi;  /* per tile bounding box */
+    /* exclusions, CSR over global atoms, partners j > i */
+    const int *ex_ptr, *ex_j, *inv;
+    int natoms;
+    /* LJ dense tables, concatenated per molecule */
+    const double *lja, *ljb;
+    const float *lja_f, *ljb_f;
+    const double *neck_pos, *neck_val;
+    const float *neck_pos_f, *neck_val_f;
+};
+
+struct GBPar {
+    int gb, has_neck, nslots, nmol;
+    double cut2, rgbmax, rgbmax1i, rgbmax2i, rgbmaxpsmax2;
+    double neckscale, kappa, epsext;
+};
+
+struct Engine {
+    int nmol = 0, natoms = 0, nslots = 0, ntiles = 0;
+    long long tp_cap = 0;
+    int precision = GBHIP_FP64;
+    int pair_blocks = 0;
+    GBPar par{};
+    DevPtrs d{};
+    hipStream_t stream = nullptr;
+    std::vector<int> mol_atom_off, mol_natom, mol_slot_off;
+    std::vector<int> perm_h;  /* slot -> global atom */
+    std::vector<int> frz_h;   /* slot -> frozen flag */
+    double *ene14_dev = nullptr; /* mme energies + rms gradient (14 doubles) */
+    std::vector<void *> allocs;
+    double *x_dev = nullptr;   /* concatenated coordinates, AoS */
+    double *g_dev = nullptr;   /* concatenated gradient scratch, AoS */
+    double *tmp_dev = nullptr; /* natoms doubles */
+    bool have_list = false;
+    bool have_eval = false;
+
+    ~Engine()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+
+    template <typename T> int dalloc(T **p, size_t n)
+    {
+        void *q = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&q, n * sizeof(T)) != hipSuccess)
+            return seterr(GBHIP_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+        allocs.push_back(q);
+        *p = (T *)q;
+        return 0;
+    }
+
+    template <typename T> int upload(const T **dst, const std::vector<T> &src)
+    {
+        T *p;
+        int rc = dalloc(&p, src.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+        *dst = p;
+        return 0;
+    }
+};
+
+
+/* bonded and 1-4 terms of one molecule, original atom order (mm_device.hip) */
+struct BondedDev {
+    bool set = false;
+    int nb = 0, na = 0, nd = 0, n14 = 0, dield = 1;
+    const int *b_i = nullptr, *b_j = nullptr;
+    const double *b_rk = nullptr, *b_req = nullptr;
+    const int *a_i = nullptr, *a_j = nullptr, *a_k = nullptr;
+    const double *a_tk = nullptr, *a_teq = nullptr;
+    const int *d_i = nullptr, *d_j = nullptr, *d_k = nullptr, *d_l = nullptr;
+    const double *d_pk = nullptr, *d_pn = nullptr, *d_phase = nullptr;
+    const int *p_i = nullptr, *p_j = nullptr, *p_hb = nullptr;
+    const double *p_scee = nullptr, *p_scnb = nullptr, *p_a = nullptr, *p_b = nullptr;
+    const double *q = nullptr;
+    const int *frozen = nullptr;  /* original order, always allocated */
+    double *ene = nullptr;        /* 13 doubles, mme34 layout */
+};
+
+struct gbhip_ctx {
+    Engine E;
+    BondedDev B;
+};
+
+struct gbhip_batch {
+    Engine E;
+};
+
+int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_options *o,
+                 const double *const *x0);
+int engine_gather(Engine &E, const double *x_dev);
+int engine_nblist(Engine &E);
+int engine_eval(Engine &E, const double *x_dev, double *g_dev);
+
+#endif

@@ -1,0 +1,1192 @@
+/*
+ * gbhip.hip -- MI355X (gfx950) generalized-Born energy + gradient engine.
+ *
+ * One engine serves a single molecule (gbhip_ctx) or a batch (gbhip_batch):
+ * molecules are concatenated, each padded to whole 64-atom tiles, so every
+ * kernel below runs once per pass for the whole set.
+ *
+ * Layout in HBM (per atom "slot" = position in the spatially sorted order):
+ *   SoA FP64 coordinates xs/ys/zs, charge, intrinsic radius rb, offset
+ *   radius ri0 = rb - gboffset, scaled radius sj = fs*ri0, OBC alpha/beta/
+ *   gamma, neck index, LJ row/column into a dense per-molecule table.
+ *   Pair list = tile pairs (I <= J, same molecule) with an exact 64x64
+ *   membership bit matrix (r^2 < cut^2 at build time, not both frozen, one
+ *   bit per unordered pair) and a parallel exclusion bit matrix.
+ *
+ * Per evaluation (egb(), eff.c:1363-2417):
+ *   k_gather   sort coordinates into slots, clear accumulators
+ *   k_born     pass 1: descreening sums, both directions per pair
+ *   k_radii    reff/psi (HCT or OBC/neck tanh rescale, FP64), the diagonal
+ *              GB self term, and the chain-rule factor dR/dsum per atom
+ *   k_pairs    pass 2: GB + Coulomb + LJ energies, sumdeijda, direct forces
+ *   k_chain    pass 3: sumdeijda * dR/dr chain-rule forces; the per-atom
+ *              OBC factor is applied as the atoms are loaded, so the
+ *              separate sumdeijda reduction of eff.c:2077-2129 disappears
+ *   k_scatter  add the slot gradients into the caller's AoS array
+ * Pair kernels: one 64-lane wavefront per tile pair; lane l holds i-atom l,
+ * the j-tile is staged in LDS and visited in the rotated order
+ * j = (l + s) mod 64 so that every lane touches a different j each step;
+ * j-side sums are accumulated in LDS and flushed with one FP64 atomic per
+ * atom per tile pair.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "engine.h"
+#include "gb_math.h"
+#include "neck_tables.h"
+
+#define PAIR_BLOCKS_MAX 4096
+
+/* ------------------------------------------------------------------ errors */
+
+static thread_local char g_err[512] = "";
+
+int seterr(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *gbhip_last_error(void) { return g_err; }
+
+extern "C" int gbhip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+/* ---------------------------------------------------------- pair-list build */
+
+__global__ __launch_bounds__(64) void k_bbox(DevPtrs d)
+{
+    const int T = blockIdx.x, l = threadIdx.x;
+    const int s = T * TILE + l;
+    const bool ok = d.perm[s] >= 0;
+    double lo[3], hi[3];
+    lo[0] = ok ? d.xs[s] : 1e300; hi[0] = ok ? d.xs[s] : -1e300;
+    lo[1] = ok ? d.ys[s] : 1e300; hi[1] = ok ? d.ys[s] : -1e300;
+    lo[2] = ok ? d.zs[s] : 1e300; hi[2] = ok ? d.zs[s] : -1e300;
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], off));
+        }
+    }
+    if (l == 0) {
+        for (int k = 0; k < 3; k++) {
+            d.bblo[3 * T + k] = lo[k];
+            d.bbhi[3 * T + k] = hi[k];
+        }
+    }
+}
+
+/* candidates of tile row I: tiles J >= I of the same molecule whose boxes
+ * are closer than the cutoff; written in increasing J */
+__global__ __launch_bounds__(256) void k_tilecand(DevPtrs d, GBPar p)
+{
+    const int I = blockIdx.x;
+    const int jend = d.tile_end[I];
+    __shared__ int s_cnt;
+    __shared__ int s_warp[4];
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    double lo0 = d.bblo[3 * I], lo1 = d.bblo[3 * I + 1], lo2 = d.bblo[3 * I + 2];
+    double hi0 = d.bbhi[3 * I], hi1 = d.bbhi[3 * I + 1], hi2 = d.bbhi[3 * I + 2];
+    int *row = d.cand + d.rowbase[I];
+    for (int base = I; base < jend; base += 256) {
+        int J = base + threadIdx.x;
+        bool take = false;
+        if (J < jend) {
+            double a = fmax(0.0, fmax(d.bblo[3 * J] - hi0, lo0 - d.bbhi[3 * J]));
+            double b = fmax(0.0, fmax(d.bblo[3 * J + 1] - hi1, lo1 - d.bbhi[3 * J + 1]));
+            double c = fmax(0.0, fmax(d.bblo[3 * J + 2] - hi2, lo2 - d.bbhi[3 * J + 2]));
+            take = a * a + b * b + c * c < p.cut2;
+        }
+        unsigned long long bal = __ballot(take);
+        int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+        if (ln == 0) s_warp[wv] = __popcll(bal);
+        __syncthreads();
+        int off = s_cnt;
+        for (int w = 0; w < wv; w++) off += s_warp[w];
+        off += __popcll(bal & ((1ull << ln) - 1ull));
+        if (take) row[off] = J;
+        __syncthreads();
+        if (threadIdx.x == 0) s_cnt += s_warp[0] + s_warp[1] + s_warp[2] + s_warp[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.rowcnt[I] = s_cnt;
+}
+
+/* exclusive scan of the row counts (single block) and total */
+__global__ __launch_bounds__(1024) void k_rowscan(DevPtrs d)
+{
+    __shared__ int s_part[1024];
+    const int n = d.ntiles, t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    int sum = 0;
+    for (int k = b; k < e; k++) sum += d.rowcnt[k];
+    s_part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    int run = s_part[t] - sum;
+    for (int k = b; k < e; k++) {
+        d.rowstart[k] = run;
+        run += d.rowcnt[k];
+    }
+    if (t == 1023) {
+        *d.ntp = s_part[1023];
+        *d.npairs = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(DevPtrs d)
+{
+    const int I = blockIdx.x;
+    const int n = d.rowcnt[I], o = d.rowstart[I];
+    const int *row = d.cand + d.rowbase[I];
+    for (int k = threadIdx.x; k < n; k += 256) d.tp[o + k] = make_int2(I, row[k]);
+}
+
+/* exact membership bits: r^2 < cut^2 with the reference's unfused FP64
+ * arithmetic (nblist.c:748-765), i != j, not both frozen */
+__global__ __launch_bounds__(64) void k_masks(DevPtrs d, GBPar p)
+{
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+    __shared__ int s_ok[TILE], s_frz[TILE];
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    unsigned long long cnt = 0;
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const int2 ij = d.tp[t];
+        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const bool oki = d.perm[i] >= 0;
+        const int fi = d.frz[i];
+        __syncthreads();
+        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
+        s_ok[l] = d.perm[j] >= 0;
+        s_frz[l] = d.frz[j];
+        __syncthreads();
+        unsigned long long w = 0;
+        if (oki) {
+            const int b0 = ij.x == ij.y ? l + 1 : 0;
+            for (int b = b0; b < TILE; b++) {
+                double dx = __dsub_rn(xi, s_x[b]);
+                double dy = __dsub_rn(yi, s_y[b]);
+                double dz = __dsub_rn(zi, s_z[b]);
+                double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                bool in = s_ok[b] && !(fi && s_frz[b]) && r2 < p.cut2;
+                w |= (unsigned long long)in << b;
+            }
+        }
+        d.mask[(size_t)t * TILE + l] = w;
+        d.excl[(size_t)t * TILE + l] = 0ull;
+        cnt += __popcll(w);
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (l == 0 && cnt) atomicAdd(d.npairs, cnt);
+}
+
+/* exclusion bits from the per-atom lists (only partners j > i count, as in
+ * the upper-triangle loop of eff.c:1931-2025) */
+__global__ __launch_bounds__(256) void k_excl(DevPtrs d)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= d.natoms) return;
+    const int sa = d.inv[a];
+    for (int k = d.ex_ptr[a]; k < d.ex_ptr[a + 1]; k++) {
+        const int sb = d.inv[d.ex_j[k]];
+        int s1 = sa, s2 = sb;
+        int T1 = s1 / TILE, T2 = s2 / TILE;
+        if (T1 > T2 || (T1 == T2 && (s1 & 63) > (s2 & 63))) {
+            int t = s1; s1 = s2; s2 = t;
+            t = T1; T1 = T2; T2 = t;
+        }
+        /* binary search for tile T2 in row T1 (sorted by J) */
+        int lo = d.rowstart[T1], hi = lo + d.rowcnt[T1] - 1, found = -1;
+        while (lo <= hi) {
+            int mid = (lo + hi) >> 1;
+            int J = d.tp[mid].y;
+            if (J == T2) { found = mid; break; }
+            if (J < T2) lo = mid + 1; else hi = mid - 1;
+        }
+        if (found >= 0)
+            atomicOr(&d.excl[(size_t)found * TILE + (s1 & 63)], 1ull << (s2 & 63));
+    }
+}
+
+/* ------------------------------------------------------------ evaluation */
+
+__global__ __launch_bounds__(256) void k_gather(DevPtrs d, GBPar p, const double *__restrict__ x)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s < 3 * p.nmol) d.ene[s] = 0.0;
+    if (s >= p.nslots) return;
+    const int a = d.perm[s];
+    if (a >= 0) {
+        d.xs[s] = x[3 * (size_t)a];
+        d.ys[s] = x[3 * (size_t)a + 1];
+        d.zs[s] = x[3 * (size_t)a + 2];
+    } else {
+        d.xs[s] = 0.0; d.ys[s] = 0.0; d.zs[s] = 0.0;
+    }
+    d.sumi[s] = 0.0;
+    d.gx[s] = 0.0; d.gy[s] = 0.0; d.gz[s] = 0.0;
+}
+
+template <typename R> __device__ __forceinline__ R ldq(const DevPtrs &d, int s);
+template <> __device__ __forceinline__ double ldq<double>(const DevPtrs &d, int s) { return d.q[s]; }
+template <> __device__ __forceinline__ float ldq<float>(const DevPtrs &d, int s) { return d.q_f[s]; }
+template <typename R> __device__ __forceinline__ R ldri0(const DevPtrs &d, int s);
+template <> __device__ __forceinline__ double ldri0<double>(const DevPtrs &d, int s) { return d.ri0[s]; }
+template <> __device__ __forceinline__ float ldri0<float>(const DevPtrs &d, int s) { return d.ri0_f[s]; }
+template <typename R> __device__ __forceinline__ R ldsj(const DevPtrs &d, int s);
+template <> __device__ __forceinline__ double ldsj<double>(const DevPtrs &d, int s) { return d.sj[s]; }
+template <> __device__ __forceinline__ float ldsj<float>(const DevPtrs &d, int s) { return d.sj_f[s]; }
+template <typename R> __device__ __forceinline__ R ldrb(const DevPtrs &d, int s);
+template <> __device__ __forceinline__ double ldrb<double>(const DevPtrs &d, int s) { return d.rb[s]; }
+template <> __device__ __forceinline__ float ldrb<float>(const DevPtrs &d, int s) { return d.rb_f[s]; }
+template <typename R> __device__ __forceinline__ const R *neckpos(const DevPtrs &d);
+template <> __device__ __forceinline__ const double *neckpos<double>(const DevPtrs &d) { return d.neck_pos; }
+template <> __device__ __forceinline__ const float *neckpos<float>(const DevPtrs &d) { return d.neck_pos_f; }
+template <typename R> __device__ __forceinline__ const R *neckval(const DevPtrs &d);
+template <> __device__ __forceinline__ const double *neckval<double>(const DevPtrs &d) { return d.neck_val; }
+template <> __device__ __forceinline__ const float *neckval<float>(const DevPtrs &d) { return d.neck_val_f; }
+template <typename R> __device__ __forceinline__ const R *lja(const DevPtrs &d);
+template <> __device__ __forceinline__ const double *lja<double>(const DevPtrs &d) { return d.lja; }
+template <> __device__ __forceinline__ const float *lja<float>(const DevPtrs &d) { return d.lja_f; }
+template <typename R> __device__ __forceinline__ const R *ljb(const DevPtrs &d);
+template <> __device__ __forceinline__ const double *ljb<double>(const DevPtrs &d) { return d.ljb; }
+template <> __device__ __forceinline__ const float *ljb<float>(const DevPtrs &d) { return d.ljb_f; }
+
+/*
+ * Pass 1 (eff.c:1566-1733): for every listed pair, the descreening of i by
+ * j (added to i's sum) and of j by i (added to j's sum).  The r^2 test
+ * against (rgbmax+Fsmax)^2 and the FP64 sums are kept in both variants.
+ */
+template <typename R>
+__global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
+{
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
+    __shared__ R s_ri[TILE], s_sj[TILE], s_rb[TILE];
+    __shared__ int s_nk[TILE];
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
+    const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
+    const R *npos = neckpos<R>(d), *nval = neckval<R>(d);
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const int2 ij = d.tp[t];
+        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
+        const R ri1i = R(1) / ri;
+        const int ni = d.neck[i];
+        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        __syncthreads();
+        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
+        s_ri[l] = ldri0<R>(d, j); s_sj[l] = ldsj<R>(d, j); s_rb[l] = ldrb<R>(d, j);
+        s_nk[l] = d.neck[j];
+        s_acc[l] = 0.0;
+        __syncthreads();
+        double acc = 0.0;
+        for (int s = 0; s < TILE; s++) {
+            const int b = (l + s) & (TILE - 1);
+            if (!((m >> b) & 1ull)) continue;
+            const double dx = xi - s_x[b], dy = yi - s_y[b], dz = zi - s_z[b];
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            if (r2d > p.rgbmaxpsmax2) continue;
+            const R r2 = (R)r2d;
+            const R dij1i = gbm<R>::rsqrt_(r2);
+            const R dij = r2 * dij1i;
+            const R rj = s_ri[b], sj = s_sj[b];
+            bool skip_ij, skip_ji;
+            R tij = descreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij);
+            R tji = descreen<R>(rj, R(1) / rj, si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji);
+            if (p.has_neck) {
+                const int nj = s_nk[b];
+                if (dij < rbi + s_rb[b] + neckcut) {
+                    if (!skip_ij) tij += neck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
+                    if (!skip_ji) tji += neck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                }
+            }
+            acc -= (double)tij;
+            s_acc[b] -= (double)tji;
+        }
+        __syncthreads();
+        if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
+        if (s_acc[l] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[l]);
+    }
+}
+
+/*
+ * Per atom (FP64): effective radius (eff.c:1712-1727), the diagonal GB term
+ * and its sumdeijda start value (eff.c:1884-1893), and the factor that turns
+ * sumdeijda into the pass-3 weight (eff.c:2212-2231; 0 for frozen atoms).
+ */
+__global__ __launch_bounds__(64) void k_radii(DevPtrs d, GBPar p)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool ok = s < p.nslots && d.perm[s] >= 0;
+    double eself = 0.0;
+    if (s < p.nslots) {
+        double reff = 1.0, psi = 0.0, dfac = 0.0, sd = 0.0;
+        if (ok) {
+            const double ri = d.ri0[s], rb = d.rb[s];
+            const double ri1i = 1.0 / ri;
+            const double sumi = d.sumi[s];
+            if (p.gb == 1) {
+                reff = 1.0 / (ri1i + sumi);
+                if (reff < 0.0) reff = 30.0;
+                dfac = 1.0;
+            } else {
+                const double a = d.ga[s], b = d.gbb[s], g = d.gg[s];
+                psi = -ri * sumi;
+                const double th = tanh((a - b * psi + g * psi * psi) * psi);
+                reff = 1.0 / (ri1i - th / rb);
+                dfac = (a - 2.0 * b * psi + 3.0 * g * psi * psi) * (1.0 - th * th) * ri / rb;
+            }
+            if (!d.frz[s]) {
+                const double qi = d.q[s];
+                const double expmkf = exp(-GB_KSCALE * p.kappa * reff) / p.epsext;
+                const double dielfac = 1.0 - expmkf;
+                const double qi2h = 0.5 * qi * qi;
+                const double qid2h = qi2h * dielfac;
+                eself = -qid2h / reff;
+                sd = qid2h - GB_KSCALE * p.kappa * qi2h * expmkf * reff;
+            } else {
+                dfac = 0.0;
+            }
+        }
+        d.reff[s] = reff;
+        d.psi[s] = psi;
+        d.dfac[s] = dfac;
+        d.sumdeijda[s] = sd;
+    }
+    /* a wave is one tile, so one molecule */
+    for (int off = 32; off > 0; off >>= 1) eself += __shfl_xor(eself, off);
+    if (threadIdx.x == 0 && eself != 0.0) unsafeAtomicAdd(&d.ene[3 * d.slot_mol[s]], eself);
+}
+
+/*
+ * Pass 2 (eff.c:1768-2066): GB pair energy with f = sqrt(r^2 + RiRj
+ * exp(-r^2/4RiRj)) and Debye-Hueckel screening, its sumdeijda contributions
+ * to both atoms, and for non-excluded pairs Coulomb and 12-6 LJ; the direct
+ * pair force de*xij goes to i (+) and j (-).
+ */
+template <typename R>
+__global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
+{
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+    __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE], s_sd[TILE];
+    __shared__ R s_q[TILE], s_r[TILE];
+    __shared__ int s_lj[TILE];
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    const R epsinv = (R)(1.0 / p.epsext);
+    const R kks = (R)(-GB_KSCALE * p.kappa);
+    const bool screened = p.kappa != 0.0;
+    const R *A = lja<R>(d), *B = ljb<R>(d);
+    double epol = 0.0, eel = 0.0, evdw = 0.0;
+    int cur_mol = -1;
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const int2 ij = d.tp[t];
+        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+        const int mol = d.slot_mol[i];
+        if (mol != cur_mol) {
+            if (cur_mol >= 0) {
+                for (int off = 32; off > 0; off >>= 1) {
+                    epol += __shfl_xor(epol, off);
+                    eel += __shfl_xor(eel, off);
+                    evdw += __shfl_xor(evdw, off);
+                }
+                if (l == 0) {
+                    unsafeAtomicAdd(&d.ene[3 * cur_mol], epol);
+                    unsafeAtomicAdd(&d.ene[3 * cur_mol + 1], evdw);
+                    unsafeAtomicAdd(&d.ene[3 * cur_mol + 2], eel);
+                }
+                epol = eel = evdw = 0.0;
+            }
+            cur_mol = mol;
+        }
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const R qi = ldq<R>(d, i), ri = (R)d.reff[i];
+        const int row = d.ljrow[i];
+        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        const unsigned long long e = d.excl[(size_t)t * TILE + l];
+        __syncthreads();
+        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
+        s_q[l] = ldq<R>(d, j); s_r[l] = (R)d.reff[j];
+        s_lj[l] = d.ljcol[j];
+        s_fx[l] = 0.0; s_fy[l] = 0.0; s_fz[l] = 0.0; s_sd[l] = 0.0;
+        __syncthreads();
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0, sdi = 0.0;
+        for (int s = 0; s < TILE; s++) {
+            const int b = (l + s) & (TILE - 1);
+            if (!((m >> b) & 1ull)) continue;
+            const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
+            const R r2 = (R)(dxd * dxd + dyd * dyd + dzd * dzd);
+            const R qiqj = qi * s_q[b];
+            const R rj = s_r[b];
+            const R rb2 = ri * rj;
+            const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
+            const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
+            R expmkf, dielfac, fgbk;
+            if (screened) {
+                fgbk = kks * gbm<R>::rcp(fgbi);
+                expmkf = gbm<R>::exp_(fgbk) * epsinv;
+            } else {
+                fgbk = R(0);
+                expmkf = epsinv;
+            }
+            dielfac = R(1) - expmkf;
+            epol -= (double)(qiqj * dielfac * fgbi);
+            const R temp4 = fgbi * fgbi * fgbi;
+            const R temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
+            R de = temp6 * (R(1) - R(0.25) * efac);
+            const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
+            sdi += (double)(ri * temp5);
+            s_sd[b] += (double)(rj * temp5);
+            if (!((e >> b) & 1ull)) {
+                const R rinv = gbm<R>::rsqrt_(r2);
+                const R r2inv = rinv * rinv;
+                const R el = qiqj * rinv;
+                eel += (double)el;
+                de -= el * r2inv;
+                const R r6inv = r2inv * r2inv * r2inv;
+                const int k = row + s_lj[b];
+                const R f6 = B[k] * r6inv;
+                const R f12 = A[k] * r6inv * r6inv;
+                evdw += (double)(f12 - f6);
+                de -= (R(12) * f12 - R(6) * f6) * r2inv;
+            }
+            const double ded = (double)de;
+            const double gxv = ded * dxd, gyv = ded * dyd, gzv = ded * dzd;
+            fxi += gxv; fyi += gyv; fzi += gzv;
+            s_fx[b] -= gxv; s_fy[b] -= gyv; s_fz[b] -= gzv;
+        }
+        __syncthreads();
+        if (m) {
+            unsafeAtomicAdd(&d.gx[i], fxi);
+            unsafeAtomicAdd(&d.gy[i], fyi);
+            unsafeAtomicAdd(&d.gz[i], fzi);
+            unsafeAtomicAdd(&d.sumdeijda[i], sdi);
+        }
+        if (s_sd[l] != 0.0 || s_fx[l] != 0.0) {
+            unsafeAtomicAdd(&d.gx[j], s_fx[l]);
+            unsafeAtomicAdd(&d.gy[j], s_fy[l]);
+            unsafeAtomicAdd(&d.gz[j], s_fz[l]);
+            unsafeAtomicAdd(&d.sumdeijda[j], s_sd[l]);
+        }
+    }
+    if (cur_mol >= 0) {
+        for (int off = 32; off > 0; off >>= 1) {
+            epol += __shfl_xor(epol, off);
+            eel += __shfl_xor(eel, off);
+            evdw += __shfl_xor(evdw, off);
+        }
+        if (l == 0) {
+            unsafeAtomicAdd(&d.ene[3 * cur_mol], epol);
+            unsafeAtomicAdd(&d.ene[3 * cur_mol + 1], evdw);
+            unsafeAtomicAdd(&d.ene[3 * cur_mol + 2], eel);
+        }
+    }
+}
+
+/*
+ * Pass 3 (eff.c:2131-2405): chain rule through dR/dr.  For the pair (i,j)
+ * the reference adds, from row i, -sumda_i*datmp_ij*xij to i and the
+ * opposite to j, and from row j the same with i and j swapped; both rows
+ * are done here in one visit: F = xij*(datmp_ij*w_i + datmp_ji*w_j),
+ * grad_i -= F, grad_j += F, with w = sumdeijda * dR/dsum (0 if frozen).
+ */
+template <typename R>
+__global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
+{
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+    __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
+    __shared__ R s_ri[TILE], s_sj[TILE], s_rb[TILE], s_w[TILE];
+    __shared__ int s_nk[TILE];
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    const R rgbmax = (R)p.rgbmax, rgbmax2i = (R)p.rgbmax2i;
+    const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
+    const R *npos = neckpos<R>(d), *nval = neckval<R>(d);
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const int2 ij = d.tp[t];
+        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
+        const R ri1i = R(1) / ri;
+        const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
+        const int ni = d.neck[i];
+        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        __syncthreads();
+        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
+        s_ri[l] = ldri0<R>(d, j); s_sj[l] = ldsj<R>(d, j); s_rb[l] = ldrb<R>(d, j);
+        s_w[l] = (R)(d.sumdeijda[j] * d.dfac[j]);
+        s_nk[l] = d.neck[j];
+        s_fx[l] = 0.0; s_fy[l] = 0.0; s_fz[l] = 0.0;
+        __syncthreads();
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        for (int s = 0; s < TILE; s++) {
+            const int b = (l + s) & (TILE - 1);
+            if (!((m >> b) & 1ull)) continue;
+            const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            if (r2d > p.rgbmaxpsmax2) continue;
+            const R r2 = (R)r2d;
+            const R dij1i = gbm<R>::rsqrt_(r2);
+            const R dij = r2 * dij1i;
+            const R rj = s_ri[b], sj = s_sj[b];
+            bool skip_ij, skip_ji;
+            R aij = ddescreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax2i, skip_ij);
+            R aji = ddescreen<R>(rj, R(1) / rj, si, r2, dij, dij1i, rgbmax, rgbmax2i, skip_ji);
+            if (p.has_neck) {
+                const int nj = s_nk[b];
+                if (dij < rbi + s_rb[b] + neckcut) {
+                    if (!skip_ij) aij += dneck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
+                    if (!skip_ji) aji += dneck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                }
+            }
+            const double f = (double)(aij * wi + aji * s_w[b]);
+            const double gxv = f * dxd, gyv = f * dyd, gzv = f * dzd;
+            fxi -= gxv; fyi -= gyv; fzi -= gzv;
+            s_fx[b] += gxv; s_fy[b] += gyv; s_fz[b] += gzv;
+        }
+        __syncthreads();
+        if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
+            unsafeAtomicAdd(&d.gx[i], fxi);
+            unsafeAtomicAdd(&d.gy[i], fyi);
+            unsafeAtomicAdd(&d.gz[i], fzi);
+        }
+        if (s_fx[l] != 0.0 || s_fy[l] != 0.0 || s_fz[l] != 0.0) {
+            unsafeAtomicAdd(&d.gx[j], s_fx[l]);
+            unsafeAtomicAdd(&d.gy[j], s_fy[l]);
+            unsafeAtomicAdd(&d.gz[j], s_fz[l]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(DevPtrs d, GBPar p, double *__restrict__ g)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= p.nslots) return;
+    const int a = d.perm[s];
+    if (a < 0) return;
+    g[3 * (size_t)a] += d.gx[s];
+    g[3 * (size_t)a + 1] += d.gy[s];
+    g[3 * (size_t)a + 2] += d.gz[s];
+}
+
+/* expand the membership bits into (i, j) original-index pairs (read-back) */
+__global__ __launch_bounds__(64) void k_listpairs(DevPtrs d, int *pi, int *pj, unsigned long long *cnt,
+                                                  long long cap)
+{
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const int2 ij = d.tp[t];
+        unsigned long long w = d.mask[(size_t)t * TILE + l];
+        const int a = d.perm[ij.x * TILE + l];
+        while (w) {
+            const int b = __ffsll((unsigned long long)w) - 1;
+            w &= w - 1ull;
+            const unsigned long long k = atomicAdd(cnt, 1ull);
+            if ((long long)k < cap) {
+                pi[k] = a;
+                pj[k] = d.perm[ij.y * TILE + b];
+            }
+        }
+    }
+}
+
+/* per-atom values in original order (for reff / sumdeijda read-back) */
+__global__ __launch_bounds__(256) void k_unsort(DevPtrs d, GBPar p, const double *src, double *dst)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= p.nslots) return;
+    const int a = d.perm[s];
+    if (a >= 0) dst[a] = src[s];
+}
+
+/* ------------------------------------------------------------------ engine */
+
+static unsigned int spread_bits(unsigned int v)
+{
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+/* Morton (Z-order) ordering of one molecule's atoms */
+static void morton_order(int n, const double *x, std::vector<int> &order)
+{
+    order.resize(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    if (!x || n < 2) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], x[3 * i + k]);
+            hi[k] = std::max(hi[k], x[3 * i + k]);
+        }
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) return;
+    std::vector<unsigned long long> key(n);
+    for (int i = 0; i < n; i++) {
+        unsigned int c[3];
+        for (int k = 0; k < 3; k++) {
+            double f = (x[3 * i + k] - lo[k]) / ext * 1023.999;
+            c[k] = (unsigned int)std::max(0.0, std::min(1023.0, f));
+        }
+        unsigned long long code = spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2);
+        key[i] = (code << 32) | (unsigned long long)i;
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < n; i++) order[i] = (int)(key[i] & 0xffffffffu);
+}
+
+static int check_options(const gbhip_options *o)
+{
+    if (!o) return seterr(GBHIP_EINVAL, "options pointer is NULL");
+    if (o->gb != 1 && o->gb != 2 && o->gb != 5 && o->gb != 7 && o->gb != 8)
+        return seterr(GBHIP_EINVAL, "gb=%d is not supported (1, 2, 5, 7, 8)", o->gb);
+    if (o->precision != GBHIP_FP64 && o->precision != GBHIP_MIXED)
+        return seterr(GBHIP_EINVAL, "unknown precision %d", o->precision);
+    if (o->rgbmax > o->cut)
+        return seterr(GBHIP_EINVAL, "Error in egb: rgbmax = %f is greater than cutoff = %f", o->rgbmax, o->cut);
+    if (!(o->cut > 0) || !(o->rgbmax > 0) || !(o->epsext > 0))
+        return seterr(GBHIP_EINVAL, "cut, rgbmax and epsext must be positive");
+    return 0;
+}
+
+static int check_topology(const gbhip_topology *t, int gb)
+{
+    if (!t || t->natom < 1) return seterr(GBHIP_EINVAL, "empty topology");
+    if (!t->charge || !t->rborn || !t->fs || !t->iac || !t->cno || !t->cn1 || !t->cn2 || !t->iblo || !t->exclat)
+        return seterr(GBHIP_EINVAL, "topology is missing a required array");
+    if (gb != 1 && (!t->gbalpha || !t->gbbeta || !t->gbgamma))
+        return seterr(GBHIP_EINVAL, "gb=%d needs gbalpha/gbbeta/gbgamma", gb);
+    if ((gb == 7 || gb == 8) && !t->neckidx) return seterr(GBHIP_EINVAL, "gb=%d needs neckidx", gb);
+    if (t->ntypes < 1) return seterr(GBHIP_EINVAL, "ntypes must be >= 1");
+    for (int i = 0; i < t->natom; i++) {
+        if (t->iac[i] < 1 || t->iac[i] > t->ntypes)
+            return seterr(GBHIP_EINVAL, "atom %d has LJ type %d outside 1..%d", i + 1, t->iac[i], t->ntypes);
+        if ((gb == 7 || gb == 8) && (t->neckidx[i] < 0 || t->neckidx[i] > 20))
+            return seterr(GBHIP_EINVAL, "Atom %d outside the allowed range of 1-2 Angstroms for igb==%d", i, gb);
+    }
+    return 0;
+}
+
+int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_options *o,
+                        const double *const *x0)
+{
+    int rc = check_options(o);
+    if (rc) return rc;
+    for (int m = 0; m < nmol; m++)
+        if ((rc = check_topology(&tops[m], o->gb))) return rc;
+    if (gbhip_device_count() < 1) return seterr(GBHIP_ENODEV, "no GPU is visible to the HIP runtime");
+
+    E.nmol = nmol;
+    E.precision = o->precision;
+    E.mol_atom_off.resize(nmol + 1);
+    E.mol_natom.resize(nmol);
+    E.mol_slot_off.resize(nmol + 1);
+    long long atoms = 0, slots = 0;
+    for (int m = 0; m < nmol; m++) {
+        E.mol_atom_off[m] = (int)atoms;
+        E.mol_slot_off[m] = (int)slots;
+        E.mol_natom[m] = tops[m].natom;
+        atoms += tops[m].natom;
+        slots += (long long)(tops[m].natom + TILE - 1) / TILE * TILE;
+    }
+    E.mol_atom_off[nmol] = (int)atoms;
+    E.mol_slot_off[nmol] = (int)slots;
+    if (slots > 0x7fffffffLL / 4) return seterr(GBHIP_EINVAL, "system too large");
+    E.natoms = (int)atoms;
+    E.nslots = (int)slots;
+    E.ntiles = E.nslots / TILE;
+
+    const int NS = E.nslots, NT = E.ntiles;
+    std::vector<int> perm(NS, -1), inv(E.natoms), slot_mol(NS), frz(NS, 1), neck(NS, 0), ljrow(NS, 0), ljcol(NS, 0);
+    std::vector<double> q(NS, 0.0), rb(NS, 1.0), ri0(NS, 1.0), sj(NS, 0.0), ga(NS, 0.0), gbb(NS, 0.0), gg(NS, 0.0);
+    std::vector<int> tile_mol(NT), tile_end(NT);
+    std::vector<long long> rowbase(NT);
+    std::vector<double> lja, ljb;
+    std::vector<int> ex_ptr(E.natoms + 1, 0), ex_j;
+    long long cap = 0;
+
+    for (int m = 0; m < nmol; m++) {
+        const gbhip_topology &t = tops[m];
+        const int n = t.natom, so = E.mol_slot_off[m], ao = E.mol_atom_off[m];
+        std::vector<int> order;
+        morton_order(n, x0 ? x0[m] : nullptr, order);
+        const int lj0 = (int)lja.size(), nt = t.ntypes;
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                int ic = t.cno[a * nt + b] - 1;
+                lja.push_back(ic >= 0 ? t.cn1[ic] : 0.0);
+                ljb.push_back(ic >= 0 ? t.cn2[ic] : 0.0);
+            }
+        for (int k = 0; k < n; k++) {
+            const int a = order[k], s = so + k;
+            perm[s] = ao + a;
+            inv[ao + a] = s;
+            q[s] = t.charge[a];
+            rb[s] = t.rborn[a];
+            ri0[s] = t.rborn[a] - o->gboffset;
+            sj[s] = t.fs[a] * (t.rborn[a] - o->gboffset);
+            if (o->gb != 1) {
+                ga[s] = t.gbalpha[a];
+                gbb[s] = t.gbbeta[a];
+                gg[s] = t.gbgamma[a];
+            }
+            neck[s] = (o->gb == 7 || o->gb == 8) ? t.neckidx[a] : 0;
+            frz[s] = t.frozen ? (t.frozen[a] != 0) : 0;
+            ljrow[s] = lj0 + (t.iac[a] - 1) * nt;
+            ljcol[s] = t.iac[a] - 1;
+        }
+        const int nst = (E.mol_slot_off[m + 1] - so);
+        for (int s = so; s < so + nst; s++) slot_mol[s] = m;
+        const int T0 = so / TILE, T1 = (so + nst) / TILE;
+        for (int T = T0; T < T1; T++) {
+            tile_mol[T] = m;
+            tile_end[T] = T1;
+            rowbase[T] = cap;
+            cap += T1 - T;
+        }
+        /* exclusions with partner index > own index (eff.c:1920-1923) */
+        int e = 0;
+        for (int i = 0; i < n; i++) {
+            ex_ptr[ao + i] = (int)ex_j.size();
+            for (int k = 0; k < t.iblo[i]; k++) {
+                int jj = t.exclat[e + k] - 1;
+                if (jj > i && jj < n) ex_j.push_back(ao + jj);
+            }
+            e += t.iblo[i];
+        }
+    }
+    ex_ptr[E.natoms] = (int)ex_j.size();
+    E.tp_cap = cap;
+    E.perm_h = perm;
+    E.frz_h = frz;
+
+    GBPar &P = E.par;
+    P.gb = o->gb;
+    P.has_neck = (o->gb == 7 || o->gb == 8);
+    P.nslots = NS;
+    P.nmol = nmol;
+    P.cut2 = o->cut * o->cut;
+    P.rgbmax = o->rgbmax;
+    P.rgbmax1i = 1.0 / o->rgbmax;
+    P.rgbmax2i = P.rgbmax1i * P.rgbmax1i;
+    P.rgbmaxpsmax2 = (o->rgbmax + o->fsmax) * (o->rgbmax + o->fsmax);
+    P.neckscale = o->gbneckscale;
+    P.kappa = o->kappa;
+    P.epsext = o->epsext;
+
+    DevPtrs &D = E.d;
+    D.ntiles = NT;
+    D.natoms = E.natoms;
+    if ((rc = E.upload(&D.perm, perm)) || (rc = E.upload(&D.slot_mol, slot_mol)) ||
+        (rc = E.upload(&D.q, q)) || (rc = E.upload(&D.rb, rb)) || (rc = E.upload(&D.ri0, ri0)) ||
+        (rc = E.upload(&D.sj, sj)) || (rc = E.upload(&D.ga, ga)) || (rc = E.upload(&D.gbb, gbb)) ||
+        (rc = E.upload(&D.gg, gg)) || (rc = E.upload(&D.neck, neck)) || (rc = E.upload(&D.frz, frz)) ||
+        (rc = E.upload(&D.ljrow, ljrow)) || (rc = E.upload(&D.ljcol, ljcol)) ||
+        (rc = E.upload(&D.tile_mol, tile_mol)) || (rc = E.upload(&D.tile_end, tile_end)) ||
+        (rc = E.upload(&D.rowbase, rowbase)) || (rc = E.upload(&D.ex_ptr, ex_ptr)) ||
+        (rc = E.upload(&D.ex_j, ex_j)) || (rc = E.upload(&D.inv, inv)) ||
+        (rc = E.upload(&D.lja, lja)) || (rc = E.upload(&D.ljb, ljb)))
+        return rc;
+    {
+        std::vector<float> qf(q.begin(), q.end()), rf(ri0.begin(), ri0.end()), sf(sj.begin(), sj.end()),
+            bf(rb.begin(), rb.end()), af(lja.begin(), lja.end()), cf(ljb.begin(), ljb.end());
+        std::vector<double> np(NECK_POS, NECK_POS + NECK_N * NECK_N), nv(NECK_VAL, NECK_VAL + NECK_N * NECK_N);
+        std::vector<float> npf(np.begin(), np.end()), nvf(nv.begin(), nv.end());
+        if ((rc = E.upload(&D.q_f, qf)) || (rc = E.upload(&D.ri0_f, rf)) || (rc = E.upload(&D.sj_f, sf)) ||
+            (rc = E.upload(&D.rb_f, bf)) || (rc = E.upload(&D.lja_f, af)) || (rc = E.upload(&D.ljb_f, cf)) ||
+            (rc = E.upload(&D.neck_pos, np)) || (rc = E.upload(&D.neck_val, nv)) ||
+            (rc = E.upload(&D.neck_pos_f, npf)) || (rc = E.upload(&D.neck_val_f, nvf)))
+            return rc;
+    }
+    if ((rc = E.dalloc(&D.xs, NS)) || (rc = E.dalloc(&D.ys, NS)) || (rc = E.dalloc(&D.zs, NS)) ||
+        (rc = E.dalloc(&D.sumi, NS)) || (rc = E.dalloc(&D.reff, NS)) || (rc = E.dalloc(&D.psi, NS)) ||
+        (rc = E.dalloc(&D.dfac, NS)) || (rc = E.dalloc(&D.sumdeijda, NS)) || (rc = E.dalloc(&D.gx, NS)) ||
+        (rc = E.dalloc(&D.gy, NS)) || (rc = E.dalloc(&D.gz, NS)) || (rc = E.dalloc(&D.ene, 3 * (size_t)nmol)) ||
+        (rc = E.dalloc(&D.cand, (size_t)cap)) || (rc = E.dalloc(&D.rowcnt, NT)) ||
+        (rc = E.dalloc(&D.rowstart, NT)) || (rc = E.dalloc(&D.ntp, 1)) || (rc = E.dalloc(&D.tp, (size_t)cap)) ||
+        (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
+        (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
+        (rc = E.dalloc(&D.bbhi, 3 * (size_t)NT)) || (rc = E.dalloc(&E.x_dev, 3 * (size_t)E.natoms)) ||
+        (rc = E.dalloc(&E.g_dev, 3 * (size_t)E.natoms)) || (rc = E.dalloc(&E.tmp_dev, (size_t)E.natoms)) ||
+        (rc = E.dalloc(&E.ene14_dev, 14)))
+        return rc;
+    HIPCHK(hipMemset(D.ntp, 0, sizeof(int)));
+    HIPCHK(hipMemset(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms));
+    if (x0) {
+        for (int m = 0; m < nmol; m++)
+            if (x0[m])
+                HIPCHK(hipMemcpy(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x0[m],
+                                 3 * sizeof(double) * (size_t)E.mol_natom[m], hipMemcpyHostToDevice));
+    }
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    E.pair_blocks = (int)std::min<long long>(std::max<long long>(1, cap), (long long)ncu * 16);
+    E.pair_blocks = std::min(E.pair_blocks, PAIR_BLOCKS_MAX * 4);
+    return 0;
+}
+
+int engine_gather(Engine &E, const double *x_dev)
+{
+    const int nb = (E.nslots + 255) / 256;
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, E.stream, E.d, E.par, x_dev);
+    LAUNCHCHK();
+    return 0;
+}
+
+int engine_nblist(Engine &E)
+{
+    const int NT = E.ntiles;
+    int rc = engine_gather(E, E.x_dev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bbox, dim3(NT), dim3(64), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_tilecand, dim3(NT), dim3(256), 0, E.stream, E.d, E.par);
+    hipLaunchKernelGGL(k_rowscan, dim3(1), dim3(1024), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_compact, dim3(NT), dim3(256), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_masks, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    if (E.natoms > 0)
+        hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
+    LAUNCHCHK();
+    E.have_list = true;
+    return 0;
+}
+
+/* one evaluation; coordinates already in E.x_dev or x_dev */
+int engine_eval(Engine &E, const double *x_dev, double *g_dev)
+{
+    if (!E.have_list) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
+    int rc = engine_gather(E, x_dev);
+    if (rc) return rc;
+    const int nb64 = (E.nslots + 63) / 64;
+    if (E.precision == GBHIP_FP64) {
+        hipLaunchKernelGGL(k_born<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_chain<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    } else {
+        hipLaunchKernelGGL(k_born<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_chain<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    }
+    if (g_dev)
+        hipLaunchKernelGGL(k_scatter, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev);
+    LAUNCHCHK();
+    E.have_eval = true;
+    return 0;
+}
+
+/* ---------------------------------------------------------- single context */
+
+
+extern "C" int gbhip_ctx_create(const gbhip_topology *top, const gbhip_options *opt, const double *x0,
+                                gbhip_ctx **out)
+{
+    if (!out) return seterr(GBHIP_EINVAL, "out pointer is NULL");
+    *out = nullptr;
+    gbhip_ctx *c = new (std::nothrow) gbhip_ctx;
+    if (!c) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    const double *xs[1] = {x0};
+    int rc = engine_build(c->E, 1, top, opt, xs);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int gbhip_ctx_destroy(gbhip_ctx *ctx)
+{
+    if (!ctx) return 0;
+    if (ctx->E.stream) (void)hipStreamSynchronize(ctx->E.stream);
+    else (void)hipDeviceSynchronize();
+    delete ctx;
+    return 0;
+}
+
+extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    ctx->E.stream = (hipStream_t)stream;
+    return 0;
+}
+
+extern "C" int gbhip_set_coords(gbhip_ctx *ctx, const double *x)
+{
+    if (!ctx || !x) return seterr(GBHIP_EINVAL, "NULL argument");
+    Engine &E = ctx->E;
+    HIPCHK(hipMemcpyAsync(E.x_dev, x, 3 * sizeof(double) * (size_t)E.natoms, hipMemcpyHostToDevice, E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_set_coords_device(gbhip_ctx *ctx, const double *d_x)
+{
+    if (!ctx || !d_x) return seterr(GBHIP_EINVAL, "NULL argument");
+    Engine &E = ctx->E;
+    HIPCHK(hipMemcpyAsync(E.x_dev, d_x, 3 * sizeof(double) * (size_t)E.natoms, hipMemcpyDeviceToDevice, E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_nblist_async(gbhip_ctx *ctx)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return engine_nblist(ctx->E);
+}
+
+extern "C" int gbhip_npairs(gbhip_ctx *ctx, long long *npairs)
+{
+    if (!ctx || !npairs) return seterr(GBHIP_EINVAL, "NULL argument");
+    Engine &E = ctx->E;
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, E.d.npairs, sizeof(v), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    *npairs = (long long)v;
+    return 0;
+}
+
+extern "C" int gbhip_nblist(gbhip_ctx *ctx, long long *nentries)
+{
+    int rc = gbhip_nblist_async(ctx);
+    if (rc) return rc;
+    long long np = 0;
+    if ((rc = gbhip_npairs(ctx, &np))) return rc;
+    if (nentries) *nentries = 2 * np;
+    return 0;
+}
+
+extern "C" int gbhip_tile_stats(gbhip_ctx *ctx, int *ntiles, int *ntilepairs)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    Engine &E = ctx->E;
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, E.d.ntp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (ntiles) *ntiles = E.ntiles;
+    if (ntilepairs) *ntilepairs = n;
+    return 0;
+}
+
+extern "C" int gbhip_egb_device(gbhip_ctx *ctx, double *d_grad, double *d_ene)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    Engine &E = ctx->E;
+    int rc = engine_eval(E, E.x_dev, d_grad);
+    if (rc) return rc;
+    if (d_ene) HIPCHK(hipMemcpyAsync(d_ene, E.d.ene, 3 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_egb(gbhip_ctx *ctx, double *grad, double *epol, double *evdw, double *eelec)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    Engine &E = ctx->E;
+    const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
+    if (grad) HIPCHK(hipMemsetAsync(E.g_dev, 0, nb, E.stream));
+    int rc = engine_eval(E, E.x_dev, grad ? E.g_dev : nullptr);
+    if (rc) return rc;
+    double ene[3];
+    HIPCHK(hipMemcpyAsync(ene, E.d.ene, sizeof(ene), hipMemcpyDeviceToHost, E.stream));
+    std::vector<double> g;
+    if (grad) {
+        g.resize(3 * (size_t)E.natoms);
+        HIPCHK(hipMemcpyAsync(g.data(), E.g_dev, nb, hipMemcpyDeviceToHost, E.stream));
+    }
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (grad)
+        for (size_t k = 0; k < g.size(); k++) grad[k] += g[k];
+    if (epol) *epol = ene[0];
+    if (evdw) *evdw = ene[1];
+    if (eelec) *eelec = ene[2];
+    return 0;
+}
+
+static int read_unsorted(Engine &E, const double *src, double *dst)
+{
+    if (!E.have_eval) return seterr(GBHIP_ESTATE, "no evaluation has run yet");
+    hipLaunchKernelGGL(k_unsort, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, src, E.tmp_dev);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(dst, E.tmp_dev, sizeof(double) * (size_t)E.natoms, hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_get_reff(gbhip_ctx *ctx, double *reff)
+{
+    if (!ctx || !reff) return seterr(GBHIP_EINVAL, "NULL argument");
+    return read_unsorted(ctx->E, ctx->E.d.reff, reff);
+}
+
+extern "C" int gbhip_get_sumdeijda(gbhip_ctx *ctx, double *sd)
+{
+    if (!ctx || !sd) return seterr(GBHIP_EINVAL, "NULL argument");
+    return read_unsorted(ctx->E, ctx->E.d.sumdeijda, sd);
+}
+
+extern "C" int gbhip_get_pairs(gbhip_ctx *ctx, int *pi, int *pj, long long cap, long long *npairs)
+{
+    if (!ctx || !npairs || cap < 0 || (cap > 0 && (!pi || !pj))) return seterr(GBHIP_EINVAL, "bad argument");
+    Engine &E = ctx->E;
+    if (!E.have_list) return seterr(GBHIP_ESTATE, "no pair list has been built");
+    int *dpi = nullptr, *dpj = nullptr;
+    unsigned long long *dcnt = nullptr, cnt = 0;
+    const size_t n = cap > 0 ? (size_t)cap : 1;
+    if (hipMalloc(&dpi, n * sizeof(int)) != hipSuccess || hipMalloc(&dpj, n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&dcnt, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(dpi);
+        (void)hipFree(dpj);
+        (void)hipFree(dcnt);
+        return seterr(GBHIP_ENOMEM, "read-back buffers");
+    }
+    int rc = 0;
+    if (hipMemsetAsync(dcnt, 0, sizeof(cnt), E.stream) != hipSuccess) rc = seterr(GBHIP_EHIP, "memset");
+    if (!rc) {
+        hipLaunchKernelGGL(k_listpairs, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, dpi, dpj, dcnt, cap);
+        if (hipGetLastError() != hipSuccess) rc = seterr(GBHIP_EHIP, "k_listpairs launch");
+    }
+    if (!rc && hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, E.stream) != hipSuccess)
+        rc = seterr(GBHIP_EHIP, "copy");
+    if (!rc && hipStreamSynchronize(E.stream) != hipSuccess) rc = seterr(GBHIP_EHIP, "sync");
+    const long long m = std::min<long long>((long long)cnt, cap);
+    if (!rc && m > 0 &&
+        (hipMemcpy(pi, dpi, m * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(pj, dpj, m * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = seterr(GBHIP_EHIP, "copy pairs");
+    (void)hipFree(dpi);
+    (void)hipFree(dpj);
+    (void)hipFree(dcnt);
+    *npairs = (long long)cnt;
+    return rc;
+}
+
+extern "C" int gbhip_sync(gbhip_ctx *ctx)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    HIPCHK(hipStreamSynchronize(ctx->E.stream));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ batch */
+
+
+extern "C" int gbhip_batch_create(int nmol, const gbhip_topology *tops, const gbhip_options *opt,
+                                  const double *const *x0, gbhip_batch **out)
+{
+    if (!out || nmol < 1 || !tops) return seterr(GBHIP_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    gbhip_batch *b = new (std::nothrow) gbhip_batch;
+    if (!b) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    int rc = engine_build(b->E, nmol, tops, opt, x0);
+    if (!rc) rc = engine_nblist(b->E);
+    if (!rc && hipStreamSynchronize(b->E.stream) != hipSuccess) rc = seterr(GBHIP_EHIP, "pair-list build failed");
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+extern "C" int gbhip_batch_destroy(gbhip_batch *b)
+{
+    if (!b) return 0;
+    (void)hipDeviceSynchronize();
+    delete b;
+    return 0;
+}
+
+extern "C" int gbhip_batch_set_coords(gbhip_batch *b, int m, const double *x)
+{
+    if (!b || !x || m < 0 || m >= b->E.nmol) return seterr(GBHIP_EINVAL, "bad molecule index");
+    Engine &E = b->E;
+    HIPCHK(hipMemcpyAsync(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x, 3 * sizeof(double) * (size_t)E.mol_natom[m],
+                          hipMemcpyHostToDevice, E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_batch_nblist(gbhip_batch *b)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return engine_nblist(b->E);
+}
+
+extern "C" int gbhip_batch_eval_device(gbhip_batch *b)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return engine_eval(b->E, b->E.x_dev, nullptr);
+}
+
+extern "C" int gbhip_batch_eval(gbhip_batch *b, double *ene, double *const *grads)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    Engine &E = b->E;
+    const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
+    bool want_g = false;
+    if (grads)
+        for (int m = 0; m < E.nmol; m++) want_g |= grads[m] != nullptr;
+    if (want_g) HIPCHK(hipMemsetAsync(E.g_dev, 0, nb, E.stream));
+    int rc = engine_eval(E, E.x_dev, want_g ? E.g_dev : nullptr);
+    if (rc) return rc;
+    std::vector<double> g;
+    if (want_g) {
+        g.resize(3 * (size_t)E.natoms);
+        HIPCHK(hipMemcpyAsync(g.data(), E.g_dev, nb, hipMemcpyDeviceToHost, E.stream));
+    }
+    if (ene) HIPCHK(hipMemcpyAsync(ene, E.d.ene, 3 * sizeof(double) * (size_t)E.nmol, hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (want_g)
+        for (int m = 0; m < E.nmol; m++) {
+            if (!grads[m]) continue;
+            const double *src = g.data() + 3 * (size_t)E.mol_atom_off[m];
+            for (int k = 0; k < 3 * E.mol_natom[m]; k++) grads[m][k] += src[k];
+        }
+    return 0;
+}
+
+extern "C" int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms, int *ntilepairs)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    Engine &E = b->E;
+    unsigned long long v = 0;
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&v, E.d.npairs, sizeof(v), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipMemcpyAsync(&n, E.d.ntp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (npairs) *npairs = (long long)v;
+    if (natoms) *natoms = E.natoms;
+    if (ntilepairs) *ntilepairs = n;
+    return 0;
+}

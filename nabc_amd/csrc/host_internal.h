@@ -1,0 +1,44 @@
+/*
+ * host_internal.h -- state shared by the host-side C files of nabc_amd.
+ * Not part of the public API.
+ */
+#ifndef NABC_AMD_HOST_INTERNAL_H
+#define NABC_AMD_HOST_INTERNAL_H
+
+#include <stdio.h>
+#include "sff_gb.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Option values set by mm_options().  Defaults are those of the file-scope
+ * statics in src/sff/sff.c:68-234; names follow the option keywords.
+ */
+typedef struct {
+    double cut, cutnp, scnb, scee, rgbmax, gboffset, epsext, kappa, dielc;
+    double dt, t, tautp, gamma_ln, temp0, boltz2, vlimit, genmass, tempi;
+    double wcons, k4d, surften, dradius, deltar, rwater, alphanp, kappanp;
+    double min_volume, wdssp;
+    int ntpr, nscm, nsnb, nsnp, nchk, nchk2, ntwx, ntpr_md, zerov;
+    int gb, gbsa, gb_debug, nr_debug, gb2_debug, gbsa_debug, e_debug;
+    int dield, irattle, ips, hcp, blocksize, dim, max_set_size;
+    int mpi_min_tasks, cull_np_lists, use_lower_tri, dynamic_loops;
+    /* nabc_amd extensions */
+    int gbprec;          /* 0 = fp64, 1 = mixed fp32-pair/fp64-accumulate */
+    unsigned long long md_seed; /* Langevin noise stream seed */
+} nabc_opts_t;
+
+extern nabc_opts_t nabc_opts;
+extern FILE *nabc_out;
+
+void nabc_seterr(const char *fmt, ...);
+int nabc_atomic_number_from_mass(double mass);
+void nabc_build_14_list(PARMSTRUCT_T *prm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,124 @@
+"""The oracle: CPU FP64 restatement of the reference GB path (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY -- the checker for the GPU path.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it; the
+product (nabc_amd) never does.  See oracle.h for the reference functions
+each entry point restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
+lib = C.CDLL(LIB_PATH)
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+_sig("orc_create", vp, [vp, ip, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                        C.c_double, C.c_int])
+_sig("orc_destroy", None, [vp])
+_sig("orc_gbparams", None, [vp, dp, dp, dp, dp, dp, dp, ip, dp])
+_sig("orc_nblist", C.c_longlong, [vp, dp])
+_sig("orc_list_counts", None, [vp, ip, ip])
+_sig("orc_list_copy", None, [vp, ip])
+_sig("orc_egb", C.c_double, [vp, dp, dp, dp, dp, dp, dp])
+_sig("orc_mme", C.c_double, [vp, dp, dp, C.c_int, dp, dp])
+_sig("orc_set_current", None, [vp, C.c_int])
+_sig("orc_mme_cb", C.c_double, [dp, dp, ip])
+_sig("orc_log_count", C.c_int, [])
+_sig("orc_log_get", None, [C.c_int, ip, dp, dp])
+_sig("orc_log_clear", None, [])
+_sig("orc_set_threads", None, [C.c_int])
+
+MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
+MME_CB = MMEFUNC(("orc_mme_cb", lib))
+
+
+def _p(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """Reference-algorithm evaluation over a PARMSTRUCT_T read by rdparm."""
+
+    def __init__(self, parm, natom, gb, cut, rgbmax, kappa=0.0, epsext=78.5, dielc=1.0,
+                 dield=1, frozen=None):
+        self.n = natom
+        self._frozen = None if frozen is None else np.ascontiguousarray(frozen, dtype=np.int32)
+        self.h = lib.orc_create(C.cast(parm, vp), _p(self._frozen, ip) if self._frozen is not None else None,
+                                gb, cut, rgbmax, kappa, epsext, dielc, dield)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def nblist(self, x):
+        return lib.orc_nblist(self.h, _p(np.ascontiguousarray(x, dtype=np.float64)))
+
+    def lists(self):
+        lo = np.zeros(self.n, np.int32)
+        up = np.zeros(self.n, np.int32)
+        lib.orc_list_counts(self.h, _p(lo, ip), _p(up, ip))
+        flat = np.zeros(max(1, int(lo.sum() + up.sum())), np.int32)
+        lib.orc_list_copy(self.h, _p(flat, ip))
+        return lo, up, flat
+
+    def egb(self, x):
+        """egb() on the current list: (epol, evdw, eelec, grad, reff, sumdeijda)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros(3 * self.n)
+        reff = np.zeros(self.n)
+        sd = np.zeros(self.n)
+        enb, eel = C.c_double(), C.c_double()
+        epol = lib.orc_egb(self.h, _p(x), _p(g), C.byref(enb), C.byref(eel), _p(reff), _p(sd))
+        return epol, enb.value, eel.value, g, reff, sd
+
+    def mme(self, x, rebuild=True):
+        """mme34(): (ene[13], frms, grad)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros(3 * self.n)
+        ene = np.zeros(13)
+        frms = C.c_double()
+        lib.orc_mme(self.h, _p(x), _p(g), 1 if rebuild else 0, _p(ene), C.byref(frms))
+        return ene, frms.value, g
+
+    def gbparams(self):
+        n = self.n
+        fs, a, b, g, q = (np.zeros(n) for _ in range(5))
+        neck = np.zeros(n, np.int32)
+        off, ns = C.c_double(), C.c_double()
+        lib.orc_gbparams(self.h, C.byref(off), C.byref(ns), _p(fs), _p(a), _p(b), _p(g), _p(neck, ip), _p(q))
+        return dict(gboffset=off.value, neckscale=ns.value, fs=fs, alpha=a, beta=b, gamma=g,
+                    neckidx=neck, charge=q)
+
+
+def set_threads(n):
+    lib.orc_set_threads(int(n))
+
+
+def log_entries():
+    out = []
+    for k in range(lib.orc_log_count()):
+        it = C.c_int()
+        e = np.zeros(13)
+        fr = C.c_double()
+        lib.orc_log_get(k, C.byref(it), _p(e), C.byref(fr))
+        out.append((it.value, e, fr.value))
+    return out

@@ -1,0 +1,67 @@
+/*
+ * oracle.h -- CPU FP64 restatement of the reference's GB energy path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for nabc_amd's GPU
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product never links or calls it.
+ *
+ * It follows, function by function, src/sff of dacase/nabc:
+ *   orc_create   GB parameter set-up of mme_init_sff (sff.c:1153-1162,
+ *                :1187-1333, gbneck.h:150-170)
+ *   orc_nblist   pair-list semantics of nblist()/searchkdtree()
+ *                (nblist.c:748-765, :1146-1147, :1189-1196) by brute force
+ *   orc_egb      egb() passes 1-3 (eff.c:1363-2417)
+ *   orc_mme      mme34() assembly (eff.c:2442-2947) with ebond/eangl/ephi
+ *                (eff.c:82-627) and the 1-4 nbond() (eff.c:896-1331)
+ */
+#ifndef NABC_AMD_ORACLE_H
+#define NABC_AMD_ORACLE_H
+
+#include "sff_gb.h"
+
+typedef struct orc orc_t;
+
+/* dield: 1 = constant dielectric (diel=C), 0 = distance-dependent (diel=R) */
+orc_t *orc_create(PARMSTRUCT_T *prm, const int *frozen, int gb, double cut,
+                  double rgbmax, double kappa, double epsext, double dielc,
+                  int dield);
+void orc_destroy(orc_t *o);
+
+/* GB parameters derived at creation (for comparison with the product). */
+void orc_gbparams(orc_t *o, double *gboffset, double *neckscale, double *fs,
+                  double *alpha, double *beta, double *gamma, int *neckidx,
+                  double *charges);
+
+/* Build the list from x; returns the number of entries (2 per pair). */
+long long orc_nblist(orc_t *o, const double *x);
+/* Per-atom lower/upper counts and the flattened lists (sizes from counts). */
+void orc_list_counts(orc_t *o, int *lpears, int *upears);
+void orc_list_copy(orc_t *o, int *flat);
+
+/* egb() on the current list; f (3N) accumulated into; reff (N) optional. */
+double orc_egb(orc_t *o, const double *x, double *f, double *enb, double *eel,
+               double *reff, double *sumdeijda);
+
+/*
+ * mme34(): builds the list when none exists or when rebuild != 0, zeroes f,
+ * adds bonded, 1-4 and GB terms, zeroes frozen gradients; fills ene[13]
+ * (mme34's layout) and frms; returns the total.
+ */
+double orc_mme(orc_t *o, const double *x, double *f, int rebuild, double *ene,
+               double *frms);
+
+/*
+ * mme-compatible callback over a current oracle, with the reference's
+ * rebuild cadence (first call, iter % nsnb == 0, iter == -4); every call's
+ * ene[0..12] and frms are appended to a log.
+ */
+void orc_set_current(orc_t *o, int nsnb);
+double orc_mme_cb(double *x, double *f, int *iter);
+int orc_log_count(void);
+void orc_log_get(int k, int *iter, double *ene13, double *frms);
+void orc_log_clear(void);
+
+/* threads used by the OpenMP loops (0 = library default) */
+void orc_set_threads(int n);
+
+#endif

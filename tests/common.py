@@ -1,0 +1,52 @@
+"""Shared helpers for the test suite (fixture loading, generated systems)."""
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+REPO = os.path.dirname(HERE)
+
+
+def golden():
+    with open(os.path.join(GOLDEN, "golden.json")) as fp:
+        return json.load(fp)
+
+
+def read_pdb_xyz(path):
+    """Coordinates of ATOM/HETATM records in file order (PDB columns 31-54)."""
+    xyz, names = [], []
+    for line in open(path):
+        if line.startswith(("ATOM", "HETATM")):
+            xyz.append((float(line[30:38]), float(line[38:46]), float(line[46:54])))
+            names.append(line[12:16].strip())
+    return np.array(xyz, dtype=np.float64).ravel(), names
+
+
+_gen_cache = {}
+
+
+def generated_system(natom, seed, tmpdir):
+    """Write (once per session) and read back a synthetic system."""
+    import nabc_amd as na
+    key = (natom, seed)
+    if key not in _gen_cache:
+        top = os.path.join(tmpdir, "gen_%d_%d.prmtop" % key)
+        rst = os.path.join(tmpdir, "gen_%d_%d.rst7" % key)
+        na.generate(natom, seed, top, rst)
+        _gen_cache[key] = (top, rst)
+    top, rst = _gen_cache[key]
+    parm = na.read_parm(top)
+    arrs = na.parm_arrays(parm)
+    x, _ = na.read_rst7(rst, arrs["natom"])
+    return parm, arrs, x
+
+
+def rel_rms(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)

@@ -1,0 +1,88 @@
+"""Collect the reference's own known-answer data for the GB path as fixtures.
+
+Run in a container where dacase/nabc is mounted at /root/reference.  It
+copies the input files the reference's tests use (data files only) and
+transcribes the expected numbers from their golden outputs into
+golden.json, each with its source file:line.
+
+  aspgb8   test/nab/aspgb8.nab:8-18, aspgb8.out.check:12-47
+           24 igb=8 effective Born radii (7 decimals) and per-term
+           energies (9 decimals), cut = rgbmax = 100.
+  gb5trx, gb8trx, gb7trx, gbtrx
+           test/nab/gb*trx.nab, gb*trx.out.check: the "ff:" line of every
+           mme() call of a 10-iteration conjgrad run on thioredoxin
+           (1654 atoms), 2 decimals, frms to 3 significant digits.
+"""
+import json
+import os
+import re
+import shutil
+
+REF = "/root/reference/test/nab"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def ff_lines(path):
+    out = []
+    for k, line in enumerate(open(path), 1):
+        if line.startswith("ff:"):
+            f = line.split()
+            out.append(dict(line=k, iter=int(f[1]), total=float(f[2]), bad=float(f[3]),
+                            vdw=float(f[4]), elect=float(f[5]), nonpolar=float(f[6]),
+                            genborn=float(f[7]), frms=float(f[8])))
+    return out
+
+
+def main():
+    for name in ("asp.top", "asp.pdb", "prmtop.gb2", "trxox.2.4ns.pdb"):
+        shutil.copyfile(os.path.join(REF, name), os.path.join(HERE, name))
+    g = {}
+    # aspgb8: radii + per-term energies
+    path = os.path.join(REF, "aspgb8.out.check")
+    radii, terms = [], {}
+    for k, line in enumerate(open(path), 1):
+        m = re.match(r"^(\d+)\t\s*([\d.]+)\t\s*([\d.]+)\s*$", line)
+        if m:
+            radii.append(dict(line=k, atom=int(m.group(1)), rborn=float(m.group(2)),
+                              reff=float(m.group(3))))
+        m = re.match(r"^\s*(bond|angle|dihedral|enb14|eel14|enb|eel|egb|econs|esurf|Total):\s+(\S+)", line)
+        if m:
+            terms[m.group(1)] = dict(line=k, value=float(m.group(2)))
+    g["aspgb8_check"] = dict(source="test/nab/aspgb8.out.check", prmtop="asp.top", pdb="asp.pdb",
+                             options=dict(gb=8, cut=100.0, rgbmax=100.0, kappa=0.0, epsext=78.5,
+                                          diel="C"),
+                             radii=radii, terms=terms)
+    # aspgb8.out: the same input run with the reference's current code (an MPI
+    # run, so the radii lines are interleaved out of order); its igb=8 radii
+    # and egb differ from the older .check file, whose other terms agree.
+    path = os.path.join(REF, "aspgb8.out")
+    radii, terms = [], {}
+    for k, line in enumerate(open(path), 1):
+        m = re.match(r"^(\d+)\t\s*([\d.]+)\t\s*([\d.]+)\s*$", line)
+        if m:
+            radii.append(dict(line=k, atom=int(m.group(1)), rborn=float(m.group(2)),
+                              reff=float(m.group(3))))
+        m = re.match(r"^\s*(bond|angle|dihedral|enb14|eel14|enb|eel|egb|econs|esurf|Total|frms):\s+(\S+)",
+                     line)
+        if m:
+            terms[m.group(1)] = dict(line=k, value=float(m.group(2)))
+    radii.sort(key=lambda r: r["atom"])
+    g["aspgb8"] = dict(source="test/nab/aspgb8.out", prmtop="asp.top", pdb="asp.pdb",
+                       options=dict(gb=8, cut=100.0, rgbmax=100.0, kappa=0.0, epsext=78.5,
+                                    diel="C"),
+                       radii=radii, terms=terms)
+    for name, opt in (("gb5trx", dict(gb=5, kappa=0.14701, rgbmax=99.0, cut=99.0)),
+                      ("gb8trx", dict(gb=8, kappa=0.0, rgbmax=15.0, cut=99.0)),
+                      ("gb7trx", dict(gb=7, kappa=0.0, rgbmax=15.0, cut=99.0)),
+                      ("gbtrx", dict(gb=2, kappa=0.14701, rgbmax=99.0, cut=99.0))):
+        src = os.path.join(REF, name + ".out.check")
+        opt.update(epsext=78.5, diel="C", nsnb=10, cg=dict(maxiter=10, dfpred=10.0,
+                                                             rmsgrad_per_dof=0.0001))
+        g[name] = dict(source="test/nab/%s.out.check" % name, prmtop="prmtop.gb2",
+                       pdb="trxox.2.4ns.pdb", options=opt, ff=ff_lines(src))
+    with open(os.path.join(HERE, "golden.json"), "w") as fp:
+        json.dump(g, fp, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""GPU path (C-ABI) against the oracle and the reference's known answers.
+
+Tolerances: the FP64 kernels must agree with the oracle's FP64 egb() to
+1e-9 relative on energies and 1e-9 relative RMS on gradients (they differ
+only in summation order and libm last-ulp effects); the mixed FP32-pair /
+FP64-accumulate kernels to the project targets, 1e-6 relative per energy
+term and 1e-5 relative RMS on gradients.  Pair lists must be identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from common import GOLDEN, generated_system, golden, read_pdb_xyz, rel, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+FP64_E, FP64_G = 1e-9, 1e-9
+MIX_E, MIX_G = 1e-6, 1e-5
+
+
+def _compare(na, parm, arr, x, gb, cut, rgbmax, kappa=0.0, precision=0, frozen=None,
+             e_tol=None, g_tol=None, check_list=True):
+    e_tol = e_tol if e_tol is not None else (FP64_E if precision == 0 else MIX_E)
+    g_tol = g_tol if g_tol is not None else (FP64_G if precision == 0 else MIX_G)
+    ctx = na.GBContext(arr, x, gb=gb, cut=cut, rgbmax=rgbmax, kappa=kappa, precision=precision,
+                       frozen=frozen)
+    nent = ctx.nblist()
+    orc = oracle.Oracle(parm, arr["natom"], gb=gb, cut=cut, rgbmax=rgbmax, kappa=kappa, frozen=frozen)
+    r_nent = orc.nblist(x)
+    assert nent == r_nent
+    if check_list:
+        lo, up, flat = orc.lists()
+        ref = []
+        off = 0
+        for i in range(arr["natom"]):
+            js = flat[off:off + lo[i] + up[i]]
+            off += lo[i] + up[i]
+            ref.extend((i, j) for j in js if j > i)
+        ref = np.array(ref, dtype=np.int64).reshape(-1, 2)
+        got = ctx.pairs()
+        assert got.shape == ref.shape and np.array_equal(got, ref)
+    epol, evdw, eel, g = ctx.egb()
+    r_epol, r_evdw, r_eel, r_g, r_reff, r_sd = orc.egb(x)
+    errs = dict(epol=rel(epol, r_epol), evdw=rel(evdw, r_evdw), eel=rel(eel, r_eel),
+                grad=rel_rms(g, r_g))
+    assert errs["epol"] <= e_tol and errs["evdw"] <= e_tol and errs["eel"] <= e_tol, errs
+    assert errs["grad"] <= g_tol, errs
+    if precision == 0:
+        assert np.max(np.abs(ctx.reff() - r_reff) / r_reff) <= 1e-11
+    return ctx, orc, errs
+
+
+def test_aspgb8_known_answers(gpu):
+    na = gpu
+    g = golden()["aspgb8"]
+    parm = na.read_parm(GOLDEN + "/asp.top")
+    arr = na.parm_arrays(parm)
+    x, _ = read_pdb_xyz(GOLDEN + "/asp.pdb")
+    ctx, _, _ = _compare(na, parm, arr, x, gb=8, cut=100.0, rgbmax=100.0)
+    reff = ctx.reff()
+    for r in g["radii"]:
+        assert abs(reff[r["atom"] - 1] - r["reff"]) <= 6e-8
+    epol, evdw, eel, _ = ctx.egb()
+    assert abs(epol - g["terms"]["egb"]["value"]) <= 1e-9
+    assert abs(evdw - g["terms"]["enb"]["value"]) <= 1e-9
+    assert abs(eel - g["terms"]["eel"]["value"]) <= 1e-9
+
+
+@pytest.mark.parametrize("case", ["gb5trx", "gb8trx", "gb7trx", "gbtrx"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_trx_models(gpu, case, precision):
+    na = gpu
+    o = golden()[case]["options"]
+    parm = na.read_parm(GOLDEN + "/prmtop.gb2")
+    arr = na.parm_arrays(parm)
+    x, _ = read_pdb_xyz(GOLDEN + "/trxox.2.4ns.pdb")
+    _compare(na, parm, arr, x, gb=o["gb"], cut=o["cut"], rgbmax=o["rgbmax"], kappa=o["kappa"],
+             precision=precision)
+
+
+@pytest.mark.parametrize("natom,gb,cut,rgbmax,kappa", [
+    (600, 5, 999.0, 999.0, 0.0),
+    (600, 1, 999.0, 999.0, 0.10395),
+    (2500, 5, 999.0, 999.0, 0.0),
+    (2500, 8, 12.0, 10.0, 0.127315),
+    (2500, 2, 15.0, 15.0, 0.14701),
+])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_generated(gpu, gen_dir, natom, gb, cut, rgbmax, kappa, precision):
+    na = gpu
+    parm, arr, x = generated_system(natom, 11, gen_dir)
+    _compare(na, parm, arr, x, gb=gb, cut=cut, rgbmax=rgbmax, kappa=kappa, precision=precision)
+
+
+def test_frozen_atoms(gpu, gen_dir):
+    """Not-both-frozen list rule, no self term and no chain rule for frozen atoms."""
+    na = gpu
+    parm, arr, x = generated_system(600, 3, gen_dir)
+    frozen = np.zeros(arr["natom"], np.int32)
+    frozen[::3] = 1
+    frozen[100:160] = 1
+    _compare(na, parm, arr, x, gb=5, cut=14.0, rgbmax=12.0, frozen=frozen)
+
+
+def test_list_frozen_between_rebuilds(gpu, gen_dir):
+    """Membership is fixed at build time: after the atoms move, both sides
+    keep the old list until the next rebuild (eff.c:2564-2578)."""
+    na = gpu
+    parm, arr, x0 = generated_system(2500, 5, gen_dir)
+    ctx = na.GBContext(arr, x0, gb=5, cut=10.0, rgbmax=10.0, kappa=0.1)
+    orc = oracle.Oracle(parm, arr["natom"], gb=5, cut=10.0, rgbmax=10.0, kappa=0.1)
+    assert ctx.nblist() == orc.nblist(x0)
+    rng = np.random.default_rng(0)
+    x1 = x0 + rng.normal(0, 0.3, x0.shape)
+    ctx.set_coords(x1)
+    e = ctx.egb()
+    r = orc.egb(x1)
+    assert rel(e[0], r[0]) <= FP64_E and rel(e[2], r[2]) <= FP64_E and rel_rms(e[3], r[3]) <= FP64_G
+    # rebuild at x1 on both sides
+    assert ctx.nblist() == orc.nblist(x1)
+    e = ctx.egb()
+    r = orc.egb(x1)
+    assert rel(e[0], r[0]) <= FP64_E and rel_rms(e[3], r[3]) <= FP64_G
+
+
+def test_mme_full_path(gpu, gen_dir):
+    """Host mme() (bonded + 1-4 + GB on the device) against the oracle's mme34."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 7, gen_dir)
+    na.lib.mm_reset_options()
+    assert na.lib.mm_options(b"gb=8, cut=20.0, rgbmax=18.0, kappa=0.127315, diel=C, ntpr=100000") == 0
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+    f = np.zeros(3 * arr["natom"])
+    it = na.C.c_int(-2)
+    xc = np.ascontiguousarray(x)
+    # first call builds the list (no list yet)
+    e0 = na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+    ene = np.zeros(13)
+    frms = na.C.c_double()
+    na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(frms))
+    # the oracle reads the same (gb-8-modified) topology
+    orc = oracle.Oracle(parm, arr["natom"], gb=8, cut=20.0, rgbmax=18.0, kappa=0.127315)
+    r_ene, r_frms, r_g = orc.mme(x)
+    assert rel(e0, r_ene[0]) <= 1e-9
+    for k in (1, 2, 3, 4, 5, 7, 8, 10):
+        assert abs(ene[k] - r_ene[k]) <= 1e-9 * max(1.0, abs(r_ene[k])), (k, ene[k], r_ene[k])
+    assert rel_rms(f, r_g) <= 1e-9
+    assert rel(frms.value, r_frms) <= 1e-9
+    it = na.C.c_int(-3)
+    na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_cg_trajectory_on_gpu(gpu):
+    """gb5trx.nab run entirely through the product: conjgrad over mme()."""
+    na = gpu
+    g = golden()["gb5trx"]
+    parm = na.read_parm(GOLDEN + "/prmtop.gb2")
+    arr = na.parm_arrays(parm)
+    x, _ = read_pdb_xyz(GOLDEN + "/trxox.2.4ns.pdb")
+    na.lib.mm_reset_options()
+    assert na.lib.mm_options(b"ntpr=1, gb=5, kappa=0.14701, rgbmax=99.0, cut=99.0, diel=C ") == 0
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+    seen = []
+
+    def cb(xp, fp, itp):
+        e = na.lib.mme(xp, fp, itp)
+        ene = np.zeros(13)
+        fr = na.C.c_double()
+        na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(fr))
+        seen.append((itp[0], ene.copy(), fr.value))
+        return e
+
+    fn = na.MMEFUNC(cb)
+    n = na.C.c_int(3 * arr["natom"])
+    f = na.C.c_double()
+    rms = na.C.c_double(3 * arr["natom"] * 0.0001)
+    dfp = na.C.c_double(10.0)
+    mx = na.C.c_int(10)
+    xc = np.ascontiguousarray(x)
+    ret = na.lib.conjgrad(xc.ctypes.data_as(na.dp), na.C.byref(n), na.C.byref(f), fn, na.C.byref(rms),
+                          na.C.byref(dfp), na.C.byref(mx))
+    assert ret == -3
+    assert [s[0] for s in seen] == [w["iter"] for w in g["ff"]]
+    for (it, ene, frms), w in zip(seen, g["ff"]):
+        assert abs(ene[0] - w["total"]) <= 0.0051
+        assert abs(ene[10] - w["genborn"]) <= 0.0051
+        assert abs(ene[2] + ene[8] - w["elect"]) <= 0.0051
+    it = na.C.c_int(-3)
+    na.lib.mme(xc.ctypes.data_as(na.dp), xc.ctypes.data_as(na.dp), na.C.byref(it))
