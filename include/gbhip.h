@@ -134,9 +134,19 @@ int gbhip_nblist_async(gbhip_ctx *ctx);
 
 /*
  * Evaluation writing to device memory: d_grad[3*natom] is accumulated into
- * (may be NULL), d_ene[3] receives {epol, evdw, eelec} (overwritten).
+ * (NULL: the gradient goes to a buffer of the context, cleared first),
+ * d_ene[3] (may be NULL) receives {epol, evdw, eelec}.
  */
 int gbhip_egb_device(gbhip_ctx *ctx, double *d_grad, double *d_ene);
+
+/*
+ * Per-pass timing with HIP events on the context's stream: after
+ * gbhip_timing_start, up to max_evals evaluations record events around
+ * pass 1, the radii kernel, pass 2 and pass 3; gbhip_timing_stop waits and
+ * returns the summed milliseconds ms4[4] and the number of evaluations.
+ */
+int gbhip_timing_start(gbhip_ctx *ctx, int max_evals);
+int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
 
 /* Wait for the context's stream. */
 int gbhip_sync(gbhip_ctx *ctx);
@@ -221,6 +231,10 @@ int gbhip_batch_eval(gbhip_batch *b, double *ene, double *const *grads);
 
 /* Evaluation only, results left on the device (for timing). */
 int gbhip_batch_eval_device(gbhip_batch *b);
+
+/* Per-pass timing of batch evaluations (as gbhip_timing_start/stop). */
+int gbhip_batch_timing_start(gbhip_batch *b, int max_evals);
+int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals);
 
 /* Total unordered pairs and atoms of the batch. */
 int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms,

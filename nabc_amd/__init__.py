@@ -100,6 +100,10 @@ _sig("gbhip_sync", C.c_int, [vp])
 _sig("gbhip_npairs", C.c_int, [vp, C.POINTER(C.c_longlong)])
 _sig("gbhip_tile_stats", C.c_int, [vp, ip, ip])
 _sig("gbhip_get_pairs", C.c_int, [vp, ip, ip, C.c_longlong, C.POINTER(C.c_longlong)])
+_sig("gbhip_timing_start", C.c_int, [vp, C.c_int])
+_sig("gbhip_timing_stop", C.c_int, [vp, dp, ip])
+_sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
+_sig("gbhip_batch_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
 _sig("gbhip_mme", C.c_int, [vp, dp, dp])
 _sig("gbhip_mme_device", C.c_int, [vp, dp, dp])
@@ -334,6 +338,26 @@ class GBContext:
         n = C.c_longlong()
         _check(lib.gbhip_npairs(self.h, C.byref(n)))
         return n.value
+
+    def nblist_async(self):
+        _check(lib.gbhip_nblist_async(self.h))
+
+    def egb_device(self):
+        """Asynchronous evaluation, results kept on the device."""
+        _check(lib.gbhip_egb_device(self.h, None, None))
+
+    def sync(self):
+        _check(lib.gbhip_sync(self.h))
+
+    def timing_start(self, max_evals):
+        _check(lib.gbhip_timing_start(self.h, max_evals))
+
+    def timing_stop(self):
+        """Summed ms of (pass 1, radii, pass 2, pass 3) and the eval count."""
+        ms = np.zeros(4)
+        n = C.c_int()
+        _check(lib.gbhip_timing_stop(self.h, _p(ms), C.byref(n)))
+        return ms, n.value
 
     def pairs(self):
         """Current list as a sorted (P, 2) array of (min, max) atom indices."""

@@ -86,9 +86,14 @@ struct Engine {
     double *tmp_dev = nullptr; /* natoms doubles */
     bool have_list = false;
     bool have_eval = false;
+    /* optional per-pass timing: 5 events per evaluation on the engine stream
+     * (before pass 1, after pass 1, after the radii, after pass 2, after pass 3) */
+    std::vector<hipEvent_t> tev;
+    int tev_cap = 0, tev_used = 0;
 
     ~Engine()
     {
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
     }
 

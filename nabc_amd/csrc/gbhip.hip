@@ -891,17 +891,29 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
     int rc = engine_gather(E, x_dev);
     if (rc) return rc;
     const int nb64 = (E.nslots + 63) / 64;
-    if (E.precision == GBHIP_FP64) {
-        hipLaunchKernelGGL(k_born<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_chain<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
-    } else {
-        hipLaunchKernelGGL(k_born<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_chain<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    hipEvent_t *ev = nullptr;
+    if (E.tev_used + 5 <= E.tev_cap) {
+        ev = &E.tev[E.tev_used];
+        E.tev_used += 5;
     }
+    if (ev) HIPCHK(hipEventRecord(ev[0], E.stream));
+    if (E.precision == GBHIP_FP64)
+        hipLaunchKernelGGL(k_born<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    else
+        hipLaunchKernelGGL(k_born<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
+    hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
+    if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
+    if (E.precision == GBHIP_FP64)
+        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    else
+        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
+    if (E.precision == GBHIP_FP64)
+        hipLaunchKernelGGL(k_chain<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    else
+        hipLaunchKernelGGL(k_chain<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (g_dev)
         hipLaunchKernelGGL(k_scatter, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev);
     LAUNCHCHK();
@@ -909,7 +921,73 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
     return 0;
 }
 
+/* ----------------------------------------------------------------- timing */
+
+__global__ __launch_bounds__(256) void k_zero(double *p, long long n)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
+
+static int timing_start(Engine &E, int max_evals)
+{
+    if (max_evals < 0 || max_evals > 1000000) return seterr(GBHIP_EINVAL, "max_evals out of range");
+    HIPCHK(hipStreamSynchronize(E.stream));
+    for (hipEvent_t e : E.tev) (void)hipEventDestroy(e);
+    E.tev.assign(5 * (size_t)max_evals, nullptr);
+    for (auto &e : E.tev) HIPCHK(hipEventCreate(&e));
+    E.tev_cap = 5 * max_evals;
+    E.tev_used = 0;
+    return 0;
+}
+
+/* ms4: summed milliseconds of pass 1, radii, pass 2, pass 3 over nevals */
+static int timing_stop(Engine &E, double *ms4, int *nevals)
+{
+    HIPCHK(hipStreamSynchronize(E.stream));
+    double acc[4] = {0, 0, 0, 0};
+    const int n = E.tev_used / 5;
+    for (int k = 0; k < n; k++) {
+        for (int s = 0; s < 4; s++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, E.tev[5 * k + s], E.tev[5 * k + s + 1]));
+            acc[s] += ms;
+        }
+    }
+    for (hipEvent_t e : E.tev) (void)hipEventDestroy(e);
+    E.tev.clear();
+    E.tev_cap = E.tev_used = 0;
+    if (ms4)
+        for (int s = 0; s < 4; s++) ms4[s] = acc[s];
+    if (nevals) *nevals = n;
+    return 0;
+}
+
 /* ---------------------------------------------------------- single context */
+
+extern "C" int gbhip_timing_start(gbhip_ctx *ctx, int max_evals)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return timing_start(ctx->E, max_evals);
+}
+
+extern "C" int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return timing_stop(ctx->E, ms4, nevals);
+}
+
+extern "C" int gbhip_batch_timing_start(gbhip_batch *b, int max_evals)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return timing_start(b->E, max_evals);
+}
+
+extern "C" int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return timing_stop(b->E, ms4, nevals);
+}
 
 
 extern "C" int gbhip_ctx_create(const gbhip_topology *top, const gbhip_options *opt, const double *x0,
@@ -1004,7 +1082,13 @@ extern "C" int gbhip_egb_device(gbhip_ctx *ctx, double *d_grad, double *d_ene)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
     Engine &E = ctx->E;
-    int rc = engine_eval(E, E.x_dev, d_grad);
+    if (!d_grad) {
+        /* gradient into the context's own buffer, cleared first */
+        const long long n3 = 3LL * E.natoms;
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, E.stream, E.g_dev, n3);
+        LAUNCHCHK();
+    }
+    int rc = engine_eval(E, E.x_dev, d_grad ? d_grad : E.g_dev);
     if (rc) return rc;
     if (d_ene) HIPCHK(hipMemcpyAsync(d_ene, E.d.ene, 3 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
     return 0;
