@@ -205,6 +205,43 @@ int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14);
 /* Same, device pointers, asynchronous. */
 int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14);
 
+/* ---- dynamics on the device (md(), sff.c:2071-2366) ---- */
+
+typedef struct {
+    double dt;        /* time step, ps */
+    double temp0;     /* target temperature, K */
+    double tempi;     /* initial temperature, K (0: keep v0) */
+    double gamma_ln;  /* Langevin collision frequency, 1/ps (0: Berendsen) */
+    double tautp;     /* Berendsen coupling time, ps */
+    double vlimit;    /* velocity clamp */
+    double boltz2;    /* one half of Boltzmann's constant, 9.93595e-4 */
+    int nsnb;         /* pair-list rebuild every nsnb steps */
+    int zerov;        /* start from zero velocities */
+    unsigned long long seed; /* noise stream */
+} gbhip_md_options;
+
+/*
+ * Set up dynamics on a context that has bonded terms (gbhip_set_bonded):
+ * masses[natom], coordinates x[3*natom], velocities v0 (may be NULL).
+ * Builds the list and the step-0 forces as md() does; log_cap rows of
+ * per-step energies are kept on the device (ring buffer).
+ */
+int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double *x,
+                  const double *v0, const gbhip_md_options *opt, int log_cap);
+/*
+ * Run nsteps steps without host synchronisation (asynchronous).  With
+ * use_graph != 0 each step is one hipGraph launch (a plain step, or a
+ * step with the pair-list rebuild every nsnb steps).
+ */
+int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph);
+/* Current coordinates, velocities and gradient (any may be NULL); syncs. */
+int gbhip_md_state(gbhip_ctx *ctx, double *x, double *v, double *grad);
+/*
+ * Per-step rows of the last min(steps, log_cap, max_rows) steps, 16 doubles
+ * each: mme energies [0..12] (mme34 layout), rms gradient, ekin, temp.
+ */
+int gbhip_md_log(gbhip_ctx *ctx, double *rows, int max_rows, int *nrows);
+
 /* ---- batched scoring: many molecules in one launch per pass ---- */
 
 /*

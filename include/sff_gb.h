@@ -98,6 +98,14 @@ double mme(double *x, double *f, int *iter);
  */
 void mme_get_energies(double ene[13], double *frms);
 
+/*
+ * The GPU context behind mme() (a gbhip_ctx *, see gbhip.h), created from
+ * coordinates x if needed, with the bonded terms of the mme_init_sff
+ * topology attached; NULL on error.  Lets a caller drive the device
+ * directly (gbhip_md_*, gbhip_mme_device) on the same state.
+ */
+void *mme_context(double *x);
+
 /* Dynamics (sff.c:2071) and conjugate gradients (conjgrad.c:34). */
 int md(int n, int maxstep, double *x, double *f, double *v,
        double (*func)(double *, double *, int *));

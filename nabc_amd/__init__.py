@@ -72,6 +72,26 @@ class Parm(C.Structure):
                 ("N14sceelist", dp), ("N14scnblist", dp)]
 
 
+class MDOptions(C.Structure):
+    _fields_ = [("dt", C.c_double), ("temp0", C.c_double), ("tempi", C.c_double),
+                ("gamma_ln", C.c_double), ("tautp", C.c_double), ("vlimit", C.c_double),
+                ("boltz2", C.c_double), ("nsnb", C.c_int), ("zerov", C.c_int),
+                ("seed", C.c_ulonglong)]
+
+
+class Bonded(C.Structure):
+    _fields_ = [("nbond", C.c_int), ("bond_i", C.c_void_p), ("bond_j", C.c_void_p),
+                ("bond_rk", C.c_void_p), ("bond_req", C.c_void_p), ("nangle", C.c_int),
+                ("angle_i", C.c_void_p), ("angle_j", C.c_void_p), ("angle_k", C.c_void_p),
+                ("angle_tk", C.c_void_p), ("angle_teq", C.c_void_p), ("ndihed", C.c_int),
+                ("dihed_i", C.c_void_p), ("dihed_j", C.c_void_p), ("dihed_k", C.c_void_p),
+                ("dihed_l", C.c_void_p), ("dihed_pk", C.c_void_p), ("dihed_pn", C.c_void_p),
+                ("dihed_phase", C.c_void_p), ("npair14", C.c_int), ("p14_i", C.c_void_p),
+                ("p14_j", C.c_void_p), ("p14_hb", C.c_void_p), ("p14_scee", C.c_void_p),
+                ("p14_scnb", C.c_void_p), ("p14_a", C.c_void_p), ("p14_b", C.c_void_p),
+                ("charge", C.c_void_p), ("dield", C.c_int)]
+
+
 ParmP = C.POINTER(Parm)
 MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
 
@@ -102,6 +122,10 @@ _sig("gbhip_tile_stats", C.c_int, [vp, ip, ip])
 _sig("gbhip_get_pairs", C.c_int, [vp, ip, ip, C.c_longlong, C.POINTER(C.c_longlong)])
 _sig("gbhip_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_timing_stop", C.c_int, [vp, dp, ip])
+_sig("gbhip_md_init", C.c_int, [vp, dp, dp, dp, C.POINTER(MDOptions), C.c_int])
+_sig("gbhip_md_run", C.c_int, [vp, C.c_int, C.c_int])
+_sig("gbhip_md_state", C.c_int, [vp, dp, dp, dp])
+_sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
@@ -128,6 +152,7 @@ _sig("mm_get_error", C.c_char_p, [])
 _sig("mme_init_sff", C.c_int, [ParmP, ip, ip, dp, C.c_char_p])
 _sig("mme", C.c_double, [dp, dp, ip])
 _sig("mme_get_energies", None, [dp, dp])
+_sig("mme_context", vp, [dp])
 _sig("md", C.c_int, [C.c_int, C.c_int, dp, dp, dp, MMEFUNC])
 _sig("conjgrad", C.c_int, [dp, ip, dp, MMEFUNC, dp, dp, ip])
 _sig("nabc_md_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])

@@ -76,6 +76,7 @@ struct Engine {
     GBPar par{};
     DevPtrs d{};
     hipStream_t stream = nullptr;
+    bool own_stream = false;   /* created by the engine, destroyed with it */
     std::vector<int> mol_atom_off, mol_natom, mol_slot_off;
     std::vector<int> perm_h;  /* slot -> global atom */
     std::vector<int> frz_h;   /* slot -> frozen flag */
@@ -94,6 +95,10 @@ struct Engine {
     ~Engine()
     {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        if (own_stream && stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
         for (void *p : allocs) (void)hipFree(p);
     }
 
@@ -137,9 +142,33 @@ struct BondedDev {
     double *ene = nullptr;        /* 13 doubles, mme34 layout */
 };
 
+/* device-resident dynamics state (mm_device.hip) */
+struct MDState {
+    bool ready = false;
+    double *v = nullptr, *accel = nullptr, *minv = nullptr, *sqrmass = nullptr;
+    double *scal = nullptr;   /* [0] ekin of the last step, [1] ekin accumulator */
+    int *step = nullptr;      /* device step counter (noise stream index) */
+    double *log = nullptr;    /* per step: ene[0..13], ekin, temp */
+    int log_cap = 0, steps_done = 0;
+    double dtx = 0, dt5 = 0, c_implic = 1, c_explic = 1, c_ave = 1, sdfac = 0, gammai = 0;
+    double ekin0 = 0, dttp = 0, vlimit = 10, rndf = 1, boltz2 = 9.93595e-4;
+    int nsnb = 10;
+    unsigned long long seed = 0;
+    hipGraphExec_t g_step = nullptr, g_rebuild = nullptr;
+    hipGraph_t gr_step = nullptr, gr_rebuild = nullptr;
+};
+
 struct gbhip_ctx {
     Engine E;
     BondedDev B;
+    MDState M;
+    ~gbhip_ctx()
+    {
+        if (M.g_step) (void)hipGraphExecDestroy(M.g_step);
+        if (M.g_rebuild) (void)hipGraphExecDestroy(M.g_rebuild);
+        if (M.gr_step) (void)hipGraphDestroy(M.gr_step);
+        if (M.gr_rebuild) (void)hipGraphDestroy(M.gr_rebuild);
+    }
 };
 
 struct gbhip_batch {

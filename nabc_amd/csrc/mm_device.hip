@@ -368,3 +368,318 @@ extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
         for (int k = 0; k < 14; k++) ene14[k] = e[k];
     return 0;
 }
+
+/* ------------------------------------------------------------- dynamics */
+
+/*
+ * Device-resident md() (sff.c:2071-2366): velocity Verlet with the
+ * Langevin (gamma_ln > 0) or Berendsen (gamma_ln = 0) scheme of the
+ * reference, vlimit clamp, frozen atoms held.  One step is
+ *   k_md_drift -> [pair-list rebuild] -> mme (bonded + 1-4 + GB) ->
+ *   k_md_kick -> k_md_finish
+ * with no host synchronisation; the two variants (with / without the
+ * rebuild) are captured once as hipGraphs and replayed.  Noise: one N(0,1)
+ * per (step, coordinate) from a counter-based hash (splitmix64 mixing of
+ * seed, step and index, then Box-Muller), the same function as the host
+ * nabc_md_gauss().
+ */
+
+__device__ __forceinline__ unsigned long long md_mix(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double md_gauss(unsigned long long seed, long long step, long long idx)
+{
+    const unsigned long long h =
+        md_mix(seed ^ md_mix((unsigned long long)step * 0x100000001B3ULL ^ md_mix((unsigned long long)idx)));
+    const unsigned long long h2 = md_mix(h);
+    const double u1 = ((double)(h >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+    const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+struct MDPar {
+    double dtx, dt5, c_implic, c_explic, c_ave, sdfac, gammai, ekin0, dttp, vlimit, rndf, boltz2;
+    unsigned long long seed;
+    int n3, log_cap;
+};
+
+__device__ __forceinline__ double md_tscal(const MDPar &P, const double *scal)
+{
+    const double ekin = scal[0];
+    return ekin > 0.01 ? sqrt(1.0 + P.dttp * (P.ekin0 / ekin - 1.0)) : 1.0;
+}
+
+/* initial velocities (sff.c:2165-2188) and kinetic energy */
+__global__ __launch_bounds__(MM_THREADS) void k_md_init_v(MDState M, MDPar P, const int *frz, double tempi,
+                                                          int zerov, int keep)
+{
+    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
+    double e = 0.0;
+    if (i < P.n3) {
+        double v = M.v[i];
+        if (zerov) {
+            v = 0.0;
+        } else if (!keep) {
+            if (frz[i / 3]) {
+                v = 0.0;
+            } else {
+                v = sqrt(2.0 * P.boltz2 * tempi * M.minv[i]) * md_gauss(P.seed, -1, i);
+                e = v * v / M.minv[i];
+            }
+        } else {
+            /* the reference sums v^2 * minv here (sff.c:2186) */
+            e = v * v * M.minv[i];
+        }
+        M.v[i] = v;
+    }
+    block_add(e, &M.scal[1]);
+}
+
+__global__ void k_md_init_done(MDState M)
+{
+    M.scal[0] = 0.5 * M.scal[1];
+    M.scal[1] = 0.0;
+    *M.step = 1;
+}
+
+/* first accelerations after the step-0 force (sff.c:2199-2215) */
+__global__ __launch_bounds__(MM_THREADS) void k_md_accel0(MDState M, MDPar P, const int *frz, const double *g)
+{
+    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
+    if (i >= P.n3) return;
+    if (P.gammai == 0.0) {
+        M.accel[i] = -g[i] * M.minv[i] * P.dt5;
+    } else if (frz[i / 3]) {
+        M.accel[i] = 0.0;
+    } else {
+        M.sqrmass[i] = 1.0 / sqrt(M.minv[i]);
+        M.accel[i] = (-g[i] + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, 0, i)) * M.minv[i] * P.dt5;
+    }
+}
+
+/* half kick + drift (sff.c:2225-2249) */
+__global__ __launch_bounds__(MM_THREADS) void k_md_drift(MDState M, MDPar P, double *x)
+{
+    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
+    if (i == 0) M.scal[1] = 0.0;
+    if (i >= P.n3) return;
+    double v;
+    if (P.gammai == 0.0) v = (M.v[i] + M.accel[i]) * md_tscal(P, M.scal);
+    else v = P.c_explic * M.v[i] + M.accel[i];
+    v = fmin(P.vlimit, fmax(-P.vlimit, v));
+    M.v[i] = v;
+    x[i] += v * P.dtx;
+}
+
+/* second half kick with the new forces (sff.c:2262-2277) and 2*ekin */
+__global__ __launch_bounds__(MM_THREADS) void k_md_kick(MDState M, MDPar P, const int *frz, const double *g)
+{
+    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
+    double e = 0.0;
+    if (i < P.n3) {
+        double v = M.v[i];
+        if (P.gammai == 0.0) {
+            const double a = -g[i] * M.minv[i] * P.dt5;
+            M.accel[i] = a;
+            v = (v + a) * md_tscal(P, M.scal);
+        } else if (!frz[i / 3]) {
+            const double a = (-g[i] + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, *M.step, i)) * M.minv[i] * P.dt5;
+            M.accel[i] = a;
+            v = (v + a) * P.c_implic;
+        }
+        M.v[i] = v;
+        e = v * v / M.minv[i];
+    }
+    block_add(e, &M.scal[1]);
+}
+
+/* kinetic energy, temperature, per-step log row, step counter (sff.c:2294-2339) */
+__global__ void k_md_finish(MDState M, MDPar P, const double *ene14)
+{
+    double ekin = 0.5 * M.scal[1];
+    if (P.gammai > 0.0) ekin *= P.c_ave;
+    const int s = *M.step;
+    double *row = M.log + 16 * (size_t)((s - 1) % P.log_cap);
+    for (int k = 0; k < 14; k++) row[k] = ene14[k];
+    row[14] = ekin;
+    row[15] = ekin / (P.boltz2 * P.rndf);
+    M.scal[0] = ekin;
+    *M.step = s + 1;
+}
+
+static MDPar md_par(const MDState &M, int n3)
+{
+    MDPar P;
+    P.dtx = M.dtx; P.dt5 = M.dt5; P.c_implic = M.c_implic; P.c_explic = M.c_explic; P.c_ave = M.c_ave;
+    P.sdfac = M.sdfac; P.gammai = M.gammai; P.ekin0 = M.ekin0; P.dttp = M.dttp; P.vlimit = M.vlimit;
+    P.rndf = M.rndf; P.boltz2 = M.boltz2; P.seed = M.seed; P.n3 = n3; P.log_cap = M.log_cap;
+    return P;
+}
+
+/* one dynamics step, launched on the engine stream (also used under capture) */
+static int md_step_launch(gbhip_ctx *c, bool rebuild)
+{
+    Engine &E = c->E;
+    MDState &M = c->M;
+    const int n3 = 3 * E.natoms;
+    const MDPar P = md_par(M, n3);
+    hipLaunchKernelGGL(k_md_drift, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, E.x_dev);
+    LAUNCHCHK();
+    int rc;
+    if (rebuild && (rc = engine_nblist(E))) return rc;
+    if ((rc = mme_device(c, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
+    hipLaunchKernelGGL(k_md_kick, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, c->B.frozen,
+                       (const double *)E.g_dev);
+    hipLaunchKernelGGL(k_md_finish, dim3(1), dim3(1), 0, E.stream, M, P, (const double *)E.ene14_dev);
+    LAUNCHCHK();
+    return 0;
+}
+
+static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t *ge)
+{
+    Engine &E = c->E;
+    const int saved_cap = E.tev_cap;
+    E.tev_cap = 0; /* no timing events inside a graph */
+    HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    int rc = md_step_launch(c, rebuild);
+    hipError_t e2 = hipStreamEndCapture(E.stream, g);
+    E.tev_cap = saved_cap;
+    if (rc) return rc;
+    if (e2 != hipSuccess) return seterr(GBHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    HIPCHK(hipGraphInstantiate(ge, *g, nullptr, nullptr, 0));
+    return 0;
+}
+
+extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double *x, const double *v0,
+                             const gbhip_md_options *o, int log_cap)
+{
+    if (!ctx || !masses || !x || !o) return seterr(GBHIP_EINVAL, "NULL argument");
+    Engine &E = ctx->E;
+    MDState &M = ctx->M;
+    if (!ctx->B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded must precede gbhip_md_init");
+    if (log_cap < 1 || o->dt <= 0 || o->nsnb < 1) return seterr(GBHIP_EINVAL, "bad dynamics options");
+    const int n = E.natoms, n3 = 3 * n;
+    int rc;
+    if (!E.stream) {
+        /* graph capture needs a stream of our own (default priority) */
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        E.stream = s;
+        E.own_stream = true;
+    }
+    if (!M.v) {
+        if ((rc = E.dalloc(&M.v, n3)) || (rc = E.dalloc(&M.accel, n3)) || (rc = E.dalloc(&M.minv, n3)) ||
+            (rc = E.dalloc(&M.sqrmass, n3)) || (rc = E.dalloc(&M.scal, 2)) || (rc = E.dalloc(&M.step, 1)))
+            return rc;
+    }
+    if (M.log_cap != log_cap) {
+        if ((rc = E.dalloc(&M.log, 16 * (size_t)log_cap))) return rc;
+        M.log_cap = log_cap;
+    }
+    std::vector<double> minv(n3), zero(n3, 0.0);
+    for (int a = 0; a < n; a++) {
+        if (!(masses[a] > 0)) return seterr(GBHIP_EINVAL, "atom %d has mass %g", a + 1, masses[a]);
+        minv[3 * a] = minv[3 * a + 1] = minv[3 * a + 2] = 1.0 / masses[a];
+    }
+    int nfrozen = 0;
+    for (int s = 0; s < E.nslots; s++)
+        if (E.perm_h[s] >= 0 && E.frz_h[s]) nfrozen++;
+    HIPCHK(hipMemcpyAsync(M.minv, minv.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(M.v, v0 ? v0 : zero.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(M.sqrmass, zero.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(E.x_dev, x, sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
+    const double dtx = o->dt * 20.455;
+    M.dtx = dtx;
+    M.dt5 = 0.5 * dtx;
+    M.rndf = n3 - 3.0 * nfrozen;
+    M.boltz2 = o->boltz2;
+    M.ekin0 = o->boltz2 * M.rndf * o->temp0;
+    M.dttp = o->dt / o->tautp;
+    M.gammai = o->gamma_ln / 20.455;
+    M.c_implic = 1.0 / (1.0 + M.gammai * M.dt5);
+    M.c_explic = 1.0 - M.gammai * M.dt5;
+    M.c_ave = 1.0 + M.gammai * M.dt5;
+    M.sdfac = sqrt(4.0 * M.gammai * o->boltz2 * o->temp0 / dtx);
+    M.vlimit = o->vlimit;
+    M.nsnb = o->nsnb;
+    M.seed = o->seed;
+    M.steps_done = 0;
+    const MDPar P = md_par(M, n3);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(MM_THREADS), 0, E.stream, M.scal, 2LL, 0.0);
+    hipLaunchKernelGGL(k_md_init_v, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, ctx->B.frozen,
+                       o->tempi, o->zerov, (o->tempi > 0.0 || o->zerov) ? 0 : 1);
+    hipLaunchKernelGGL(k_md_init_done, dim3(1), dim3(1), 0, E.stream, M);
+    LAUNCHCHK();
+    /* step 0: list build and forces (md() calls mme with iter 0) */
+    if ((rc = engine_nblist(E))) return rc;
+    if ((rc = mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
+    hipLaunchKernelGGL(k_md_accel0, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, ctx->B.frozen,
+                       (const double *)E.g_dev);
+    LAUNCHCHK();
+    if (M.g_step) { (void)hipGraphExecDestroy(M.g_step); M.g_step = nullptr; }
+    if (M.g_rebuild) { (void)hipGraphExecDestroy(M.g_rebuild); M.g_rebuild = nullptr; }
+    if (M.gr_step) { (void)hipGraphDestroy(M.gr_step); M.gr_step = nullptr; }
+    if (M.gr_rebuild) { (void)hipGraphDestroy(M.gr_rebuild); M.gr_rebuild = nullptr; }
+    HIPCHK(hipStreamSynchronize(E.stream));
+    M.ready = true;
+    return 0;
+}
+
+extern "C" int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    MDState &M = ctx->M;
+    Engine &E = ctx->E;
+    if (!M.ready) return seterr(GBHIP_ESTATE, "gbhip_md_init has not been called");
+    int rc;
+    if (use_graph && !M.g_step) {
+        if ((rc = md_capture(ctx, false, &M.gr_step, &M.g_step))) return rc;
+        if ((rc = md_capture(ctx, true, &M.gr_rebuild, &M.g_rebuild))) return rc;
+    }
+    for (int k = 0; k < nsteps; k++) {
+        const int step = M.steps_done + 1;
+        const bool rebuild = step % M.nsnb == 0; /* mme34: iter % nsnb == 0 (eff.c:2564) */
+        if (use_graph) HIPCHK(hipGraphLaunch(rebuild ? M.g_rebuild : M.g_step, E.stream));
+        else if ((rc = md_step_launch(ctx, rebuild))) return rc;
+        M.steps_done = step;
+    }
+    return 0;
+}
+
+extern "C" int gbhip_md_state(gbhip_ctx *ctx, double *x, double *v, double *grad)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    Engine &E = ctx->E;
+    MDState &M = ctx->M;
+    if (!M.ready) return seterr(GBHIP_ESTATE, "gbhip_md_init has not been called");
+    const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
+    if (x) HIPCHK(hipMemcpyAsync(x, E.x_dev, nb, hipMemcpyDeviceToHost, E.stream));
+    if (v) HIPCHK(hipMemcpyAsync(v, M.v, nb, hipMemcpyDeviceToHost, E.stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, E.g_dev, nb, hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
+extern "C" int gbhip_md_log(gbhip_ctx *ctx, double *rows, int max_rows, int *nrows)
+{
+    if (!ctx || !rows || !nrows) return seterr(GBHIP_EINVAL, "NULL argument");
+    MDState &M = ctx->M;
+    Engine &E = ctx->E;
+    const int n = std::min(std::min(M.steps_done, M.log_cap), max_rows);
+    /* rows of the last n steps in step order */
+    std::vector<double> all(16 * (size_t)M.log_cap);
+    HIPCHK(hipMemcpyAsync(all.data(), M.log, all.size() * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    for (int r = 0; r < n; r++) {
+        const int step = M.steps_done - n + 1 + r;
+        const double *src = all.data() + 16 * (size_t)((step - 1) % M.log_cap);
+        for (int k = 0; k < 16; k++) rows[16 * r + k] = src[k];
+    }
+    *nrows = n;
+    return 0;
+}

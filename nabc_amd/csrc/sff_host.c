@@ -338,6 +338,16 @@ static int ensure_ctx(const double *x)
     return 0;
 }
 
+void *mme_context(double *x)
+{
+    if (!S.prm) {
+        nabc_seterr("mme_context: mme_init_sff has not been called");
+        return NULL;
+    }
+    if (ensure_ctx(x) != 0) return NULL;
+    return S.ctx;
+}
+
 void mme_get_energies(double ene[13], double *frms)
 {
     if (ene) memcpy(ene, S.ene, sizeof(double) * 13);
@@ -434,6 +444,55 @@ double nabc_md_gauss(unsigned long long seed, long long step, long long idx)
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+/*
+ * md() over this library's own mme(): the whole loop runs on the device
+ * (gbhip_md_*), each step a hipGraph launch; the md: lines are printed
+ * from the device's per-step log afterwards.
+ */
+static int md_device(int n, int maxstep, double *x, double *f, double *v)
+{
+    const nabc_opts_t *o = &nabc_opts;
+    gbhip_md_options mo;
+    int cap = maxstep > 0 ? maxstep : 1, rows = 0, k, it;
+    double *log;
+    if (ensure_ctx(x) != 0) return -1;
+    memset(&mo, 0, sizeof(mo));
+    mo.dt = o->dt;
+    mo.temp0 = o->temp0;
+    mo.tempi = o->tempi;
+    mo.gamma_ln = o->gamma_ln;
+    mo.tautp = o->tautp;
+    mo.vlimit = o->vlimit;
+    mo.boltz2 = o->boltz2;
+    mo.nsnb = o->nsnb > 0 ? o->nsnb : 1;
+    mo.zerov = o->zerov;
+    mo.seed = o->md_seed;
+    if (cap > (1 << 20)) cap = 1 << 20;
+    if (gbhip_md_init(S.ctx, S.prm->Masses, x, v, &mo, cap) != 0 || gbhip_md_run(S.ctx, maxstep, 1) != 0 ||
+        gbhip_md_state(S.ctx, x, v, f) != 0) {
+        nabc_seterr("%s", gbhip_last_error());
+        return -1;
+    }
+    S.have_list = 1;
+    log = (double *)malloc(sizeof(double) * 16 * (size_t)cap);
+    if (gbhip_md_log(S.ctx, log, cap, &rows) == 0) {
+        for (k = 0; k < rows; k++) {
+            const int step = maxstep - rows + 1 + k;
+            const double *r = log + 16 * (size_t)k;
+            if (step % o->ntpr_md == 0 || step == 1)
+                fprintf(out(), "md:       %5d %10.3f %10.2f %10.2f %10.2f %10.2f\n", step, o->t + step * o->dt,
+                        r[14], r[0], r[14] + r[0], r[15]);
+        }
+        if (rows > 0) memcpy(S.ene, log + 16 * (size_t)(rows - 1), sizeof(double) * 14);
+    }
+    free(log);
+    fflush(out());
+    nabc_opts.t += maxstep * o->dt;
+    it = -3;
+    mme(x, f, &it);
+    return 0;
+}
+
 /* md(), sff.c:2071-2366 (no rattle, no COM removal, no trajectory output) */
 int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(double *, double *, int *))
 {
@@ -447,6 +506,7 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
         nabc_seterr("md: mme_init_sff has not been called");
         return -1;
     }
+    if (func == mme && n == 3 * S.prm->Natom) return md_device(n, maxstep, x, f, v);
     minv = (double *)calloc(n, sizeof(double));
     accel = (double *)calloc(n, sizeof(double));
     sqrmass = (double *)calloc(n, sizeof(double));
