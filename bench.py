@@ -109,17 +109,129 @@ def cpu_baseline(parm, natom, x, budget_s=15.0):
                        (n, NSNB, threads, cpu or "cpu model unknown")), ref
 
 
+def bench_batch(args):
+    """Batched scoring: 1,024 molecules, N log-uniform on [200, 3000], seeded
+    per instance, igb=5, no cutoff, one evaluation per molecule in one launch
+    per pass; every instance checked against the oracle."""
+    import numpy as np
+    import nabc_amd as na
+    import oracle
+    nmol = args.nmol
+    rng = np.random.default_rng(4242)
+    sizes = [int(v) for v in np.exp(rng.uniform(np.log(200), np.log(3000), nmol))]
+    t0 = time.perf_counter()
+    systems = [na.build_system(n, 100000 + k) for k, n in enumerate(sizes)]
+    t_gen = time.perf_counter() - t0
+    res = {}
+    for name, prec in (("fp64", na.GBHIP_FP64), ("mixed", na.GBHIP_MIXED)):
+        b = na.Batch([(a, x) for _, a, x in systems], gb=5, precision=prec)
+        npairs, natoms, ntp = b.stats()
+        for _ in range(args.warmup):
+            b.eval_device()
+        ene, grads = b.eval()
+        b.timing_start(args.steps)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.eval_device()
+        ene2, _ = b.eval()
+        t1 = time.perf_counter()
+        ms, n = b.timing_stop()
+        dt = (t1 - t0) / (args.steps + 1)
+        res[name] = dict(batch_ms=1e3 * dt, molecules_per_s=nmol / dt, pairs_per_s=npairs / dt,
+                         pass_ms={k: round(v / n, 4) for k, v in zip(("born", "radii", "pairs", "chain"), ms)},
+                         ene=ene, grads=grads, npairs=npairs, natoms=natoms, tilepairs=ntp)
+        b.close()
+    # per-instance check against the oracle
+    worst = dict(fp64_e=0.0, fp64_g=0.0, mixed_e=0.0, mixed_g=0.0)
+    t0 = time.perf_counter()
+    for k, (p, a, x) in enumerate(systems):
+        o = oracle.Oracle(p, a["natom"], gb=5, cut=999.0, rgbmax=999.0)
+        o.nblist(x)
+        r = o.egb(x)
+        for name in ("fp64", "mixed"):
+            e = res[name]["ene"][k]
+            g = res[name]["grads"][k]
+            worst[name + "_e"] = max(worst[name + "_e"], max(abs(e[c] - r[c]) / abs(r[c]) for c in range(3)))
+            worst[name + "_g"] = max(worst[name + "_g"],
+                                     float(np.sqrt(np.mean((g - r[3]) ** 2) / np.mean(r[3] ** 2))))
+        o.close()
+    t_check = time.perf_counter() - t0
+    out = {"metric": "batched GB energy+grad, molecules/s (1,024 molecules, N log-uniform 200-3000)",
+           "value": round(res["mixed"]["molecules_per_s"], 1), "unit": "molecules/s",
+           "fp64_molecules_per_s": round(res["fp64"]["molecules_per_s"], 1),
+           "mixed_vs_fp64": round(res["mixed"]["molecules_per_s"] / res["fp64"]["molecules_per_s"], 3),
+           "batch_ms": {k: round(res[k]["batch_ms"], 3) for k in res},
+           "pair_interactions_per_s": {k: float("%.4g" % res[k]["pairs_per_s"]) for k in res},
+           "pass_ms": {k: res[k]["pass_ms"] for k in res},
+           "npairs": res["mixed"]["npairs"], "natoms": res["mixed"]["natoms"],
+           "tile_pairs": res["mixed"]["tilepairs"], "nmol": nmol,
+           "worst_instance_error_vs_oracle": worst, "generate_s": round(t_gen, 1),
+           "oracle_check_s": round(t_check, 1)}
+    print(json.dumps(out))
+
+
+def bench_md(args):
+    """Dynamics through the host API: md() over mme() with every step a
+    hipGraph launch (bonded + 1-4 + GB on the device)."""
+    import numpy as np
+    import nabc_amd as na
+    natom, opts = {"c1": (600, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
+                   "c2": (2500, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
+                   "c3": (25000, "gb=8, cut=25.0, rgbmax=25.0, kappa=0.127315, nsnb=20")}[args.md]
+    parm, arr, x0 = na.build_system(natom, SEED)
+    out = {"metric": "MD steps/s (%s, %d atoms)" % (args.md, natom), "unit": "steps/s"}
+    for name, prec in (("fp64", "fp64"), ("mixed", "mixed")):
+        na.lib.mm_reset_options()
+        assert na.lib.mm_options(("%s, gbprec=%s, dt=0.001, temp0=300, tempi=300, gamma_ln=5.0, ntpr=1000000, "
+                                  "ntpr_md=1000000" % (opts, prec)).encode()) == 0
+        assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+        ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+        masses = arr["mass"]
+        o = na.MDOptions(dt=0.001, temp0=300.0, tempi=300.0, gamma_ln=5.0, tautp=999999.0, vlimit=10.0,
+                         boltz2=9.93595e-4, nsnb=int(opts.split("nsnb=")[1]), zerov=0, seed=SEED)
+        assert na.lib.gbhip_md_init(ctx, masses.ctypes.data_as(na.dp), x0.ctypes.data_as(na.dp), None,
+                                    na.C.byref(o), args.md_steps) == 0
+        assert na.lib.gbhip_md_run(ctx, 20, 1) == 0
+        assert na.lib.gbhip_md_state(ctx, None, None, None) == 0
+        t0 = time.perf_counter()
+        assert na.lib.gbhip_md_run(ctx, args.md_steps, 1) == 0
+        assert na.lib.gbhip_md_state(ctx, None, None, None) == 0
+        t1 = time.perf_counter()
+        rows = np.zeros((args.md_steps, 16))
+        nr = na.C.c_int()
+        na.lib.gbhip_md_log(ctx, rows.ctypes.data_as(na.dp), args.md_steps, na.C.byref(nr))
+        out[name] = {"steps_per_s": round(args.md_steps / (t1 - t0), 1),
+                     "ms_per_step": round(1e3 * (t1 - t0) / args.md_steps, 4),
+                     "ns_per_day": round(args.md_steps / (t1 - t0) * 0.001 * 86400 / 1000, 2),
+                     "final_temp": round(float(rows[-200:, 15].mean()), 1),
+                     "final_epot": float(rows[-1, 0])}
+        it = na.C.c_int(-3)
+        na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+    out["value"] = out["mixed"]["steps_per_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md"],
+                    help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
+                         "md: dynamics (see --md)")
+    ap.add_argument("--nmol", type=int, default=1024)
+    ap.add_argument("--md", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--md-steps", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "batch":
+        return bench_batch(args)
+    if args.config == "md":
+        return bench_md(args)
 
     import numpy as np
     import nabc_amd as na

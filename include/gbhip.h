@@ -64,6 +64,8 @@ typedef struct {
     const double *gbbeta;    /*   may be NULL for gb 1                       */
     const double *gbgamma;
     const int *neckidx;      /* gb 7/8: (int)((rborn-1)*20+0.5), else NULL */
+    double fsmax;            /* prm->Fsmax as read from the prmtop (prm.c:1652-1656):
+                                pairs beyond rgbmax + fsmax skip passes 1 and 3 */
 } gbhip_topology;
 
 typedef struct {
@@ -71,7 +73,6 @@ typedef struct {
     int precision;       /* GBHIP_FP64 or GBHIP_MIXED */
     double cut;          /* pair-list cutoff (A) */
     double rgbmax;       /* Born-integral cutoff (A), must be <= cut */
-    double fsmax;        /* prm->Fsmax as read from the prmtop */
     double gboffset;     /* 0.09; 0.195141 for gb 8 */
     double gbneckscale;  /* gb 7: 0.361825, gb 8: 0.826836 */
     double kappa;        /* Debye-Hueckel inverse length (1/A) */

@@ -37,12 +37,12 @@ class Topology(C.Structure):
     _fields_ = [("natom", C.c_int), ("ntypes", C.c_int), ("charge", dp), ("rborn", dp),
                 ("fs", dp), ("iac", ip), ("cno", ip), ("cn1", dp), ("cn2", dp),
                 ("iblo", ip), ("exclat", ip), ("frozen", ip), ("gbalpha", dp),
-                ("gbbeta", dp), ("gbgamma", dp), ("neckidx", ip)]
+                ("gbbeta", dp), ("gbgamma", dp), ("neckidx", ip), ("fsmax", C.c_double)]
 
 
 class Options(C.Structure):
     _fields_ = [("gb", C.c_int), ("precision", C.c_int), ("cut", C.c_double),
-                ("rgbmax", C.c_double), ("fsmax", C.c_double), ("gboffset", C.c_double),
+                ("rgbmax", C.c_double), ("gboffset", C.c_double),
                 ("gbneckscale", C.c_double), ("kappa", C.c_double), ("epsext", C.c_double)]
 
 
@@ -201,6 +201,16 @@ def parm_arrays(p):
                 atomic_num=g(s.AtomicNum, n, np.int32))
 
 
+def build_system(natom, seed):
+    """Generate a synthetic system in memory: (PARMSTRUCT_T pointer, arrays, x)."""
+    p = ParmP()
+    x = np.zeros(3 * natom)
+    rc = lib.gbgen_build(natom, seed, C.byref(p), _p(x))
+    if rc != 0:
+        raise RuntimeError(lib.mm_get_error().decode(errors="replace"))
+    return p, parm_arrays(p), x
+
+
 def read_rst7(path, natom):
     x = np.zeros(3 * natom)
     v = np.zeros(3 * natom)
@@ -293,13 +303,14 @@ def make_topology(keep, top, gb, frozen=None, atomic_num=None):
     t.gbbeta = _p(keep.arr(b, np.float64))
     t.gbgamma = _p(keep.arr(g, np.float64))
     t.neckidx = _p(keep.arr(neck, np.int32), ip)
+    t.fsmax = top["fsmax"]
     return t, gboffset, neckscale
 
 
-def make_options(gb, cut, rgbmax, kappa, epsext, fsmax, gboffset, neckscale, precision=GBHIP_FP64):
+def make_options(gb, cut, rgbmax, kappa, epsext, gboffset, neckscale, precision=GBHIP_FP64):
     o = Options()
     o.gb, o.precision, o.cut, o.rgbmax = gb, precision, cut, rgbmax
-    o.fsmax, o.gboffset, o.gbneckscale = fsmax, gboffset, neckscale
+    o.gboffset, o.gbneckscale = gboffset, neckscale
     o.kappa, o.epsext = kappa, epsext
     return o
 
@@ -311,7 +322,7 @@ class GBContext:
                  precision=GBHIP_FP64, frozen=None, atomic_num=None):
         self._keep = _Keep()
         t, gboffset, neckscale = make_topology(self._keep, top, gb, frozen, atomic_num)
-        o = make_options(gb, cut, rgbmax, kappa, epsext, top["fsmax"], gboffset, neckscale, precision)
+        o = make_options(gb, cut, rgbmax, kappa, epsext, gboffset, neckscale, precision)
         self.natom = top["natom"]
         x = np.ascontiguousarray(x, dtype=np.float64)
         h = vp()
@@ -384,6 +395,16 @@ class GBContext:
         _check(lib.gbhip_timing_stop(self.h, _p(ms), C.byref(n)))
         return ms, n.value
 
+    def md_init(self, masses, x, v0=None, dt=0.001, temp0=300.0, tempi=0.0, gamma_ln=0.0,
+                tautp=999999.0, vlimit=10.0, nsnb=10, zerov=0, seed=71277, log_cap=1024):
+        o = MDOptions(dt=dt, temp0=temp0, tempi=tempi, gamma_ln=gamma_ln, tautp=tautp,
+                      vlimit=vlimit, boltz2=9.93595e-4, nsnb=nsnb, zerov=zerov, seed=seed)
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+        _check(lib.gbhip_md_init(self.h, _p(m), _p(x), _p(v) if v is not None else None,
+                                 C.byref(o), log_cap))
+
     def pairs(self):
         """Current list as a sorted (P, 2) array of (min, max) atom indices."""
         cap = self.npairs()
@@ -394,3 +415,63 @@ class GBContext:
         assert n.value == cap
         p = np.stack([np.minimum(pi[:cap], pj[:cap]), np.maximum(pi[:cap], pj[:cap])], axis=1)
         return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+
+class Batch:
+    """Many molecules scored in one launch per pass (gbhip_batch_*)."""
+
+    def __init__(self, systems, gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, epsext=78.5,
+                 precision=GBHIP_FP64):
+        """systems: list of (arrays, x) with arrays as returned by parm_arrays."""
+        self._keep = _Keep()
+        n = len(systems)
+        tops = (Topology * n)()
+        xs = (dp * n)()
+        self.natoms = []
+        gboffset = neckscale = None
+        for k, (arr, x) in enumerate(systems):
+            t, gboffset, neckscale = make_topology(self._keep, arr, gb)
+            tops[k] = t
+            xs[k] = _p(self._keep.arr(x, np.float64))
+            self.natoms.append(arr["natom"])
+        o = make_options(gb, cut, rgbmax, kappa, epsext, gboffset, neckscale, precision)
+        h = vp()
+        _check(lib.gbhip_batch_create(n, tops, C.byref(o), xs, C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gbhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def eval(self, grads=True):
+        """Returns (ene[n,3] = epol, evdw, eelec per molecule, list of gradients)."""
+        ene = np.zeros(3 * self.n)
+        gl = [np.zeros(3 * m) for m in self.natoms] if grads else None
+        gp = (dp * self.n)(*[_p(g) for g in gl]) if grads else None
+        _check(lib.gbhip_batch_eval(self.h, _p(ene), gp))
+        return ene.reshape(self.n, 3), gl
+
+    def eval_device(self):
+        _check(lib.gbhip_batch_eval_device(self.h))
+
+    def nblist(self):
+        _check(lib.gbhip_batch_nblist(self.h))
+
+    def stats(self):
+        p, a, t = C.c_longlong(), C.c_longlong(), C.c_int()
+        _check(lib.gbhip_batch_stats(self.h, C.byref(p), C.byref(a), C.byref(t)))
+        return p.value, a.value, t.value
+
+    def timing_start(self, max_evals):
+        _check(lib.gbhip_batch_timing_start(self.h, max_evals))
+
+    def timing_stop(self):
+        ms = np.zeros(4)
+        n = C.c_int()
+        _check(lib.gbhip_batch_timing_stop(self.h, _p(ms), C.byref(n)))
+        return ms, n.value

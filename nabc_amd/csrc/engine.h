@@ -36,6 +36,7 @@ struct DevPtrs {
     const int *perm;      /* slot -> global atom index, -1 = padding */
     const int *slot_mol;  /* slot -> molecule */
     const double *q, *rb, *ri0, *sj, *ga, *gbb, *gg;
+    const double *rgbpsmax2;  /* per molecule: (rgbmax + Fsmax)^2 */
     const float *q_f, *ri0_f, *sj_f, *rb_f;
     const int *neck, *frz, *ljrow, *ljcol;
     double *xs, *ys, *zs;
@@ -64,7 +65,7 @@ struct DevPtrs {
 
 struct GBPar {
     int gb, has_neck, nslots, nmol;
-    double cut2, rgbmax, rgbmax1i, rgbmax2i, rgbmaxpsmax2;
+    double cut2, rgbmax, rgbmax1i, rgbmax2i;
     double neckscale, kappa, epsext;
 };
 

@@ -46,6 +46,22 @@ template <> struct gbm<float> {
 };
 
 /*
+ * FP64 1/sqrt(r2) for the Coulomb/LJ terms: exact division of the FP64
+ * kernel, or (mixed kernel) the FP32 hardware estimate refined by two
+ * Newton steps y <- y (3 - r2 y^2)/2 in FP64, ~1 ulp of FP64.
+ */
+template <typename R> __device__ __forceinline__ double rsqrt_pair(double r2);
+template <> __device__ __forceinline__ double rsqrt_pair<double>(double r2) { return 1.0 / sqrt(r2); }
+template <> __device__ __forceinline__ double rsqrt_pair<float>(double r2)
+{
+    double y = (double)rsqrtf((float)r2);
+    const double h = 0.5 * r2;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+
+/*
  * Pass-1 descreening of an atom with offset radius ri (ri1i = 1/ri) by a
  * sphere of scaled radius sj at distance dij (r2 = dij^2, dij1i = 1/dij).
  * skip is set when dij > rgbmax + sj (the pair is dropped, neck included).

@@ -297,6 +297,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const int2 ij = d.tp[t];
         const int i = ij.x * TILE + l, j = ij.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const int ni = d.neck[i];
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             if (!((m >> b) & 1ull)) continue;
             const double dx = xi - s_x[b], dy = yi - s_y[b], dz = zi - s_z[b];
             const double r2d = dx * dx + dy * dy + dz * dz;
-            if (r2d > p.rgbmaxpsmax2) continue;
+            if (r2d > rp2) continue;
             const R r2 = (R)r2d;
             const R dij1i = gbm<R>::rsqrt_(r2);
             const R dij = r2 * dij1i;
@@ -397,14 +398,14 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
 {
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE], s_sd[TILE];
-    __shared__ R s_q[TILE], s_r[TILE];
+    __shared__ double s_q[TILE];
+    __shared__ R s_r[TILE];
     __shared__ int s_lj[TILE];
     const int l = threadIdx.x;
     const int ntp = *d.ntp;
     const R epsinv = (R)(1.0 / p.epsext);
     const R kks = (R)(-GB_KSCALE * p.kappa);
     const bool screened = p.kappa != 0.0;
-    const R *A = lja<R>(d), *B = ljb<R>(d);
     double epol = 0.0, eel = 0.0, evdw = 0.0;
     int cur_mol = -1;
     for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
@@ -428,13 +429,14 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             cur_mol = mol;
         }
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
-        const R qi = ldq<R>(d, i), ri = (R)d.reff[i];
+        const double qi = d.q[i];
+        const R ri = (R)d.reff[i];
         const int row = d.ljrow[i];
         const unsigned long long m = d.mask[(size_t)t * TILE + l];
         const unsigned long long e = d.excl[(size_t)t * TILE + l];
         __syncthreads();
         s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
-        s_q[l] = ldq<R>(d, j); s_r[l] = (R)d.reff[j];
+        s_q[l] = d.q[j]; s_r[l] = (R)d.reff[j];
         s_lj[l] = d.ljcol[j];
         s_fx[l] = 0.0; s_fy[l] = 0.0; s_fz[l] = 0.0; s_sd[l] = 0.0;
         __syncthreads();
@@ -443,8 +445,10 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             const int b = (l + s) & (TILE - 1);
             if (!((m >> b) & 1ull)) continue;
             const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
-            const R r2 = (R)(dxd * dxd + dyd * dyd + dzd * dzd);
-            const R qiqj = qi * s_q[b];
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            const R r2 = (R)r2d;
+            const double qiqj_d = qi * s_q[b];
+            const R qiqj = (R)qiqj_d;
             const R rj = s_r[b];
             const R rb2 = ri * rj;
             const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
@@ -465,20 +469,24 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
             sdi += (double)(ri * temp5);
             s_sd[b] += (double)(rj * temp5);
+            double ded = (double)de;
             if (!((e >> b) & 1ull)) {
-                const R rinv = gbm<R>::rsqrt_(r2);
-                const R r2inv = rinv * rinv;
-                const R el = qiqj * rinv;
-                eel += (double)el;
-                de -= el * r2inv;
-                const R r6inv = r2inv * r2inv * r2inv;
+                /* Coulomb and LJ in FP64 in both variants: their molecular
+                 * totals cancel strongly (sums of +-100 kcal/mol terms), so
+                 * FP32 terms cannot hold 1e-6 of the total; they cost a few
+                 * FP64 operations next to the FP32 transcendentals above */
+                const double rinv = rsqrt_pair<R>(r2d);
+                const double r2inv = rinv * rinv;
+                const double el = qiqj_d * rinv;
+                eel += el;
+                ded -= el * r2inv;
+                const double r6inv = r2inv * r2inv * r2inv;
                 const int k = row + s_lj[b];
-                const R f6 = B[k] * r6inv;
-                const R f12 = A[k] * r6inv * r6inv;
-                evdw += (double)(f12 - f6);
-                de -= (R(12) * f12 - R(6) * f6) * r2inv;
+                const double f6 = d.ljb[k] * r6inv;
+                const double f12 = d.lja[k] * r6inv * r6inv;
+                evdw += f12 - f6;
+                ded -= (12.0 * f12 - 6.0 * f6) * r2inv;
             }
-            const double ded = (double)de;
             const double gxv = ded * dxd, gyv = ded * dyd, gzv = ded * dzd;
             fxi += gxv; fyi += gyv; fzi += gzv;
             s_fx[b] -= gxv; s_fy[b] -= gyv; s_fz[b] -= gzv;
@@ -534,6 +542,7 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
         const int2 ij = d.tp[t];
         const int i = ij.x * TILE + l, j = ij.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
@@ -552,7 +561,7 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             if (!((m >> b) & 1ull)) continue;
             const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
             const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
-            if (r2d > p.rgbmaxpsmax2) continue;
+            if (r2d > rp2) continue;
             const R r2 = (R)r2d;
             const R dij1i = gbm<R>::rsqrt_(r2);
             const R dij = r2 * dij1i;
@@ -734,7 +743,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     std::vector<double> q(NS, 0.0), rb(NS, 1.0), ri0(NS, 1.0), sj(NS, 0.0), ga(NS, 0.0), gbb(NS, 0.0), gg(NS, 0.0);
     std::vector<int> tile_mol(NT), tile_end(NT);
     std::vector<long long> rowbase(NT);
-    std::vector<double> lja, ljb;
+    std::vector<double> lja, ljb, rgbpsmax2(nmol);
     std::vector<int> ex_ptr(E.natoms + 1, 0), ex_j;
     long long cap = 0;
 
@@ -744,6 +753,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         std::vector<int> order;
         morton_order(n, x0 ? x0[m] : nullptr, order);
         const int lj0 = (int)lja.size(), nt = t.ntypes;
+        rgbpsmax2[m] = (o->rgbmax + t.fsmax) * (o->rgbmax + t.fsmax);
         for (int a = 0; a < nt; a++)
             for (int b = 0; b < nt; b++) {
                 int ic = t.cno[a * nt + b] - 1;
@@ -802,7 +812,6 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     P.rgbmax = o->rgbmax;
     P.rgbmax1i = 1.0 / o->rgbmax;
     P.rgbmax2i = P.rgbmax1i * P.rgbmax1i;
-    P.rgbmaxpsmax2 = (o->rgbmax + o->fsmax) * (o->rgbmax + o->fsmax);
     P.neckscale = o->gbneckscale;
     P.kappa = o->kappa;
     P.epsext = o->epsext;
@@ -818,7 +827,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.upload(&D.tile_mol, tile_mol)) || (rc = E.upload(&D.tile_end, tile_end)) ||
         (rc = E.upload(&D.rowbase, rowbase)) || (rc = E.upload(&D.ex_ptr, ex_ptr)) ||
         (rc = E.upload(&D.ex_j, ex_j)) || (rc = E.upload(&D.inv, inv)) ||
-        (rc = E.upload(&D.lja, lja)) || (rc = E.upload(&D.ljb, ljb)))
+        (rc = E.upload(&D.lja, lja)) || (rc = E.upload(&D.ljb, ljb)) ||
+        (rc = E.upload(&D.rgbpsmax2, rgbpsmax2)))
         return rc;
     {
         std::vector<float> qf(q.begin(), q.end()), rf(ri0.begin(), ri0.end()), sf(sj.begin(), sj.end()),

@@ -312,7 +312,7 @@ static int ensure_ctx(const double *x)
     op.precision = o->gbprec ? GBHIP_MIXED : GBHIP_FP64;
     op.cut = o->cut;
     op.rgbmax = o->rgbmax;
-    op.fsmax = p->Fsmax;
+    t.fsmax = p->Fsmax;
     op.gboffset = S.gboffset;
     op.gbneckscale = S.gbneckscale;
     op.kappa = o->kappa;
