@@ -211,15 +211,65 @@ def bench_md(args):
     print(json.dumps(out))
 
 
+def bench_serve(args):
+    """Serving: Poisson arrivals at a fixed rate of jobs on generated
+    molecules of 300-8000 atoms (log-uniform), each 50 CG iterations or 500
+    MD steps (Bernoulli 1/2), igb=5, 20 A cutoff; continuous admission by
+    free slot and HBM footprint.  Reports throughput, p50/p99 latency and the
+    mixed-vs-FP64 energy error at each job's final coordinates."""
+    import numpy as np
+    import nabc_amd as na
+    rng = np.random.default_rng(55)
+    n = args.jobs
+    sizes = np.exp(rng.uniform(np.log(300), np.log(8000), n)).astype(int)
+    kinds = rng.integers(0, 2, n)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    t0 = time.perf_counter()
+    jobs = []
+    for k in range(n):
+        p, a, x = na.build_system(int(sizes[k]), 700000 + k)
+        jobs.append(dict(prm=p, x=x, kind=int(kinds[k]), nsteps=50 if kinds[k] == 0 else 500,
+                         seed=k + 1, arrival_s=float(arrivals[k])))
+    t_gen = time.perf_counter() - t0
+    opts = ("gb=5, cut=20.0, rgbmax=20.0, nsnb=10, dt=0.001, gamma_ln=5.0, temp0=300, tempi=300, "
+            "ntpr=100000000")
+    res = na.serve(jobs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), precision=na.GBHIP_MIXED,
+                   check_fp64=True, mm_opts=opts, cg_rmsgrad=1e-4, cg_dfpred=10.0)
+    ok = [r for r in res if r["status"] >= -3]
+    lat = np.array([r["latency_s"] for r in ok])
+    span = max(r["end_s"] for r in ok) - min(j["arrival_s"] for j in jobs)
+    err = np.array([abs(r["energy_final_eval"] - r["energy_fp64"]) / abs(r["energy_fp64"]) for r in ok
+                    if r["energy_fp64"] != 0.0])
+    lat_cg = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 0 and r["status"] >= -3])
+    lat_md = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 1 and r["status"] >= -3])
+    out = {"metric": "serving p99 latency at a fixed request rate (CG/MD jobs, 300-8000 atoms)",
+           "value": round(float(np.percentile(lat, 99)), 4), "unit": "s", "higher_is_better": False,
+           "request_rate_per_s": args.rate, "jobs": n, "completed": len(ok), "slots": args.slots,
+           "hbm_budget_gb": args.hbm_gb, "throughput_jobs_per_s": round(len(ok) / span, 2),
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "p50_cg_s": round(float(np.percentile(lat_cg, 50)), 4) if len(lat_cg) else None,
+           "p50_md_s": round(float(np.percentile(lat_md, 50)), 4) if len(lat_md) else None,
+           "max_resident_jobs": max(r["max_active"] for r in res),
+           "mixed_vs_fp64_final_energy_rel_err": {"median": float(np.median(err)), "max": float(err.max())},
+           "atoms_mean": float(sizes.mean()), "generate_s": round(t_gen, 1),
+           "job_mix": "Bernoulli(1/2) of CG maxiter=50 / MD 500 steps, seed 55"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md"],
+    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
-                         "md: dynamics (see --md)")
+                         "md: dynamics (see --md); serve: request loop")
+    ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=10.0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--hbm-gb", type=float, default=64.0)
     ap.add_argument("--nmol", type=int, default=1024)
     ap.add_argument("--md", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--md-steps", type=int, default=5000)
@@ -232,6 +282,8 @@ def main():
         return bench_batch(args)
     if args.config == "md":
         return bench_md(args)
+    if args.config == "serve":
+        return bench_serve(args)
 
     import numpy as np
     import nabc_amd as na

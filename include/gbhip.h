@@ -94,6 +94,14 @@ int gbhip_device_count(void);
 int gbhip_ctx_create(const gbhip_topology *top, const gbhip_options *opt,
                      const double *x0, gbhip_ctx **out);
 
+/*
+ * Same, with all of the context's work (set-up copies included) on the
+ * given hipStream_t, for callers that create contexts while other threads
+ * capture graphs on their own streams.
+ */
+int gbhip_ctx_create_on(const gbhip_topology *top, const gbhip_options *opt,
+                        const double *x0, void *stream, gbhip_ctx **out);
+
 /* Release everything (egb(..., freevectors=1), eff.c:1463-1491). */
 int gbhip_ctx_destroy(gbhip_ctx *ctx);
 

@@ -92,6 +92,26 @@ class Bonded(C.Structure):
                 ("charge", C.c_void_p), ("dield", C.c_int)]
 
 
+class ServeJob(C.Structure):
+    _fields_ = [("prm", C.c_void_p), ("x", dp), ("kind", C.c_int), ("nsteps", C.c_int),
+                ("seed", C.c_ulonglong), ("arrival_s", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
+                ("check_fp64", C.c_int), ("cg_rmsgrad", C.c_double), ("cg_dfpred", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("start_s", C.c_double), ("end_s", C.c_double), ("latency_s", C.c_double),
+                ("energy", C.c_double), ("energy_fp64", C.c_double),
+                ("energy_final_eval", C.c_double), ("footprint", C.c_size_t),
+                ("status", C.c_int), ("slot", C.c_int), ("max_active", C.c_int)]
+
+
+GBSERVE_CG = 0
+GBSERVE_MD = 1
+
 ParmP = C.POINTER(Parm)
 MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
 
@@ -153,6 +173,32 @@ _sig("mme_init_sff", C.c_int, [ParmP, ip, ip, dp, C.c_char_p])
 _sig("mme", C.c_double, [dp, dp, ip])
 _sig("mme_get_energies", None, [dp, dp])
 _sig("mme_context", vp, [dp])
+_sig("gbhip_footprint_bytes", C.c_size_t, [C.c_int, C.c_int])
+_sig("gbserve_run", C.c_int, [C.POINTER(ServeJob), C.c_int, C.POINTER(ServeConfig), C.c_char_p,
+                              C.POINTER(ServeResult)])
+
+
+def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=False,
+          mm_opts="gb=5, cut=20.0, rgbmax=20.0", cg_rmsgrad=1e-4, cg_dfpred=10.0):
+    """Run jobs through the serving loop.
+
+    jobs: list of dicts with prm (PARMSTRUCT_T pointer), x, kind
+    (GBSERVE_CG / GBSERVE_MD), nsteps, seed, arrival_s.
+    Returns a list of result dicts (times in seconds from the run start)."""
+    n = len(jobs)
+    arr = (ServeJob * n)()
+    keep = []
+    for k, j in enumerate(jobs):
+        x = np.ascontiguousarray(j["x"], dtype=np.float64)
+        keep.append(x)
+        arr[k] = ServeJob(C.cast(j["prm"], C.c_void_p), _p(x), j["kind"], j["nsteps"],
+                          j.get("seed", k + 1), j["arrival_s"])
+    cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred)
+    res = (ServeResult * n)()
+    rc = lib.gbserve_run(arr, n, C.byref(cfg), mm_opts.encode() if mm_opts else None, res)
+    if rc != 0:
+        raise GBHipError(lib.mm_get_error().decode(errors="replace"))
+    return [{f[0]: getattr(r, f[0]) for f in ServeResult._fields_} for r in res]
 _sig("md", C.c_int, [C.c_int, C.c_int, dp, dp, dp, MMEFUNC])
 _sig("conjgrad", C.c_int, [dp, ip, dp, MMEFUNC, dp, dp, ip])
 _sig("nabc_md_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])

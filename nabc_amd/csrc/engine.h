@@ -8,11 +8,23 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <vector>
 
 #include "gbhip.h"
 
 #define TILE 64
+
+/*
+ * Held while a stream is captured into a graph and while contexts allocate
+ * or free device memory: allocation calls synchronise implicitly, which
+ * another thread's capture in progress does not allow.
+ */
+inline std::recursive_mutex &device_alloc_lock()
+{
+    static std::recursive_mutex m;
+    return m;
+}
 
 int seterr(int code, const char *fmt, ...);
 
@@ -119,7 +131,10 @@ struct Engine {
         T *p;
         int rc = dalloc(&p, src.size());
         if (rc) return rc;
-        HIPCHK(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+        /* on the engine's own stream: a legacy-stream copy would have to
+         * order against other threads' streams, some of them capturing */
+        HIPCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         *dst = p;
         return 0;
     }

@@ -853,14 +853,15 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&E.g_dev, 3 * (size_t)E.natoms)) || (rc = E.dalloc(&E.tmp_dev, (size_t)E.natoms)) ||
         (rc = E.dalloc(&E.ene14_dev, 14)))
         return rc;
-    HIPCHK(hipMemset(D.ntp, 0, sizeof(int)));
-    HIPCHK(hipMemset(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms));
+    HIPCHK(hipMemsetAsync(D.ntp, 0, sizeof(int), E.stream));
+    HIPCHK(hipMemsetAsync(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms, E.stream));
     if (x0) {
         for (int m = 0; m < nmol; m++)
             if (x0[m])
-                HIPCHK(hipMemcpy(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x0[m],
-                                 3 * sizeof(double) * (size_t)E.mol_natom[m], hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpyAsync(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x0[m],
+                                      3 * sizeof(double) * (size_t)E.mol_natom[m], hipMemcpyHostToDevice, E.stream));
     }
+    HIPCHK(hipStreamSynchronize(E.stream));
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1003,11 +1004,19 @@ extern "C" int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals)
 extern "C" int gbhip_ctx_create(const gbhip_topology *top, const gbhip_options *opt, const double *x0,
                                 gbhip_ctx **out)
 {
+    return gbhip_ctx_create_on(top, opt, x0, nullptr, out);
+}
+
+extern "C" int gbhip_ctx_create_on(const gbhip_topology *top, const gbhip_options *opt, const double *x0,
+                                   void *stream, gbhip_ctx **out)
+{
     if (!out) return seterr(GBHIP_EINVAL, "out pointer is NULL");
     *out = nullptr;
     gbhip_ctx *c = new (std::nothrow) gbhip_ctx;
     if (!c) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    c->E.stream = (hipStream_t)stream;
     const double *xs[1] = {x0};
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     int rc = engine_build(c->E, 1, top, opt, xs);
     if (rc) {
         delete c;
@@ -1021,7 +1030,8 @@ extern "C" int gbhip_ctx_destroy(gbhip_ctx *ctx)
 {
     if (!ctx) return 0;
     if (ctx->E.stream) (void)hipStreamSynchronize(ctx->E.stream);
-    else (void)hipDeviceSynchronize();
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    if (!ctx->E.stream) (void)hipDeviceSynchronize();
     delete ctx;
     return 0;
 }
@@ -1203,6 +1213,7 @@ extern "C" int gbhip_batch_create(int nmol, const gbhip_topology *tops, const gb
     *out = nullptr;
     gbhip_batch *b = new (std::nothrow) gbhip_batch;
     if (!b) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     int rc = engine_build(b->E, nmol, tops, opt, x0);
     if (!rc) rc = engine_nblist(b->E);
     if (!rc && hipStreamSynchronize(b->E.stream) != hipSuccess) rc = seterr(GBHIP_EHIP, "pair-list build failed");
@@ -1217,6 +1228,7 @@ extern "C" int gbhip_batch_create(int nmol, const gbhip_topology *tops, const gb
 extern "C" int gbhip_batch_destroy(gbhip_batch *b)
 {
     if (!b) return 0;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     (void)hipDeviceSynchronize();
     delete b;
     return 0;

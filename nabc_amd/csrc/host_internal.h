@@ -33,6 +33,26 @@ typedef struct {
 extern nabc_opts_t nabc_opts;
 extern FILE *nabc_out;
 
+/* what mme_init_sff derives from a topology (model.c) */
+typedef struct {
+    PARMSTRUCT_T *prm;
+    int *frozen;
+    int nfrozen, gb, dield;
+    double gboffset, gbneckscale;
+    double *alpha, *beta, *gamma;
+    int *neckidx;
+    /* bonded tables: 0-based atom indices, one entry per Fourier term */
+    int nb, na, nd, n14;
+    int *bi, *bj, *ai, *aj, *ak, *di, *dj, *dk, *dl, *pi, *pj, *phb;
+    double *brk, *breq, *atk, *ateq, *dpk, *dpn, *dph, *pscee, *pscnb, *pa, *pb;
+} nabc_model;
+
+struct gbhip_ctx;
+int nabc_model_build(nabc_model *m, PARMSTRUCT_T *prm, const int *frozen, const nabc_opts_t *o);
+void nabc_model_free(nabc_model *m);
+int nabc_model_context(const nabc_model *m, const double *x, const nabc_opts_t *o, void *stream,
+                       struct gbhip_ctx **out);
+
 void nabc_seterr(const char *fmt, ...);
 int nabc_atomic_number_from_mass(double mass);
 void nabc_build_14_list(PARMSTRUCT_T *prm);

@@ -271,6 +271,7 @@ __global__ __launch_bounds__(MM_THREADS) void k_finish(BondedDev B, const double
 extern "C" int gbhip_set_bonded(gbhip_ctx *ctx, const gbhip_bonded *b)
 {
     if (!ctx || !b) return seterr(GBHIP_EINVAL, "NULL argument");
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     Engine &E = ctx->E;
     BondedDev &B = ctx->B;
     if (E.nmol != 1) return seterr(GBHIP_EINVAL, "bonded terms need a single-molecule context");
@@ -545,6 +546,7 @@ static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t 
     Engine &E = c->E;
     const int saved_cap = E.tev_cap;
     E.tev_cap = 0; /* no timing events inside a graph */
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
     int rc = md_step_launch(c, rebuild);
     hipError_t e2 = hipStreamEndCapture(E.stream, g);
@@ -572,14 +574,17 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
         E.stream = s;
         E.own_stream = true;
     }
-    if (!M.v) {
-        if ((rc = E.dalloc(&M.v, n3)) || (rc = E.dalloc(&M.accel, n3)) || (rc = E.dalloc(&M.minv, n3)) ||
-            (rc = E.dalloc(&M.sqrmass, n3)) || (rc = E.dalloc(&M.scal, 2)) || (rc = E.dalloc(&M.step, 1)))
-            return rc;
-    }
-    if (M.log_cap != log_cap) {
-        if ((rc = E.dalloc(&M.log, 16 * (size_t)log_cap))) return rc;
-        M.log_cap = log_cap;
+    {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        if (!M.v) {
+            if ((rc = E.dalloc(&M.v, n3)) || (rc = E.dalloc(&M.accel, n3)) || (rc = E.dalloc(&M.minv, n3)) ||
+                (rc = E.dalloc(&M.sqrmass, n3)) || (rc = E.dalloc(&M.scal, 2)) || (rc = E.dalloc(&M.step, 1)))
+                return rc;
+        }
+        if (M.log_cap != log_cap) {
+            if ((rc = E.dalloc(&M.log, 16 * (size_t)log_cap))) return rc;
+            M.log_cap = log_cap;
+        }
     }
     std::vector<double> minv(n3), zero(n3, 0.0);
     for (int a = 0; a < n; a++) {

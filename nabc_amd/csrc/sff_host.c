@@ -24,40 +24,14 @@
 #include "host_internal.h"
 
 static struct {
-    PARMSTRUCT_T *prm;
-    int *frozen;
-    int nfrozen;
-    int gb;
-    double gboffset, gbneckscale;
-    double *alpha, *beta, *gamma;
-    int *neckidx;
-    gbhip_ctx *ctx;
+    nabc_model m;       /* GB parameters and bonded tables (model.c) */
+    gbhip_ctx *ctx;     /* created at the first mme() call */
     int have_list;
     long long nb_pairs;
-    double ene[14];
-    /* bonded tables (0-based atom indices, one entry per Fourier term) */
-    int nb, na, nd, n14;
-    int *bi, *bj, *ai, *aj, *ak, *di, *dj, *dk, *dl, *pi, *pj, *phb;
-    double *brk, *breq, *atk, *ateq, *dpk, *dpn, *dph, *pscee, *pscnb, *pa, *pb;
+    double ene[14];     /* last mme(): mme34 energies + rms gradient */
 } S;
 
 static FILE *out(void) { return nabc_out ? nabc_out : stdout; }
-
-static void free_tables(void)
-{
-    free(S.alpha); free(S.beta); free(S.gamma); free(S.neckidx);
-    free(S.bi); free(S.bj); free(S.brk); free(S.breq);
-    free(S.ai); free(S.aj); free(S.ak); free(S.atk); free(S.ateq);
-    free(S.di); free(S.dj); free(S.dk); free(S.dl); free(S.dpk); free(S.dpn); free(S.dph);
-    free(S.pi); free(S.pj); free(S.phb); free(S.pscee); free(S.pscnb); free(S.pa); free(S.pb);
-    free(S.frozen);
-    S.alpha = S.beta = S.gamma = NULL;
-    S.neckidx = NULL;
-    S.bi = S.bj = S.ai = S.aj = S.ak = S.di = S.dj = S.dk = S.dl = S.pi = S.pj = S.phb = NULL;
-    S.brk = S.breq = S.atk = S.ateq = S.dpk = S.dpn = S.dph = NULL;
-    S.pscee = S.pscnb = S.pa = S.pb = NULL;
-    S.frozen = NULL;
-}
 
 static void release_ctx(void)
 {
@@ -66,281 +40,28 @@ static void release_ctx(void)
     S.have_list = 0;
 }
 
-/* GB parameter set-up of mme_init_sff, sff.c:1187-1333 */
-static int gb_setup(PARMSTRUCT_T *prm, int gb)
-{
-    int n = prm->Natom, i;
-    S.alpha = (double *)calloc(n, sizeof(double));
-    S.beta = (double *)calloc(n, sizeof(double));
-    S.gamma = (double *)calloc(n, sizeof(double));
-    S.neckidx = (int *)calloc(n, sizeof(int));
-    S.gboffset = 0.09;
-    S.gbneckscale = 0.0;
-    for (i = 0; i < n; i++) {
-        double a = 0, b = 0, g = 0;
-        switch (gb) {
-        case 2: a = 0.8; b = 0.0; g = 2.909125; break;
-        case 5: a = 1.0; b = 0.8; g = 4.85; break;
-        case 7:
-            a = 1.09511284; b = 1.90792938; g = 2.50798245;
-            switch (prm->AtomicNum[i]) {
-            case 1: prm->Fs[i] = 1.09085413633; break;
-            case 6: prm->Fs[i] = 0.484353823306; break;
-            case 7: prm->Fs[i] = 0.700147318409; break;
-            case 8: prm->Fs[i] = 1.06557401132; break;
-            case 16: prm->Fs[i] = 0.602256336067; break;
-            default: prm->Fs[i] = 0.5;
-            }
-            break;
-        case 8:
-            switch (prm->AtomicNum[i]) {
-            case 1: a = 0.788440; b = 0.798699; g = 0.437334; prm->Fs[i] = 1.425952; break;
-            case 6: a = 0.733756; b = 0.506378; g = 0.205844; prm->Fs[i] = 1.058554; break;
-            case 7: a = 0.503364; b = 0.316828; g = 0.192915; prm->Fs[i] = 0.733599; break;
-            case 8: a = 0.867814; b = 0.876635; g = 0.387882; prm->Fs[i] = 1.061039; break;
-            case 15: a = 1.0; b = 0.8; g = 4.85; prm->Fs[i] = 0.5; break;
-            case 16: a = 0.867814; b = 0.876635; g = 0.387882; prm->Fs[i] = -0.703469; break;
-            default: a = 1.0; b = 0.8; g = 4.85; prm->Fs[i] = 0.5;
-            }
-            break;
-        default: break;
-        }
-        S.alpha[i] = a;
-        S.beta[i] = b;
-        S.gamma[i] = g;
-        if (gb == 7 || gb == 8) {
-            S.neckidx[i] = (int)((prm->Rborn[i] - 1.0) * 20.0 + 0.5);
-            if (S.neckidx[i] < 0 || S.neckidx[i] > 20) {
-                nabc_seterr("Atom %d outside the allowed range of 1-2 Angstroms for igb==%d.", i, gb);
-                return -1;
-            }
-        }
-    }
-    if (gb == 7) S.gbneckscale = 0.361825;
-    if (gb == 8) {
-        S.gboffset = 0.195141;
-        S.gbneckscale = 0.826836;
-    }
-    return 0;
-}
-
-/* bonded tables for the device (eff.c:2617-2653 inputs) */
-static void bonded_setup(PARMSTRUCT_T *p)
-{
-    int k, m, pass;
-    S.nb = p->Nbonh + p->Mbona;
-    S.bi = (int *)malloc(sizeof(int) * (S.nb + 1));
-    S.bj = (int *)malloc(sizeof(int) * (S.nb + 1));
-    S.brk = (double *)malloc(sizeof(double) * (S.nb + 1));
-    S.breq = (double *)malloc(sizeof(double) * (S.nb + 1));
-    for (k = 0, m = 0; k < p->Nbonh; k++, m++) {
-        S.bi[m] = p->BondHAt1[k] / 3; S.bj[m] = p->BondHAt2[k] / 3;
-        S.brk[m] = p->Rk[p->BondHNum[k] - 1]; S.breq[m] = p->Req[p->BondHNum[k] - 1];
-    }
-    for (k = 0; k < p->Mbona; k++, m++) {
-        S.bi[m] = p->BondAt1[k] / 3; S.bj[m] = p->BondAt2[k] / 3;
-        S.brk[m] = p->Rk[p->BondNum[k] - 1]; S.breq[m] = p->Req[p->BondNum[k] - 1];
-    }
-    S.na = p->Ntheth + p->Ntheta;
-    S.ai = (int *)malloc(sizeof(int) * (S.na + 1));
-    S.aj = (int *)malloc(sizeof(int) * (S.na + 1));
-    S.ak = (int *)malloc(sizeof(int) * (S.na + 1));
-    S.atk = (double *)malloc(sizeof(double) * (S.na + 1));
-    S.ateq = (double *)malloc(sizeof(double) * (S.na + 1));
-    for (k = 0, m = 0; k < p->Ntheth; k++, m++) {
-        S.ai[m] = p->AngleHAt1[k] / 3; S.aj[m] = p->AngleHAt2[k] / 3; S.ak[m] = p->AngleHAt3[k] / 3;
-        S.atk[m] = p->Tk[p->AngleHNum[k] - 1]; S.ateq[m] = p->Teq[p->AngleHNum[k] - 1];
-    }
-    for (k = 0; k < p->Ntheta; k++, m++) {
-        S.ai[m] = p->AngleAt1[k] / 3; S.aj[m] = p->AngleAt2[k] / 3; S.ak[m] = p->AngleAt3[k] / 3;
-        S.atk[m] = p->Tk[p->AngleNum[k] - 1]; S.ateq[m] = p->Teq[p->AngleNum[k] - 1];
-    }
-    /* dihedrals: one entry per term of a Pn < 0 series (eff.c:619-623) */
-    {
-        int cap = 2 * (p->Nphih + p->Mphia) + 8;
-        S.di = (int *)malloc(sizeof(int) * cap); S.dj = (int *)malloc(sizeof(int) * cap);
-        S.dk = (int *)malloc(sizeof(int) * cap); S.dl = (int *)malloc(sizeof(int) * cap);
-        S.dpk = (double *)malloc(sizeof(double) * cap); S.dpn = (double *)malloc(sizeof(double) * cap);
-        S.dph = (double *)malloc(sizeof(double) * cap);
-        m = 0;
-        for (pass = 0; pass < 2; pass++) {
-            int nd = pass == 0 ? p->Nphih : p->Mphia;
-            int *a1 = pass == 0 ? p->DihHAt1 : p->DihAt1, *a2 = pass == 0 ? p->DihHAt2 : p->DihAt2;
-            int *a3 = pass == 0 ? p->DihHAt3 : p->DihAt3, *a4 = pass == 0 ? p->DihHAt4 : p->DihAt4;
-            int *tp = pass == 0 ? p->DihHNum : p->DihNum;
-            for (k = 0; k < nd; k++) {
-                int ty = tp[k] - 1;
-                for (;;) {
-                    if (m == cap) {
-                        cap *= 2;
-                        S.di = (int *)realloc(S.di, sizeof(int) * cap); S.dj = (int *)realloc(S.dj, sizeof(int) * cap);
-                        S.dk = (int *)realloc(S.dk, sizeof(int) * cap); S.dl = (int *)realloc(S.dl, sizeof(int) * cap);
-                        S.dpk = (double *)realloc(S.dpk, sizeof(double) * cap);
-                        S.dpn = (double *)realloc(S.dpn, sizeof(double) * cap);
-                        S.dph = (double *)realloc(S.dph, sizeof(double) * cap);
-                    }
-                    S.di[m] = a1[k] / 3; S.dj[m] = a2[k] / 3;
-                    S.dk[m] = abs(a3[k]) / 3; S.dl[m] = abs(a4[k]) / 3;
-                    S.dpk[m] = p->Pk[ty]; S.dpn[m] = p->Pn[ty]; S.dph[m] = p->Phase[ty];
-                    m++;
-                    if (p->Pn[ty] < 0.0 && ty + 1 < p->Nptra) ty++;
-                    else break;
-                }
-            }
-        }
-        S.nd = m;
-    }
-    /* 1-4 pairs (prm.c:1854-1889 list; LJ selection as eff.c:1209-1234) */
-    {
-        int i, e = 0;
-        S.n14 = 0;
-        for (i = 0; i < p->Natom - 1; i++) S.n14 += p->N14pairs[i];
-        S.pi = (int *)malloc(sizeof(int) * (S.n14 + 1)); S.pj = (int *)malloc(sizeof(int) * (S.n14 + 1));
-        S.phb = (int *)malloc(sizeof(int) * (S.n14 + 1));
-        S.pscee = (double *)malloc(sizeof(double) * (S.n14 + 1));
-        S.pscnb = (double *)malloc(sizeof(double) * (S.n14 + 1));
-        S.pa = (double *)malloc(sizeof(double) * (S.n14 + 1));
-        S.pb = (double *)malloc(sizeof(double) * (S.n14 + 1));
-        m = 0;
-        for (i = 0; i < p->Natom - 1; i++) {
-            for (k = 0; k < p->N14pairs[i]; k++, e++, m++) {
-                int j = p->N14pairlist[e];
-                int ic = p->Cno[p->Ntypes * (p->Iac[i] - 1) + p->Iac[j] - 1];
-                double fac = p->N14scnblist[e];
-                S.pi[m] = i; S.pj[m] = j;
-                S.pscee[m] = p->N14sceelist[e];
-                S.pscnb[m] = fac;
-                if (ic > 0 || fac != 1.0) {
-                    if (ic > 0) ic--;
-                    else {
-                        int big = p->Iac[i] > p->Iac[j] ? p->Iac[i] : p->Iac[j];
-                        int sml = p->Iac[i] > p->Iac[j] ? p->Iac[j] : p->Iac[i];
-                        ic = big * (big - 1) / 2 + sml - 1;
-                    }
-                    S.phb[m] = 0;
-                    S.pa[m] = p->Cn1[ic];
-                    S.pb[m] = p->Cn2[ic];
-                } else {
-                    int hc = -ic - 1;
-                    S.phb[m] = 1;
-                    S.pa[m] = p->HB12 ? p->HB12[hc] : 0.0;
-                    S.pb[m] = p->HB10 ? p->HB10[hc] : 0.0;
-                }
-            }
-        }
-    }
-}
-
 int mme_init_sff(PARMSTRUCT_T *prm, int *frozen, int *constrained, double *x0, char *ncname)
 {
-    const nabc_opts_t *o = &nabc_opts;
-    int i, n;
     (void)x0;
+    (void)constrained;
     release_ctx();
-    free_tables();
-    S.prm = NULL;
-    if (!prm) {
-        nabc_seterr("mme_init_sff: NULL topology");
-        return -1;
-    }
-    if (prm->IfBox && o->gb) {
-        nabc_seterr("Error: gb>0 is incompatible with periodic boundary conditions.");
-        return -1;
-    }
-    if (o->gb != 1 && o->gb != 2 && o->gb != 5 && o->gb != 7 && o->gb != 8) {
-        nabc_seterr("mme_init_sff: gb=%d is not provided by this library (GB models 1, 2, 5, 7, 8 only)", o->gb);
-        return -1;
-    }
-    if (o->gbsa || o->hcp || o->ips || o->dim != 3 || o->irattle || o->nscm || o->wdssp != 0.0) {
-        nabc_seterr("mme_init_sff: gbsa, hcp, ips, dim=4, rattle, nscm and wdssp are not provided by this library");
-        return -1;
-    }
+    nabc_model_free(&S.m);
     if (ncname) {
         nabc_seterr("mme_init_sff: netCDF trajectories are not provided by this library");
         return -1;
     }
-    if (o->dield != 1 && o->dield != 0) {
-        nabc_seterr("mme_init_sff: only diel=C and diel=R are provided");
-        return -1;
-    }
-    n = prm->Natom;
-    S.prm = prm;
-    S.gb = o->gb;
-    S.frozen = (int *)calloc(n, sizeof(int));
-    S.nfrozen = 0;
-    for (i = 0; i < n; i++) {
-        S.frozen[i] = frozen ? frozen[i] != 0 : 0;
-        S.nfrozen += S.frozen[i];
-    }
-    (void)constrained;
-    if (o->dielc != 1.0) {
-        double s = 1.0 / sqrt(o->dielc);
-        for (i = 0; i < n; i++) prm->Charges[i] *= s;
-    }
-    if (gb_setup(prm, o->gb) != 0) return -1;
-    bonded_setup(prm);
-    return 0;
+    return nabc_model_build(&S.m, prm, frozen, &nabc_opts);
 }
 
 static int ensure_ctx(const double *x)
 {
-    PARMSTRUCT_T *p = S.prm;
-    const nabc_opts_t *o = &nabc_opts;
-    gbhip_topology t;
-    gbhip_options op;
-    gbhip_bonded b;
     if (S.ctx) return 0;
-    memset(&t, 0, sizeof(t));
-    t.natom = p->Natom;
-    t.ntypes = p->Ntypes;
-    t.charge = p->Charges;
-    t.rborn = p->Rborn;
-    t.fs = p->Fs;
-    t.iac = p->Iac;
-    t.cno = p->Cno;
-    t.cn1 = p->Cn1;
-    t.cn2 = p->Cn2;
-    t.iblo = p->Iblo;
-    t.exclat = p->ExclAt;
-    t.frozen = S.frozen;
-    t.gbalpha = S.alpha;
-    t.gbbeta = S.beta;
-    t.gbgamma = S.gamma;
-    t.neckidx = S.neckidx;
-    memset(&op, 0, sizeof(op));
-    op.gb = S.gb;
-    op.precision = o->gbprec ? GBHIP_MIXED : GBHIP_FP64;
-    op.cut = o->cut;
-    op.rgbmax = o->rgbmax;
-    t.fsmax = p->Fsmax;
-    op.gboffset = S.gboffset;
-    op.gbneckscale = S.gbneckscale;
-    op.kappa = o->kappa;
-    op.epsext = o->epsext;
-    if (gbhip_ctx_create(&t, &op, x, &S.ctx) != 0) {
-        nabc_seterr("%s", gbhip_last_error());
-        return -1;
-    }
-    memset(&b, 0, sizeof(b));
-    b.nbond = S.nb; b.bond_i = S.bi; b.bond_j = S.bj; b.bond_rk = S.brk; b.bond_req = S.breq;
-    b.nangle = S.na; b.angle_i = S.ai; b.angle_j = S.aj; b.angle_k = S.ak; b.angle_tk = S.atk; b.angle_teq = S.ateq;
-    b.ndihed = S.nd; b.dihed_i = S.di; b.dihed_j = S.dj; b.dihed_k = S.dk; b.dihed_l = S.dl;
-    b.dihed_pk = S.dpk; b.dihed_pn = S.dpn; b.dihed_phase = S.dph;
-    b.npair14 = S.n14; b.p14_i = S.pi; b.p14_j = S.pj; b.p14_hb = S.phb;
-    b.p14_scee = S.pscee; b.p14_scnb = S.pscnb; b.p14_a = S.pa; b.p14_b = S.pb;
-    b.charge = p->Charges;
-    b.dield = o->dield;
-    if (gbhip_set_bonded(S.ctx, &b) != 0) {
-        nabc_seterr("%s", gbhip_last_error());
-        release_ctx();
-        return -1;
-    }
-    return 0;
+    return nabc_model_context(&S.m, x, &nabc_opts, NULL, &S.ctx);
 }
 
 void *mme_context(double *x)
 {
-    if (!S.prm) {
+    if (!S.m.prm) {
         nabc_seterr("mme_context: mme_init_sff has not been called");
         return NULL;
     }
@@ -363,7 +84,7 @@ double mme(double *x, double *f, int *iter)
         release_ctx();
         return 0.0;
     }
-    if (!S.prm) {
+    if (!S.m.prm) {
         nabc_seterr("mme: mme_init_sff has not been called");
         return NAN;
     }
@@ -387,11 +108,11 @@ double mme(double *x, double *f, int *iter)
         return NAN;
     }
     if (o->gb_debug) {
-        int i, n = S.prm->Natom;
+        int i, n = S.m.prm->Natom;
         double *reff = (double *)malloc(sizeof(double) * n);
         if (gbhip_get_reff(S.ctx, reff) == 0) {
             fprintf(out(), "Effective Born radii:\n");
-            for (i = 0; i < n; i++) fprintf(out(), "%d\t%15.7f\t%15.7f\n", i + 1, S.prm->Rborn[i], reff[i]);
+            for (i = 0; i < n; i++) fprintf(out(), "%d\t%15.7f\t%15.7f\n", i + 1, S.m.prm->Rborn[i], reff[i]);
         }
         free(reff);
     }
@@ -468,7 +189,7 @@ static int md_device(int n, int maxstep, double *x, double *f, double *v)
     mo.zerov = o->zerov;
     mo.seed = o->md_seed;
     if (cap > (1 << 20)) cap = 1 << 20;
-    if (gbhip_md_init(S.ctx, S.prm->Masses, x, v, &mo, cap) != 0 || gbhip_md_run(S.ctx, maxstep, 1) != 0 ||
+    if (gbhip_md_init(S.ctx, S.m.prm->Masses, x, v, &mo, cap) != 0 || gbhip_md_run(S.ctx, maxstep, 1) != 0 ||
         gbhip_md_state(S.ctx, x, v, f) != 0) {
         nabc_seterr("%s", gbhip_last_error());
         return -1;
@@ -502,15 +223,15 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
     double ekin = 0.0, epot, tscal, temp, t = o->t;
     int i, nstep, dim = 3;
     unsigned long long seed = o->md_seed;
-    if (!S.prm && func == mme) {
+    if (!S.m.prm && func == mme) {
         nabc_seterr("md: mme_init_sff has not been called");
         return -1;
     }
-    if (func == mme && n == 3 * S.prm->Natom) return md_device(n, maxstep, x, f, v);
+    if (func == mme && n == 3 * S.m.prm->Natom) return md_device(n, maxstep, x, f, v);
     minv = (double *)calloc(n, sizeof(double));
     accel = (double *)calloc(n, sizeof(double));
     sqrmass = (double *)calloc(n, sizeof(double));
-    rndf = n - 3 * S.nfrozen;
+    rndf = n - 3 * S.m.nfrozen;
     ekin0 = o->boltz2 * rndf * o->temp0;
     dttp = o->dt / o->tautp;
     gammai = o->gamma_ln / 20.455;
@@ -518,8 +239,8 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
     c_explic = 1.0 - gammai * dt5;
     c_ave = 1.0 + gammai * dt5;
     sdfac = sqrt(4.0 * gammai * o->boltz2 * o->temp0 / dtx);
-    if (S.prm) {
-        for (i = 0; i < n; i++) minv[i] = 1.0 / S.prm->Masses[i / dim];
+    if (S.m.prm) {
+        for (i = 0; i < n; i++) minv[i] = 1.0 / S.m.prm->Masses[i / dim];
     } else {
         for (i = 0; i < n; i++) minv[i] = 1.0 / o->genmass;
     }
@@ -528,7 +249,7 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
         ekin = 0.0;
     } else if (o->tempi > 0.0) {
         for (i = 0; i < n; i++) {
-            if (S.frozen && S.frozen[i / dim]) v[i] = 0.0;
+            if (S.m.frozen && S.m.frozen[i / dim]) v[i] = 0.0;
             else {
                 double sd = sqrt(2. * o->boltz2 * o->tempi * minv[i]);
                 v[i] = sd * nabc_md_gauss(seed, -1, i);
@@ -546,7 +267,7 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
         for (i = 0; i < n; i++) accel[i] = -f[i] * minv[i] * dt5;
     } else {
         for (i = 0; i < n; i++) {
-            if (S.frozen && S.frozen[i / dim]) accel[i] = 0.0;
+            if (S.m.frozen && S.m.frozen[i / dim]) accel[i] = 0.0;
             else {
                 sqrmass[i] = 1.0 / sqrt(minv[i]);
                 accel[i] = (-f[i] + sdfac * sqrmass[i] * nabc_md_gauss(seed, 0, i)) * minv[i] * dt5;
@@ -571,7 +292,7 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
             }
         } else {
             for (i = 0; i < n; i++) {
-                if (!(S.frozen && S.frozen[i / dim])) {
+                if (!(S.m.frozen && S.m.frozen[i / dim])) {
                     accel[i] = (-f[i] + sdfac * sqrmass[i] * nabc_md_gauss(seed, nstep, i)) * minv[i] * dt5;
                     v[i] = (v[i] + accel[i]) * c_implic;
                 }
