@@ -31,18 +31,44 @@
 
 template <typename R> struct gbm;
 
+/*
+ * FP64: exp/log from the device math library; reciprocal and reciprocal
+ * square root from the hardware estimates (v_rcp_f64 / v_rsq_f64, ~2^-26
+ * relative) refined by Newton steps to within an ulp or two of the IEEE
+ * result, instead of the correctly rounded division sequence.
+ */
 template <> struct gbm<double> {
     static __device__ __forceinline__ double log_(double x) { return log(x); }
     static __device__ __forceinline__ double exp_(double x) { return exp(x); }
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double rsqrt_(double x) { return 1.0 / sqrt(x); }
+    static __device__ __forceinline__ double rcp(double x)
+    {
+        double y = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+        e = fma(-x, y, 1.0);
+        return fma(y, e, y);
+    }
+    static __device__ __forceinline__ double rsqrt_(double x)
+    {
+        double y = __builtin_amdgcn_rsq(x);
+        const double h = 0.5 * x;
+        double e = fma(-h * y, y, 0.5);
+        y = fma(y, e, y);
+        e = fma(-h * y, y, 0.5);
+        return fma(y, e, y);
+    }
 };
 
+/*
+ * FP32: the hardware transcendental instructions (v_exp_f32, v_log_f32,
+ * v_rcp_f32, v_rsq_f32; ~1 ulp), every sum they feed is accumulated in
+ * FP64 by the caller.
+ */
 template <> struct gbm<float> {
-    static __device__ __forceinline__ float log_(float x) { return logf(x); }
-    static __device__ __forceinline__ float exp_(float x) { return expf(x); }
-    static __device__ __forceinline__ float rcp(float x) { return 1.0f / x; }
-    static __device__ __forceinline__ float rsqrt_(float x) { return rsqrtf(x); }
+    static __device__ __forceinline__ float log_(float x) { return __logf(x); }
+    static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 
 /*
