@@ -44,11 +44,13 @@ def sha256(path):
     return h.hexdigest()
 
 
-def run_gpu(na, arr, x, precision, steps, warmup):
+def run_gpu(na, arr, x, precision, steps, warmup, tune=True):
     ctx = na.GBContext(arr, x, gb=GB, cut=CUT, rgbmax=RGBMAX, kappa=KAPPA, precision=precision)
     ctx.nblist()
     npairs = ctx.npairs()
     ntiles, ntp = ctx.tile_stats()
+    # launch-shape autotuning is set-up, outside the timed region
+    launch = ctx.autotune(5) if tune else ctx.launch()
     for k in range(warmup):
         if k % NSNB == 0:
             ctx.nblist_async()
@@ -74,7 +76,7 @@ def run_gpu(na, arr, x, precision, steps, warmup):
     ctx.close()
     return dict(evals_per_s=steps / (t1 - t0), ms_per_eval=1e3 * (t1 - t0) / steps,
                 pass_ms=dict(born=ms[0] / n, radii=ms[1] / n, pairs=ms[2] / n, chain=ms[3] / n),
-                nblist_ms=1e3 * tl, npairs=npairs, ntiles=ntiles, tilepairs=ntp,
+                nblist_ms=1e3 * tl, npairs=npairs, ntiles=ntiles, tilepairs=ntp, launch=launch,
                 energies=dict(epol=epol, evdw=evdw, eelec=eel), grad=g)
 
 
@@ -263,6 +265,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop")
@@ -296,8 +299,8 @@ def main():
         arr = na.parm_arrays(parm)
         x, _ = na.read_rst7(rst, arr["natom"])
 
-    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup)
-    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup)
+    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune)
+    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune)
 
     # accuracy of the mixed kernels against the FP64 kernels (same list)
     acc = {k: abs(mixed["energies"][k] - fp64["energies"][k]) / abs(fp64["energies"][k])
@@ -333,7 +336,8 @@ def main():
         "config": {"workload": "GB energy+gradient, 25,000 atoms, igb=8, cut 25 A, rgbmax 25 A, "
                                "kappa 0.127315 (0.15 M), nsnb 20",
                    "natom": NATOM, "seed": SEED, "npairs": P, "tiles": mixed["ntiles"],
-                   "tile_pairs": mixed["tilepairs"], "sha256": digests},
+                   "tile_pairs": mixed["tilepairs"], "sha256": digests,
+                   "launch_blocks": {"mixed": mixed["launch"], "fp64": fp64["launch"]}},
         "fp64": {"value": round(fp64["evals_per_s"], 2), "ms_per_step": round(fp64["ms_per_eval"], 4),
                  "pass_ms": {k: round(v, 4) for k, v in fp64["pass_ms"].items()},
                  "dominant": "k_%s<double>" % dom64, "achieved_tflops": round(a64, 3),

@@ -157,6 +157,20 @@ int gbhip_egb_device(gbhip_ctx *ctx, double *d_grad, double *d_ene);
 int gbhip_timing_start(gbhip_ctx *ctx, int max_evals);
 int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
 
+/*
+ * Launch shape of the three persistent pair kernels (pass 1, pass 2,
+ * pass 3): the number of one-wavefront workgroups each is launched with.
+ * gbhip_set_launch takes blocks3[3] (0 keeps a value); gbhip_autotune times
+ * reps evaluations on the current list for each candidate size (4 to 40
+ * workgroups per CU) and keeps the fastest per pass, writing the choice to
+ * chosen3 (may be NULL).  MD graphs captured earlier keep the shapes they
+ * were captured with.  No counterpart in the reference (egb() has no launch
+ * configuration); nblist must have run.
+ */
+int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3);
+int gbhip_get_launch(gbhip_ctx *ctx, int *blocks3);
+int gbhip_autotune(gbhip_ctx *ctx, int reps, int *chosen3);
+
 /* Wait for the context's stream. */
 int gbhip_sync(gbhip_ctx *ctx);
 
@@ -281,6 +295,9 @@ int gbhip_batch_eval_device(gbhip_batch *b);
 /* Per-pass timing of batch evaluations (as gbhip_timing_start/stop). */
 int gbhip_batch_timing_start(gbhip_batch *b, int max_evals);
 int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals);
+
+/* Launch-shape autotuning of a batch (as gbhip_autotune). */
+int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3);
 
 /* Total unordered pairs and atoms of the batch. */
 int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms,

@@ -148,6 +148,10 @@ _sig("gbhip_md_state", C.c_int, [vp, dp, dp, dp])
 _sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_timing_stop", C.c_int, [vp, dp, ip])
+_sig("gbhip_set_launch", C.c_int, [vp, ip])
+_sig("gbhip_get_launch", C.c_int, [vp, ip])
+_sig("gbhip_autotune", C.c_int, [vp, C.c_int, ip])
+_sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
 _sig("gbhip_mme", C.c_int, [vp, dp, dp])
 _sig("gbhip_mme_device", C.c_int, [vp, dp, dp])
@@ -441,6 +445,22 @@ class GBContext:
         _check(lib.gbhip_timing_stop(self.h, _p(ms), C.byref(n)))
         return ms, n.value
 
+    def set_launch(self, born=0, pairs=0, chain=0):
+        """Workgroups of the three pair kernels (0 keeps a value)."""
+        b = np.array([born, pairs, chain], np.int32)
+        _check(lib.gbhip_set_launch(self.h, _p(b, ip)))
+
+    def launch(self):
+        b = np.zeros(3, np.int32)
+        _check(lib.gbhip_get_launch(self.h, _p(b, ip)))
+        return tuple(int(v) for v in b)
+
+    def autotune(self, reps=5):
+        """Pick the fastest persistent-grid size per pass; returns the choice."""
+        b = np.zeros(3, np.int32)
+        _check(lib.gbhip_autotune(self.h, reps, _p(b, ip)))
+        return tuple(int(v) for v in b)
+
     def md_init(self, masses, x, v0=None, dt=0.001, temp0=300.0, tempi=0.0, gamma_ln=0.0,
                 tautp=999999.0, vlimit=10.0, nsnb=10, zerov=0, seed=71277, log_cap=1024):
         o = MDOptions(dt=dt, temp0=temp0, tempi=tempi, gamma_ln=gamma_ln, tautp=tautp,
@@ -521,3 +541,8 @@ class Batch:
         n = C.c_int()
         _check(lib.gbhip_batch_timing_stop(self.h, _p(ms), C.byref(n)))
         return ms, n.value
+
+    def autotune(self, reps=3):
+        b = np.zeros(3, np.int32)
+        _check(lib.gbhip_batch_autotune(self.h, reps, _p(b, ip)))
+        return tuple(int(v) for v in b)

@@ -61,8 +61,14 @@ struct DevPtrs {
     const long long *rowbase;
     int *cand, *rowcnt, *rowstart;
     int *ntp;             /* device count of tile pairs */
-    int2 *tp;
-    unsigned long long *mask, *excl;
+    int2 *tp;             /* (I, J), I <= J, sorted by I then J */
+    int2 *tpo;            /* (A, B): lanes take tile A, steps walk tile B */
+    int *tflip;           /* 1 when (A, B) = (J, I) */
+    unsigned long long *tcol; /* non-empty columns (B atoms) of the bit matrix */
+    unsigned long long *mask, *excl; /* bit matrices in (A, B) orientation */
+    int *tord;            /* non-empty tile pairs, most non-empty columns first */
+    int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
+    int *nnz;             /* number of non-empty tile pairs */
     unsigned long long *npairs;
     double *bblo, *bbhi;  /* per tile bounding box */
     /* exclusions, CSR over global atoms, partners j > i */
@@ -85,7 +91,9 @@ struct Engine {
     int nmol = 0, natoms = 0, nslots = 0, ntiles = 0;
     long long tp_cap = 0;
     int precision = GBHIP_FP64;
-    int pair_blocks = 0;
+    int pair_blocks = 0;       /* default workgroups of the persistent pair kernels */
+    int pass_blocks[3] = {0, 0, 0}; /* per pass: born, pairs, chain */
+    int ncu = 0;
     GBPar par{};
     DevPtrs d{};
     hipStream_t stream = nullptr;

@@ -88,39 +88,85 @@ template <> __device__ __forceinline__ double rsqrt_pair<float>(double r2)
 }
 
 /*
+ * The five analytic cases of the Schaefer-Froemmel descreening integral
+ * (eff.c:1663-1697 and their derivatives eff.c:2289-2334), in the order the
+ * reference tests them:
+ *   A  dij > rgbmax - sj     integral cut at rgbmax
+ *   B  dij > 4 sj            Taylor series in (sj/dij)^2
+ *   C  dij > ri + sj         sphere j wholly outside ri
+ *   D  dij > |ri - sj|       partial overlap
+ *   E  ri < sj               atom i inside sphere j
+ *   -  otherwise (sphere j inside ri): no contribution
+ * Every case but B needs one logarithm and at most two reciprocals, so they
+ * are computed once, for every lane, before the case split: a wavefront
+ * whose lanes fall into several cases then issues one log and two rcp per
+ * pair instead of one set per case present.  The arguments per case are
+ *   u1 = 1/(dij + sj)                       (C, D, E)
+ *   u2 = 1/(dij - sj) for A, 1/(r2 - sj2)   (A, C, E)
+ *   L  = log((dij - sj)/rgbmax)             A
+ *        log(|dij - sj| / (dij + sj))       C, E
+ *        log(ri / (dij + sj))               D
+ */
+struct DsCase {
+    bool skip, a, b, c, d, e;
+};
+
+template <typename R>
+__device__ __forceinline__ DsCase ds_case(R ri, R sj, R dij, R rgbmax)
+{
+    DsCase k;
+    k.skip = dij > rgbmax + sj;
+    k.a = !k.skip && dij > rgbmax - sj;
+    const bool rest = !k.skip && !k.a;
+    k.b = rest && dij > R(4) * sj;
+    k.c = rest && !k.b && dij > ri + sj;
+    k.d = rest && !k.b && !k.c && dij > fabs(ri - sj);
+    k.e = rest && !k.b && !k.c && !k.d && ri < sj;
+    return k;
+}
+
+template <typename R>
+__device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R dij, R rgbmax1i,
+                                         R &u1, R &u2, R &L)
+{
+    typedef gbm<R> M;
+    const R dm = dij - sj;
+    u1 = M::rcp(dij + sj);
+    u2 = M::rcp(k.a ? dm : r2 - sj * sj);
+    const R arg = k.a ? dm * rgbmax1i : (k.d ? ri * u1 : fabs(dm) * u1);
+    L = M::log_(arg);
+}
+
+/*
  * Pass-1 descreening of an atom with offset radius ri (ri1i = 1/ri) by a
- * sphere of scaled radius sj at distance dij (r2 = dij^2, dij1i = 1/dij).
- * skip is set when dij > rgbmax + sj (the pair is dropped, neck included).
+ * sphere of scaled radius sj at distance dij (r2 = dij^2, dij1i = 1/dij);
+ * the amount egb() subtracts from sumi.  skip is set when dij > rgbmax + sj
+ * (the pair is dropped, neck included).
  */
 template <typename R>
 __device__ __forceinline__ R descreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
                                       R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
 {
-    typedef gbm<R> M;
+    const DsCase k = ds_case<R>(ri, sj, dij, rgbmax);
+    skip = k.skip;
+    R u1, u2, L;
+    ds_trans<R>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
     const R sj2 = sj * sj;
-    skip = dij > rgbmax + sj;
-    if (skip) return R(0);
-    if (dij > rgbmax - sj) {
-        R uij = M::rcp(dij - sj);
-        return R(0.125) * dij1i *
-               (R(1) + R(2) * dij * uij + rgbmax2i * (r2 - R(4) * rgbmax * dij - sj2) +
-                R(2) * M::log_((dij - sj) * rgbmax1i));
-    } else if (dij > R(4) * sj) {
-        R dij2i = dij1i * dij1i;
-        R tmpsd = sj2 * dij2i;
-        R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
-        return sj * tmpsd * dij2i * dumbo;
-    } else if (dij > ri + sj) {
-        return R(0.5) * (sj * M::rcp(r2 - sj2) + R(0.5) * dij1i * M::log_((dij - sj) * M::rcp(dij + sj)));
-    } else if (dij > fabs(ri - sj)) {
-        R theta = R(0.5) * ri1i * dij1i * (r2 + ri * ri - sj2);
-        R uij = M::rcp(dij + sj);
-        return R(0.25) * (ri1i * (R(2) - theta) - uij + dij1i * M::log_(ri * uij));
-    } else if (ri < sj) {
-        return R(0.5) * (sj * M::rcp(r2 - sj2) + R(2) * ri1i +
-                         R(0.5) * dij1i * M::log_((sj - dij) * M::rcp(sj + dij)));
+    R v = R(0);
+    if (k.a) {
+        v = R(0.125) * dij1i * (R(1) + R(2) * dij * u2 + rgbmax2i * (r2 - R(4) * rgbmax * dij - sj2) + R(2) * L);
+    } else if (k.b) {
+        const R dij2i = dij1i * dij1i;
+        const R tmpsd = sj2 * dij2i;
+        const R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
+        v = sj * tmpsd * dij2i * dumbo;
+    } else if (k.c || k.e) {
+        v = R(0.5) * (sj * u2 + R(0.5) * dij1i * L) + (k.e ? ri1i : R(0));
+    } else if (k.d) {
+        const R theta = R(0.5) * ri1i * dij1i * (r2 + ri * ri - sj2);
+        v = R(0.25) * (ri1i * (R(2) - theta) - u1 + dij1i * L);
     }
-    return R(0);
+    return v;
 }
 
 /* Neck contribution (subtracted from sumi) for neck maximum (pos, val). */
@@ -133,39 +179,34 @@ __device__ __forceinline__ R neck_term(R dij, R pos, R val, R neckscale)
     return neckscale * val * gbm<R>::rcp(R(1) + mdist2 + R(0.3) * mdist6);
 }
 
-/* Pass-3 derivative numerator datmp (eff.c:2289-2334). */
+/* Pass-3 derivative numerator datmp (eff.c:2289-2334), same case split. */
 template <typename R>
 __device__ __forceinline__ R ddescreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
-                                       R rgbmax, R rgbmax2i, bool &skip)
+                                       R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
 {
-    typedef gbm<R> M;
+    const DsCase k = ds_case<R>(ri, sj, dij, rgbmax);
+    skip = k.skip;
+    R u1, u2, L;
+    ds_trans<R>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
     const R sj2 = sj * sj;
     const R dij2i = dij1i * dij1i;
-    skip = dij > rgbmax + sj;
-    if (skip) return R(0);
-    if (dij > rgbmax - sj) {
-        R temp1 = M::rcp(dij - sj);
-        R dij3i = dij1i * dij2i;
-        return R(0.125) * dij3i * ((r2 + sj2) * (temp1 * temp1 - rgbmax2i) - R(2) * M::log_(rgbmax * temp1));
-    } else if (dij > R(4) * sj) {
-        R tmpsd = sj2 * dij2i;
-        R dumbo = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
-        return tmpsd * sj * dij2i * dij2i * dumbo;
-    } else if (dij > ri + sj) {
-        R temp1 = M::rcp(r2 - sj2);
-        return temp1 * sj * (R(-0.5) * dij2i + temp1) +
-               R(0.25) * dij1i * dij2i * M::log_((dij - sj) * M::rcp(dij + sj));
-    } else if (dij > fabs(ri - sj)) {
-        R temp1 = M::rcp(dij + sj);
-        R dij3i = dij2i * dij1i;
-        return R(-0.25) * (R(-0.5) * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i +
-                           dij1i * temp1 * (temp1 - dij1i) - dij3i * M::log_(ri * temp1));
-    } else if (ri < sj) {
-        R temp1 = M::rcp(r2 - sj2);
-        return R(-0.5) * (sj * dij2i * temp1 - R(2) * sj * temp1 * temp1 -
-                          R(0.5) * dij2i * dij1i * M::log_((sj - dij) * M::rcp(sj + dij)));
+    const R dij3i = dij2i * dij1i;
+    R v = R(0);
+    if (k.a) {
+        /* -2 log(rgbmax/(dij - sj)) = 2 L */
+        v = R(0.125) * dij3i * ((r2 + sj2) * (u2 * u2 - rgbmax2i) + R(2) * L);
+    } else if (k.b) {
+        const R tmpsd = sj2 * dij2i;
+        const R dumbo = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
+        v = tmpsd * sj * dij2i * dij2i * dumbo;
+    } else if (k.c) {
+        v = u2 * sj * (R(-0.5) * dij2i + u2) + R(0.25) * dij3i * L;
+    } else if (k.d) {
+        v = R(-0.25) * (R(-0.5) * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i + dij1i * u1 * (u1 - dij1i) - dij3i * L);
+    } else if (k.e) {
+        v = R(-0.5) * (sj * dij2i * u2 - R(2) * sj * u2 * u2 - R(0.5) * dij3i * L);
     }
-    return R(0);
+    return v;
 }
 
 /* Derivative of the neck term, divided by dij (eff.c:2336-2360). */

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(1024) void k_rowscan(DevPtrs d)
 {
     __shared__ int s_part[1024];
     const int n = d.ntiles, t = threadIdx.x;
+    if (t < 2 * TILE) d.tbin[t] = 0;
     const int per = (n + 1023) / 1024;
     const int b = t * per, e = min(n, b + per);
     int sum = 0;
@@ -197,9 +198,29 @@ __global__ __launch_bounds__(64) void k_masks(DevPtrs d, GBPar p)
                 w |= (unsigned long long)in << b;
             }
         }
+        cnt += __popcll(w);
+        /* orientation: the evaluation kernels step over the non-empty
+         * columns only, so the side with fewer non-empty atoms is walked
+         * and the other is held one atom per lane */
+        unsigned long long cols = w;
+        for (int off = 32; off > 0; off >>= 1) cols |= __shfl_xor(cols, off);
+        const unsigned long long rows = __ballot(w != 0ull);
+        const bool flip = __popcll(rows) < __popcll(cols);
+        if (flip) {
+            unsigned long long wt = 0ull;
+            for (int b = 0; b < TILE; b++) {
+                const unsigned long long c = __ballot((w >> b) & 1ull);
+                wt = l == b ? c : wt;
+            }
+            w = wt;
+        }
         d.mask[(size_t)t * TILE + l] = w;
         d.excl[(size_t)t * TILE + l] = 0ull;
-        cnt += __popcll(w);
+        if (l == 0) {
+            d.tpo[t] = flip ? make_int2(ij.y, ij.x) : ij;
+            d.tflip[t] = flip;
+            d.tcol[t] = flip ? rows : cols;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
     if (l == 0 && cnt) atomicAdd(d.npairs, cnt);
@@ -228,9 +249,78 @@ __global__ __launch_bounds__(256) void k_excl(DevPtrs d)
             if (J == T2) { found = mid; break; }
             if (J < T2) lo = mid + 1; else hi = mid - 1;
         }
-        if (found >= 0)
-            atomicOr(&d.excl[(size_t)found * TILE + (s1 & 63)], 1ull << (s2 & 63));
+        if (found >= 0) {
+            const int r = d.tflip[found] ? (s2 & 63) : (s1 & 63);
+            const int c = d.tflip[found] ? (s1 & 63) : (s2 & 63);
+            atomicOr(&d.excl[(size_t)found * TILE + r], 1ull << c);
+        }
     }
+}
+
+/*
+ * Processing order of the evaluation passes: the non-empty tile pairs sorted
+ * by their number of non-empty columns (the step count of a visit), largest
+ * first, by a counting sort over 64 bins.  Workgroups take tile pairs from a
+ * shared queue in this order, so the last ones taken are the shortest.
+ */
+__global__ __launch_bounds__(256) void k_bincount(DevPtrs d)
+{
+    __shared__ int h[TILE];
+    const int ntp = *d.ntp;
+    if (threadIdx.x < TILE) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < ntp; t += gridDim.x * 256) {
+        const int m = __popcll(d.tcol[t]);
+        if (m) atomicAdd(&h[TILE - m], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < TILE && h[threadIdx.x]) atomicAdd(&d.tbin[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(64) void k_binscan(DevPtrs d)
+{
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int b = 0; b < TILE; b++) {
+            d.tbin[TILE + b] = off;
+            off += d.tbin[b];
+        }
+        *d.nnz = off;
+    }
+}
+
+/* same grid as k_bincount: each block reserves its slice of every bin once */
+__global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
+{
+    __shared__ int h[TILE], base[TILE];
+    const int ntp = *d.ntp;
+    for (int t0 = blockIdx.x * 256; t0 < ntp; t0 += gridDim.x * 256) {
+        if (threadIdx.x < TILE) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int t = t0 + threadIdx.x;
+        const int m = t < ntp ? __popcll(d.tcol[t]) : 0;
+        int r = 0;
+        if (m) r = atomicAdd(&h[TILE - m], 1);
+        __syncthreads();
+        if (threadIdx.x < TILE && h[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&d.tbin[TILE + threadIdx.x], h[threadIdx.x]);
+        __syncthreads();
+        if (m) d.tord[base[TILE - m] + r] = t;
+        __syncthreads();
+    }
+}
+
+/*
+ * Static assignment of the sorted tile pairs to the persistent grid, in
+ * boustrophedon rounds: round r gives workgroup b the item r*G + b for even
+ * r and r*G + G-1-b for odd r.  Over a descending cost order this pairs each
+ * workgroup's expensive items with cheap ones (longest-processing-time
+ * balance) without a shared counter.
+ */
+__device__ __forceinline__ int snake_item(int r)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    return r * G + ((r & 1) ? (G - 1 - b) : b);
 }
 
 /* ------------------------------------------------------------ evaluation */
@@ -286,55 +376,75 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
 {
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
-    __shared__ R s_ri[TILE], s_sj[TILE], s_rb[TILE];
-    __shared__ int s_nk[TILE];
+    __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE];
+    __shared__ int s_nk[TILE], s_col[TILE];
+    __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     const int l = threadIdx.x;
-    const int ntp = *d.ntp;
+    const int nnz = *d.nnz;
     const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
     const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
-    const R *npos = neckpos<R>(d), *nval = neckval<R>(d);
-    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
-        const int2 ij = d.tp[t];
-        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+    if (p.has_neck) {
+        for (int k = l; k < NECK_N * NECK_N; k += TILE) {
+            npos[k] = neckpos<R>(d)[k];
+            nval[k] = neckval<R>(d)[k];
+        }
+    }
+    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+        const int t = d.tord[q];
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        if (m == 0) continue;
+        const int2 ab = d.tpo[t];
+        const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const int ni = d.neck[i];
-        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        const bool own = (cols >> l) & 1ull;
+        const int pos = __popcll(cols & ((1ull << l) - 1ull));
         __syncthreads();
-        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
-        s_ri[l] = ldri0<R>(d, j); s_sj[l] = ldsj<R>(d, j); s_rb[l] = ldrb<R>(d, j);
-        s_nk[l] = d.neck[j];
-        s_acc[l] = 0.0;
+        if (own) {
+            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            const R rj = ldri0<R>(d, j);
+            s_ri[pos] = rj; s_ri1i[pos] = R(1) / rj;
+            s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
+            s_nk[pos] = d.neck[j];
+            s_col[pos] = l;
+            s_acc[pos] = 0.0;
+        }
         __syncthreads();
         double acc = 0.0;
-        for (int s = 0; s < TILE; s++) {
-            const int b = (l + s) & (TILE - 1);
-            if (!((m >> b) & 1ull)) continue;
-            const double dx = xi - s_x[b], dy = yi - s_y[b], dz = zi - s_z[b];
-            const double r2d = dx * dx + dy * dy + dz * dz;
-            if (r2d > rp2) continue;
-            const R r2 = (R)r2d;
-            const R dij1i = gbm<R>::rsqrt_(r2);
-            const R dij = r2 * dij1i;
-            const R rj = s_ri[b], sj = s_sj[b];
-            bool skip_ij, skip_ji;
-            R tij = descreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij);
-            R tji = descreen<R>(rj, R(1) / rj, si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji);
-            if (p.has_neck) {
-                const int nj = s_nk[b];
-                if (dij < rbi + s_rb[b] + neckcut) {
-                    if (!skip_ij) tij += neck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
-                    if (!skip_ji) tji += neck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+        int k = l % m;
+        for (int s = 0; s < m; s++) {
+            if ((mr >> s_col[k]) & 1ull) {
+                const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
+                const double r2d = dx * dx + dy * dy + dz * dz;
+                if (r2d <= rp2) {
+                    const R r2 = (R)r2d;
+                    const R dij1i = gbm<R>::rsqrt_(r2);
+                    const R dij = r2 * dij1i;
+                    const R rj = s_ri[k], sj = s_sj[k];
+                    bool skip_ij, skip_ji;
+                    R tij = descreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij);
+                    R tji = descreen<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji);
+                    if (p.has_neck) {
+                        const int nj = s_nk[k];
+                        if (dij < rbi + s_rb[k] + neckcut) {
+                            if (!skip_ij) tij += neck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
+                            if (!skip_ji) tji += neck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                        }
+                    }
+                    acc -= (double)tij;
+                    atomicAdd(&s_acc[k], -(double)tji);
                 }
             }
-            acc -= (double)tij;
-            s_acc[b] -= (double)tji;
+            k = k + 1 == m ? 0 : k + 1;
         }
         __syncthreads();
         if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
-        if (s_acc[l] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[l]);
+        if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
     }
 }
 
@@ -400,17 +510,21 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE], s_sd[TILE];
     __shared__ double s_q[TILE];
     __shared__ R s_r[TILE];
-    __shared__ int s_lj[TILE];
+    __shared__ int s_lj[TILE], s_col[TILE];
     const int l = threadIdx.x;
-    const int ntp = *d.ntp;
+    const int nnz = *d.nnz;
     const R epsinv = (R)(1.0 / p.epsext);
     const R kks = (R)(-GB_KSCALE * p.kappa);
     const bool screened = p.kappa != 0.0;
     double epol = 0.0, eel = 0.0, evdw = 0.0;
     int cur_mol = -1;
-    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
-        const int2 ij = d.tp[t];
-        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+        const int t = d.tord[q];
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        if (m == 0) continue;
+        const int2 ab = d.tpo[t];
+        const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const int mol = d.slot_mol[i];
         if (mol != cur_mol) {
             if (cur_mol >= 0) {
@@ -432,77 +546,87 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         const double qi = d.q[i];
         const R ri = (R)d.reff[i];
         const int row = d.ljrow[i];
-        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const unsigned long long e = d.excl[(size_t)t * TILE + l];
+        const bool own = (cols >> l) & 1ull;
+        const int pos = __popcll(cols & ((1ull << l) - 1ull));
         __syncthreads();
-        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
-        s_q[l] = d.q[j]; s_r[l] = (R)d.reff[j];
-        s_lj[l] = d.ljcol[j];
-        s_fx[l] = 0.0; s_fy[l] = 0.0; s_fz[l] = 0.0; s_sd[l] = 0.0;
-        __syncthreads();
-        double fxi = 0.0, fyi = 0.0, fzi = 0.0, sdi = 0.0;
-        for (int s = 0; s < TILE; s++) {
-            const int b = (l + s) & (TILE - 1);
-            if (!((m >> b) & 1ull)) continue;
-            const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
-            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
-            const R r2 = (R)r2d;
-            const double qiqj_d = qi * s_q[b];
-            const R qiqj = (R)qiqj_d;
-            const R rj = s_r[b];
-            const R rb2 = ri * rj;
-            const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
-            const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
-            R expmkf, dielfac, fgbk;
-            if (screened) {
-                fgbk = kks * gbm<R>::rcp(fgbi);
-                expmkf = gbm<R>::exp_(fgbk) * epsinv;
-            } else {
-                fgbk = R(0);
-                expmkf = epsinv;
-            }
-            dielfac = R(1) - expmkf;
-            epol -= (double)(qiqj * dielfac * fgbi);
-            const R temp4 = fgbi * fgbi * fgbi;
-            const R temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
-            R de = temp6 * (R(1) - R(0.25) * efac);
-            const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
-            sdi += (double)(ri * temp5);
-            s_sd[b] += (double)(rj * temp5);
-            double ded = (double)de;
-            if (!((e >> b) & 1ull)) {
-                /* Coulomb and LJ in FP64 in both variants: their molecular
-                 * totals cancel strongly (sums of +-100 kcal/mol terms), so
-                 * FP32 terms cannot hold 1e-6 of the total; they cost a few
-                 * FP64 operations next to the FP32 transcendentals above */
-                const double rinv = rsqrt_pair<R>(r2d);
-                const double r2inv = rinv * rinv;
-                const double el = qiqj_d * rinv;
-                eel += el;
-                ded -= el * r2inv;
-                const double r6inv = r2inv * r2inv * r2inv;
-                const int k = row + s_lj[b];
-                const double f6 = d.ljb[k] * r6inv;
-                const double f12 = d.lja[k] * r6inv * r6inv;
-                evdw += f12 - f6;
-                ded -= (12.0 * f12 - 6.0 * f6) * r2inv;
-            }
-            const double gxv = ded * dxd, gyv = ded * dyd, gzv = ded * dzd;
-            fxi += gxv; fyi += gyv; fzi += gzv;
-            s_fx[b] -= gxv; s_fy[b] -= gyv; s_fz[b] -= gzv;
+        if (own) {
+            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            s_q[pos] = d.q[j]; s_r[pos] = (R)d.reff[j];
+            s_lj[pos] = d.ljcol[j];
+            s_col[pos] = l;
+            s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0; s_sd[pos] = 0.0;
         }
         __syncthreads();
-        if (m) {
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0, sdi = 0.0;
+        int k = l % m;
+        for (int s = 0; s < m; s++) {
+            const int b = s_col[k];
+            if ((mr >> b) & 1ull) {
+                const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
+                const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+                const R r2 = (R)r2d;
+                const double qiqj_d = qi * s_q[k];
+                const R qiqj = (R)qiqj_d;
+                const R rj = s_r[k];
+                const R rb2 = ri * rj;
+                const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
+                const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
+                R expmkf, dielfac, fgbk;
+                if (screened) {
+                    fgbk = kks * gbm<R>::rcp(fgbi);
+                    expmkf = gbm<R>::exp_(fgbk) * epsinv;
+                } else {
+                    fgbk = R(0);
+                    expmkf = epsinv;
+                }
+                dielfac = R(1) - expmkf;
+                epol -= (double)(qiqj * dielfac * fgbi);
+                const R temp4 = fgbi * fgbi * fgbi;
+                const R temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
+                R de = temp6 * (R(1) - R(0.25) * efac);
+                const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
+                sdi += (double)(ri * temp5);
+                atomicAdd(&s_sd[k], (double)(rj * temp5));
+                double ded = (double)de;
+                if (!((e >> b) & 1ull)) {
+                    /* Coulomb and LJ in FP64 in both variants: their molecular
+                     * totals cancel strongly (sums of +-100 kcal/mol terms), so
+                     * FP32 terms cannot hold 1e-6 of the total; they cost a few
+                     * FP64 operations next to the FP32 transcendentals above */
+                    const double rinv = rsqrt_pair<R>(r2d);
+                    const double r2inv = rinv * rinv;
+                    const double el = qiqj_d * rinv;
+                    eel += el;
+                    ded -= el * r2inv;
+                    const double r6inv = r2inv * r2inv * r2inv;
+                    const int kk = row + s_lj[k];
+                    const double f6 = d.ljb[kk] * r6inv;
+                    const double f12 = d.lja[kk] * r6inv * r6inv;
+                    evdw += f12 - f6;
+                    ded -= (12.0 * f12 - 6.0 * f6) * r2inv;
+                }
+                const double gxv = ded * dxd, gyv = ded * dyd, gzv = ded * dzd;
+                fxi += gxv; fyi += gyv; fzi += gzv;
+                atomicAdd(&s_fx[k], -gxv);
+                atomicAdd(&s_fy[k], -gyv);
+                atomicAdd(&s_fz[k], -gzv);
+            }
+            k = k + 1 == m ? 0 : k + 1;
+        }
+        __syncthreads();
+        if (mr) {
             unsafeAtomicAdd(&d.gx[i], fxi);
             unsafeAtomicAdd(&d.gy[i], fyi);
             unsafeAtomicAdd(&d.gz[i], fzi);
             unsafeAtomicAdd(&d.sumdeijda[i], sdi);
         }
-        if (s_sd[l] != 0.0 || s_fx[l] != 0.0) {
-            unsafeAtomicAdd(&d.gx[j], s_fx[l]);
-            unsafeAtomicAdd(&d.gy[j], s_fy[l]);
-            unsafeAtomicAdd(&d.gz[j], s_fz[l]);
-            unsafeAtomicAdd(&d.sumdeijda[j], s_sd[l]);
+        if (own) {
+            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+            unsafeAtomicAdd(&d.sumdeijda[j], s_sd[pos]);
         }
     }
     if (cur_mol >= 0) {
@@ -531,55 +655,77 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
 {
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
-    __shared__ R s_ri[TILE], s_sj[TILE], s_rb[TILE], s_w[TILE];
-    __shared__ int s_nk[TILE];
+    __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE], s_w[TILE];
+    __shared__ int s_nk[TILE], s_col[TILE];
+    __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     const int l = threadIdx.x;
-    const int ntp = *d.ntp;
-    const R rgbmax = (R)p.rgbmax, rgbmax2i = (R)p.rgbmax2i;
+    const int nnz = *d.nnz;
+    const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
     const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
-    const R *npos = neckpos<R>(d), *nval = neckval<R>(d);
-    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
-        const int2 ij = d.tp[t];
-        const int i = ij.x * TILE + l, j = ij.y * TILE + l;
+    if (p.has_neck) {
+        for (int k = l; k < NECK_N * NECK_N; k += TILE) {
+            npos[k] = neckpos<R>(d)[k];
+            nval[k] = neckval<R>(d)[k];
+        }
+    }
+    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+        const int t = d.tord[q];
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        if (m == 0) continue;
+        const int2 ab = d.tpo[t];
+        const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
         const int ni = d.neck[i];
-        const unsigned long long m = d.mask[(size_t)t * TILE + l];
+        const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        const bool own = (cols >> l) & 1ull;
+        const int pos = __popcll(cols & ((1ull << l) - 1ull));
         __syncthreads();
-        s_x[l] = d.xs[j]; s_y[l] = d.ys[j]; s_z[l] = d.zs[j];
-        s_ri[l] = ldri0<R>(d, j); s_sj[l] = ldsj<R>(d, j); s_rb[l] = ldrb<R>(d, j);
-        s_w[l] = (R)(d.sumdeijda[j] * d.dfac[j]);
-        s_nk[l] = d.neck[j];
-        s_fx[l] = 0.0; s_fy[l] = 0.0; s_fz[l] = 0.0;
+        if (own) {
+            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            const R rj = ldri0<R>(d, j);
+            s_ri[pos] = rj; s_ri1i[pos] = R(1) / rj;
+            s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
+            s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
+            s_nk[pos] = d.neck[j];
+            s_col[pos] = l;
+            s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
+        }
         __syncthreads();
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        for (int s = 0; s < TILE; s++) {
-            const int b = (l + s) & (TILE - 1);
-            if (!((m >> b) & 1ull)) continue;
-            const double dxd = xi - s_x[b], dyd = yi - s_y[b], dzd = zi - s_z[b];
-            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
-            if (r2d > rp2) continue;
-            const R r2 = (R)r2d;
-            const R dij1i = gbm<R>::rsqrt_(r2);
-            const R dij = r2 * dij1i;
-            const R rj = s_ri[b], sj = s_sj[b];
-            bool skip_ij, skip_ji;
-            R aij = ddescreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax2i, skip_ij);
-            R aji = ddescreen<R>(rj, R(1) / rj, si, r2, dij, dij1i, rgbmax, rgbmax2i, skip_ji);
-            if (p.has_neck) {
-                const int nj = s_nk[b];
-                if (dij < rbi + s_rb[b] + neckcut) {
-                    if (!skip_ij) aij += dneck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
-                    if (!skip_ji) aji += dneck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+        int k = l % m;
+        for (int s = 0; s < m; s++) {
+            if ((mr >> s_col[k]) & 1ull) {
+                const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
+                const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+                if (r2d <= rp2) {
+                    const R r2 = (R)r2d;
+                    const R dij1i = gbm<R>::rsqrt_(r2);
+                    const R dij = r2 * dij1i;
+                    const R rj = s_ri[k], sj = s_sj[k];
+                    bool skip_ij, skip_ji;
+                    R aij = ddescreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij);
+                    R aji = ddescreen<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji);
+                    if (p.has_neck) {
+                        const int nj = s_nk[k];
+                        if (dij < rbi + s_rb[k] + neckcut) {
+                            if (!skip_ij) aij += dneck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
+                            if (!skip_ji) aji += dneck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                        }
+                    }
+                    const double f = (double)(aij * wi + aji * s_w[k]);
+                    const double gxv = f * dxd, gyv = f * dyd, gzv = f * dzd;
+                    fxi -= gxv; fyi -= gyv; fzi -= gzv;
+                    atomicAdd(&s_fx[k], gxv);
+                    atomicAdd(&s_fy[k], gyv);
+                    atomicAdd(&s_fz[k], gzv);
                 }
             }
-            const double f = (double)(aij * wi + aji * s_w[b]);
-            const double gxv = f * dxd, gyv = f * dyd, gzv = f * dzd;
-            fxi -= gxv; fyi -= gyv; fzi -= gzv;
-            s_fx[b] += gxv; s_fy[b] += gyv; s_fz[b] += gzv;
+            k = k + 1 == m ? 0 : k + 1;
         }
         __syncthreads();
         if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
@@ -587,10 +733,10 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             unsafeAtomicAdd(&d.gy[i], fyi);
             unsafeAtomicAdd(&d.gz[i], fzi);
         }
-        if (s_fx[l] != 0.0 || s_fy[l] != 0.0 || s_fz[l] != 0.0) {
-            unsafeAtomicAdd(&d.gx[j], s_fx[l]);
-            unsafeAtomicAdd(&d.gy[j], s_fy[l]);
-            unsafeAtomicAdd(&d.gz[j], s_fz[l]);
+        if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
+            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
         }
     }
 }
@@ -613,7 +759,7 @@ __global__ __launch_bounds__(64) void k_listpairs(DevPtrs d, int *pi, int *pj, u
     const int l = threadIdx.x;
     const int ntp = *d.ntp;
     for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
-        const int2 ij = d.tp[t];
+        const int2 ij = d.tpo[t];
         unsigned long long w = d.mask[(size_t)t * TILE + l];
         const int a = d.perm[ij.x * TILE + l];
         while (w) {
@@ -847,6 +993,9 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.gy, NS)) || (rc = E.dalloc(&D.gz, NS)) || (rc = E.dalloc(&D.ene, 3 * (size_t)nmol)) ||
         (rc = E.dalloc(&D.cand, (size_t)cap)) || (rc = E.dalloc(&D.rowcnt, NT)) ||
         (rc = E.dalloc(&D.rowstart, NT)) || (rc = E.dalloc(&D.ntp, 1)) || (rc = E.dalloc(&D.tp, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
         (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
         (rc = E.dalloc(&D.bbhi, 3 * (size_t)NT)) || (rc = E.dalloc(&E.x_dev, 3 * (size_t)E.natoms)) ||
@@ -854,6 +1003,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&E.ene14_dev, 14)))
         return rc;
     HIPCHK(hipMemsetAsync(D.ntp, 0, sizeof(int), E.stream));
+    HIPCHK(hipMemsetAsync(D.nnz, 0, sizeof(int), E.stream));
     HIPCHK(hipMemsetAsync(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms, E.stream));
     if (x0) {
         for (int m = 0; m < nmol; m++)
@@ -867,6 +1017,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     E.pair_blocks = (int)std::min<long long>(std::max<long long>(1, cap), (long long)ncu * 16);
     E.pair_blocks = std::min(E.pair_blocks, PAIR_BLOCKS_MAX * 4);
+    E.ncu = ncu;
+    for (int k = 0; k < 3; k++) E.pass_blocks[k] = E.pair_blocks;
     return 0;
 }
 
@@ -888,6 +1040,10 @@ int engine_nblist(Engine &E)
     hipLaunchKernelGGL(k_rowscan, dim3(1), dim3(1024), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_compact, dim3(NT), dim3(256), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_masks, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+    const int nsb = (int)std::min<long long>(64, (E.tp_cap + 255) / 256);
+    hipLaunchKernelGGL(k_bincount, dim3(nsb), dim3(256), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_binscan, dim3(1), dim3(64), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_binscatter, dim3(nsb), dim3(256), 0, E.stream, E.d);
     if (E.natoms > 0)
         hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
     LAUNCHCHK();
@@ -909,21 +1065,21 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], E.stream));
     if (E.precision == GBHIP_FP64)
-        hipLaunchKernelGGL(k_born<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_born<double>, dim3(E.pass_blocks[0]), dim3(64), 0, E.stream, E.d, E.par);
     else
-        hipLaunchKernelGGL(k_born<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_born<float>, dim3(E.pass_blocks[0]), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
     hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
     if (E.precision == GBHIP_FP64)
-        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
     else
-        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
     if (E.precision == GBHIP_FP64)
-        hipLaunchKernelGGL(k_chain<double>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_chain<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
     else
-        hipLaunchKernelGGL(k_chain<float>, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL(k_chain<float>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (g_dev)
         hipLaunchKernelGGL(k_scatter, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev);
@@ -972,6 +1128,84 @@ static int timing_stop(Engine &E, double *ms4, int *nevals)
         for (int s = 0; s < 4; s++) ms4[s] = acc[s];
     if (nevals) *nevals = n;
     return 0;
+}
+
+/* ------------------------------------------------------------ launch shapes */
+
+static int set_launch(Engine &E, const int *blocks3)
+{
+    for (int k = 0; k < 3; k++) {
+        if (blocks3[k] < 0 || blocks3[k] > 64 * 1024) return seterr(GBHIP_EINVAL, "launch size out of range");
+    }
+    for (int k = 0; k < 3; k++)
+        if (blocks3[k] > 0) E.pass_blocks[k] = blocks3[k];
+    return 0;
+}
+
+/*
+ * Time whole evaluations on the current list and coordinates for each
+ * candidate persistent-grid size (workgroups per CU: 4 .. 40) and keep, per
+ * pass, the fastest.  The candidates differ in how many wavefronts share a
+ * CU (latency hiding against LDS and register limits) and in the tail of
+ * the tile-pair loop.  Leaves the engine evaluated at the current
+ * coordinates; replaces any timing session in progress.
+ */
+static int autotune(Engine &E, int reps, int *chosen3)
+{
+    static const int per_cu[] = {4, 8, 12, 16, 20, 24, 32, 40};
+    const int ncand = (int)(sizeof(per_cu) / sizeof(per_cu[0]));
+    if (!E.have_list) return seterr(GBHIP_ESTATE, "autotune needs a pair list (call nblist first)");
+    if (reps < 1 || reps > 1000) return seterr(GBHIP_EINVAL, "reps out of range");
+    double best[3] = {1e300, 1e300, 1e300};
+    int pick[3] = {E.pass_blocks[0], E.pass_blocks[1], E.pass_blocks[2]};
+    int rc;
+    for (int c = 0; c < ncand; c++) {
+        const int nb = std::max(1, E.ncu * per_cu[c]);
+        for (int k = 0; k < 3; k++) E.pass_blocks[k] = nb;
+        /* one untimed evaluation at the new shape, then reps timed */
+        if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
+        if ((rc = timing_start(E, reps))) return rc;
+        for (int r = 0; r < reps; r++)
+            if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
+        double ms4[4];
+        int n = 0;
+        if ((rc = timing_stop(E, ms4, &n))) return rc;
+        const double ms[3] = {ms4[0], ms4[2], ms4[3]};
+        for (int k = 0; k < 3; k++)
+            if (ms[k] < best[k]) {
+                best[k] = ms[k];
+                pick[k] = nb;
+            }
+    }
+    for (int k = 0; k < 3; k++) E.pass_blocks[k] = pick[k];
+    if (chosen3)
+        for (int k = 0; k < 3; k++) chosen3[k] = pick[k];
+    return engine_eval(E, E.x_dev, nullptr);
+}
+
+extern "C" int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3)
+{
+    if (!ctx || !blocks3) return seterr(GBHIP_EINVAL, "NULL argument");
+    return set_launch(ctx->E, blocks3);
+}
+
+extern "C" int gbhip_get_launch(gbhip_ctx *ctx, int *blocks3)
+{
+    if (!ctx || !blocks3) return seterr(GBHIP_EINVAL, "NULL argument");
+    for (int k = 0; k < 3; k++) blocks3[k] = ctx->E.pass_blocks[k];
+    return 0;
+}
+
+extern "C" int gbhip_autotune(gbhip_ctx *ctx, int reps, int *chosen3)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return autotune(ctx->E, reps, chosen3);
+}
+
+extern "C" int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return autotune(b->E, reps, chosen3);
 }
 
 /* ---------------------------------------------------------- single context */
