@@ -75,8 +75,7 @@ struct DevPtrs {
     const int *ex_ptr, *ex_j, *inv;
     int natoms;
     /* LJ dense tables, concatenated per molecule */
-    const double *lja, *ljb;
-    const float *lja_f, *ljb_f;
+    const double2 *ljab;  /* (A, B) = (cn1, cn2) per type pair */
     const double *neck_pos, *neck_val;
     const float *neck_pos_f, *neck_val_f;
 };

@@ -62,11 +62,13 @@ template <> struct gbm<double> {
 /*
  * FP32: the hardware transcendental instructions (v_exp_f32, v_log_f32,
  * v_rcp_f32, v_rsq_f32; ~1 ulp), every sum they feed is accumulated in
- * FP64 by the caller.
+ * FP64 by the caller.  log and exp go through base 2 directly: the
+ * arguments here (ratios of distances and radii, -r^2/4RiRj, -kappa f) are
+ * never denormal, so the library's denormal rescaling is not needed.
  */
 template <> struct gbm<float> {
-    static __device__ __forceinline__ float log_(float x) { return __logf(x); }
-    static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
+    static __device__ __forceinline__ float log_(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+    static __device__ __forceinline__ float exp_(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_rsqf(x); }
 };
@@ -80,7 +82,7 @@ template <typename R> __device__ __forceinline__ double rsqrt_pair(double r2);
 template <> __device__ __forceinline__ double rsqrt_pair<double>(double r2) { return 1.0 / sqrt(r2); }
 template <> __device__ __forceinline__ double rsqrt_pair<float>(double r2)
 {
-    double y = (double)rsqrtf((float)r2);
+    double y = (double)__builtin_amdgcn_rsqf((float)r2);
     const double h = 0.5 * r2;
     y = y * (1.5 - h * y * y);
     y = y * (1.5 - h * y * y);

@@ -23,11 +23,16 @@
  *              OBC factor is applied as the atoms are loaded, so the
  *              separate sumdeijda reduction of eff.c:2077-2129 disappears
  *   k_scatter  add the slot gradients into the caller's AoS array
- * Pair kernels: one 64-lane wavefront per tile pair; lane l holds i-atom l,
- * the j-tile is staged in LDS and visited in the rotated order
- * j = (l + s) mod 64 so that every lane touches a different j each step;
- * j-side sums are accumulated in LDS and flushed with one FP64 atomic per
- * atom per tile pair.
+ * Atoms are ordered per molecule by a k-d split into 64-atom leaf boxes.
+ * Pair kernels: one 64-lane wavefront per tile pair; lane l holds atom l of
+ * tile A, the non-empty columns (atoms of tile B with at least one partner)
+ * are staged in LDS and visited in the rotated order k = (l + s) mod m, so
+ * that lanes touch different columns each step; the pair is oriented at
+ * list build so that the side with fewer non-empty atoms is the one
+ * stepped.  j-side sums are accumulated in LDS (ds_add_f64) and flushed
+ * with one FP64 atomic per atom per tile pair.  Tile pairs are processed in
+ * descending step count, dealt to the persistent grid in boustrophedon
+ * rounds.
  */
 #include <hip/hip_runtime.h>
 
@@ -360,12 +365,6 @@ template <> __device__ __forceinline__ const float *neckpos<float>(const DevPtrs
 template <typename R> __device__ __forceinline__ const R *neckval(const DevPtrs &d);
 template <> __device__ __forceinline__ const double *neckval<double>(const DevPtrs &d) { return d.neck_val; }
 template <> __device__ __forceinline__ const float *neckval<float>(const DevPtrs &d) { return d.neck_val_f; }
-template <typename R> __device__ __forceinline__ const R *lja(const DevPtrs &d);
-template <> __device__ __forceinline__ const double *lja<double>(const DevPtrs &d) { return d.lja; }
-template <> __device__ __forceinline__ const float *lja<float>(const DevPtrs &d) { return d.lja_f; }
-template <typename R> __device__ __forceinline__ const R *ljb(const DevPtrs &d);
-template <> __device__ __forceinline__ const double *ljb<double>(const DevPtrs &d) { return d.ljb; }
-template <> __device__ __forceinline__ const float *ljb<float>(const DevPtrs &d) { return d.ljb_f; }
 
 /*
  * Pass 1 (eff.c:1566-1733): for every listed pair, the descreening of i by
@@ -602,8 +601,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     ded -= el * r2inv;
                     const double r6inv = r2inv * r2inv * r2inv;
                     const int kk = row + s_lj[k];
-                    const double f6 = d.ljb[kk] * r6inv;
-                    const double f12 = d.lja[kk] * r6inv * r6inv;
+                    const double2 ab = d.ljab[kk];
+                    const double f6 = ab.y * r6inv;
+                    const double f12 = ab.x * r6inv * r6inv;
                     evdw += f12 - f6;
                     ded -= (12.0 * f12 - 6.0 * f6) * r2inv;
                 }
@@ -785,42 +785,42 @@ __global__ __launch_bounds__(256) void k_unsort(DevPtrs d, GBPar p, const double
 
 /* ------------------------------------------------------------------ engine */
 
-static unsigned int spread_bits(unsigned int v)
+/*
+ * Atom order within a molecule: a k-d split of the atom set, recursively
+ * along the longest extent of the current box, at a multiple of the tile
+ * size (64), so that every tile is one leaf box.  Leaf boxes are compact, so
+ * the 64x64 pair matrices of two tiles have fewer empty rows and columns
+ * than with tiles cut from a space-filling curve.  Ties are broken by atom
+ * index, so the order is deterministic.
+ */
+static void kd_split(const double *x, int *idx, int n)
 {
-    v &= 0x3ff;
-    v = (v | (v << 16)) & 0x030000FF;
-    v = (v | (v << 8)) & 0x0300F00F;
-    v = (v | (v << 4)) & 0x030C30C3;
-    v = (v | (v << 2)) & 0x09249249;
-    return v;
+    if (n <= TILE) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int k = 0; k < n; k++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], x[3 * idx[k] + c]);
+            hi[c] = std::max(hi[c], x[3 * idx[k] + c]);
+        }
+    int ax = 0;
+    for (int c = 1; c < 3; c++)
+        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    const int ntile = (n + TILE - 1) / TILE;
+    const int nleft = (ntile / 2) * TILE;
+    std::nth_element(idx, idx + nleft, idx + n, [x, ax](int a, int b) {
+        const double xa = x[3 * a + ax], xb = x[3 * b + ax];
+        return xa < xb || (xa == xb && a < b);
+    });
+    kd_split(x, idx, nleft);
+    kd_split(x, idx + nleft, n - nleft);
 }
 
-/* Morton (Z-order) ordering of one molecule's atoms */
-static void morton_order(int n, const double *x, std::vector<int> &order)
+static void tile_order(int n, const double *x, std::vector<int> &order)
 {
     order.resize(n);
     for (int i = 0; i < n; i++) order[i] = i;
     if (!x || n < 2) return;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], x[3 * i + k]);
-            hi[k] = std::max(hi[k], x[3 * i + k]);
-        }
-    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    if (!(ext > 0)) return;
-    std::vector<unsigned long long> key(n);
-    for (int i = 0; i < n; i++) {
-        unsigned int c[3];
-        for (int k = 0; k < 3; k++) {
-            double f = (x[3 * i + k] - lo[k]) / ext * 1023.999;
-            c[k] = (unsigned int)std::max(0.0, std::min(1023.0, f));
-        }
-        unsigned long long code = spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2);
-        key[i] = (code << 32) | (unsigned long long)i;
-    }
-    std::sort(key.begin(), key.end());
-    for (int i = 0; i < n; i++) order[i] = (int)(key[i] & 0xffffffffu);
+    kd_split(x, order.data(), n);
 }
 
 static int check_options(const gbhip_options *o)
@@ -889,7 +889,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     std::vector<double> q(NS, 0.0), rb(NS, 1.0), ri0(NS, 1.0), sj(NS, 0.0), ga(NS, 0.0), gbb(NS, 0.0), gg(NS, 0.0);
     std::vector<int> tile_mol(NT), tile_end(NT);
     std::vector<long long> rowbase(NT);
-    std::vector<double> lja, ljb, rgbpsmax2(nmol);
+    std::vector<double2> ljab;
+    std::vector<double> rgbpsmax2(nmol);
     std::vector<int> ex_ptr(E.natoms + 1, 0), ex_j;
     long long cap = 0;
 
@@ -897,14 +898,13 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         const gbhip_topology &t = tops[m];
         const int n = t.natom, so = E.mol_slot_off[m], ao = E.mol_atom_off[m];
         std::vector<int> order;
-        morton_order(n, x0 ? x0[m] : nullptr, order);
-        const int lj0 = (int)lja.size(), nt = t.ntypes;
+        tile_order(n, x0 ? x0[m] : nullptr, order);
+        const int lj0 = (int)ljab.size(), nt = t.ntypes;
         rgbpsmax2[m] = (o->rgbmax + t.fsmax) * (o->rgbmax + t.fsmax);
         for (int a = 0; a < nt; a++)
             for (int b = 0; b < nt; b++) {
                 int ic = t.cno[a * nt + b] - 1;
-                lja.push_back(ic >= 0 ? t.cn1[ic] : 0.0);
-                ljb.push_back(ic >= 0 ? t.cn2[ic] : 0.0);
+                ljab.push_back(make_double2(ic >= 0 ? t.cn1[ic] : 0.0, ic >= 0 ? t.cn2[ic] : 0.0));
             }
         for (int k = 0; k < n; k++) {
             const int a = order[k], s = so + k;
@@ -973,16 +973,16 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.upload(&D.tile_mol, tile_mol)) || (rc = E.upload(&D.tile_end, tile_end)) ||
         (rc = E.upload(&D.rowbase, rowbase)) || (rc = E.upload(&D.ex_ptr, ex_ptr)) ||
         (rc = E.upload(&D.ex_j, ex_j)) || (rc = E.upload(&D.inv, inv)) ||
-        (rc = E.upload(&D.lja, lja)) || (rc = E.upload(&D.ljb, ljb)) ||
+        (rc = E.upload(&D.ljab, ljab)) ||
         (rc = E.upload(&D.rgbpsmax2, rgbpsmax2)))
         return rc;
     {
         std::vector<float> qf(q.begin(), q.end()), rf(ri0.begin(), ri0.end()), sf(sj.begin(), sj.end()),
-            bf(rb.begin(), rb.end()), af(lja.begin(), lja.end()), cf(ljb.begin(), ljb.end());
+            bf(rb.begin(), rb.end());
         std::vector<double> np(NECK_POS, NECK_POS + NECK_N * NECK_N), nv(NECK_VAL, NECK_VAL + NECK_N * NECK_N);
         std::vector<float> npf(np.begin(), np.end()), nvf(nv.begin(), nv.end());
         if ((rc = E.upload(&D.q_f, qf)) || (rc = E.upload(&D.ri0_f, rf)) || (rc = E.upload(&D.sj_f, sf)) ||
-            (rc = E.upload(&D.rb_f, bf)) || (rc = E.upload(&D.lja_f, af)) || (rc = E.upload(&D.ljb_f, cf)) ||
+            (rc = E.upload(&D.rb_f, bf)) ||
             (rc = E.upload(&D.neck_pos, np)) || (rc = E.upload(&D.neck_val, nv)) ||
             (rc = E.upload(&D.neck_pos_f, npf)) || (rc = E.upload(&D.neck_val_f, nvf)))
             return rc;
