@@ -51,6 +51,7 @@ def run_gpu(na, arr, x, precision, steps, warmup, tune=True):
     ntiles, ntp = ctx.tile_stats()
     # launch-shape autotuning is set-up, outside the timed region
     launch = ctx.autotune(5) if tune else ctx.launch()
+    store_need, store_cap, store_used = ctx.store_stats()
     for k in range(warmup):
         if k % NSNB == 0:
             ctx.nblist_async()
@@ -77,6 +78,7 @@ def run_gpu(na, arr, x, precision, steps, warmup, tune=True):
     return dict(evals_per_s=steps / (t1 - t0), ms_per_eval=1e3 * (t1 - t0) / steps,
                 pass_ms=dict(born=ms[0] / n, radii=ms[1] / n, pairs=ms[2] / n, chain=ms[3] / n),
                 nblist_ms=1e3 * tl, npairs=npairs, ntiles=ntiles, tilepairs=ntp, launch=launch,
+                store=dict(entries=store_need, allocated=store_cap, in_use=store_used),
                 energies=dict(epol=epol, evdw=evdw, eelec=eel), grad=g)
 
 
@@ -337,7 +339,8 @@ def main():
                                "kappa 0.127315 (0.15 M), nsnb 20",
                    "natom": NATOM, "seed": SEED, "npairs": P, "tiles": mixed["ntiles"],
                    "tile_pairs": mixed["tilepairs"], "sha256": digests,
-                   "launch_blocks": {"mixed": mixed["launch"], "fp64": fp64["launch"]}},
+                   "launch_blocks": {"mixed": mixed["launch"], "fp64": fp64["launch"]},
+                   "derivative_store": {"mixed": mixed["store"], "fp64": fp64["store"]}},
         "fp64": {"value": round(fp64["evals_per_s"], 2), "ms_per_step": round(fp64["ms_per_eval"], 4),
                  "pass_ms": {k: round(v, 4) for k, v in fp64["pass_ms"].items()},
                  "dominant": "k_%s<double>" % dom64, "achieved_tflops": round(a64, 3),

@@ -168,6 +168,21 @@ int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
  * configuration); nblist must have run.
  */
 int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3);
+
+/*
+ * Pass-3 derivative store.  Pass 1 can write, per listed pair, the two
+ * derivative numerators pass 3 needs (8 bytes per pair slot in the mixed
+ * variant, 16 in FP64), so that pass 3 does not recompute them.  mode 1
+ * (default): used whenever it holds the current list; synchronous list
+ * builds (gbhip_nblist, batch creation, gbhip_md_init) grow it to 1.25x the
+ * need when device memory allows, asynchronous ones fall back to
+ * recomputation when it is too small.  mode 0: never used (memory freed).
+ * gbhip_store_stats reports the entries the current list needs, the
+ * entries allocated and whether the store is in use.  No counterpart in
+ * the reference (eff.c:2131-2405 recomputes datmp in pass 3).
+ */
+int gbhip_set_derivative_store(gbhip_ctx *ctx, int mode);
+int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap, int *in_use);
 int gbhip_get_launch(gbhip_ctx *ctx, int *blocks3);
 int gbhip_autotune(gbhip_ctx *ctx, int reps, int *chosen3);
 
@@ -298,6 +313,9 @@ int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals);
 
 /* Launch-shape autotuning of a batch (as gbhip_autotune). */
 int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3);
+
+/* Derivative-store mode of a batch (as gbhip_set_derivative_store). */
+int gbhip_batch_set_derivative_store(gbhip_batch *b, int mode);
 
 /* Total unordered pairs and atoms of the batch. */
 int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms,

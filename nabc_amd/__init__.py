@@ -149,6 +149,9 @@ _sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_set_launch", C.c_int, [vp, ip])
+_sig("gbhip_set_derivative_store", C.c_int, [vp, C.c_int])
+_sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
+_sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_get_launch", C.c_int, [vp, ip])
 _sig("gbhip_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
@@ -455,6 +458,16 @@ class GBContext:
         _check(lib.gbhip_get_launch(self.h, _p(b, ip)))
         return tuple(int(v) for v in b)
 
+    def set_derivative_store(self, mode):
+        """1: pass 1 stores the pass-3 derivatives when they fit; 0: never."""
+        _check(lib.gbhip_set_derivative_store(self.h, mode))
+
+    def store_stats(self):
+        """(entries needed by the current list, entries allocated, in use)."""
+        need, cap, ok = C.c_longlong(), C.c_longlong(), C.c_int()
+        _check(lib.gbhip_store_stats(self.h, C.byref(need), C.byref(cap), C.byref(ok)))
+        return need.value, cap.value, bool(ok.value)
+
     def autotune(self, reps=5):
         """Pick the fastest persistent-grid size per pass; returns the choice."""
         b = np.zeros(3, np.int32)
@@ -541,6 +554,9 @@ class Batch:
         n = C.c_int()
         _check(lib.gbhip_batch_timing_stop(self.h, _p(ms), C.byref(n)))
         return ms, n.value
+
+    def set_derivative_store(self, mode):
+        _check(lib.gbhip_batch_set_derivative_store(self.h, mode))
 
     def autotune(self, reps=3):
         b = np.zeros(3, np.int32)

@@ -69,6 +69,11 @@ struct DevPtrs {
     int *tord;            /* non-empty tile pairs, most non-empty columns first */
     int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
     int *nnz;             /* number of non-empty tile pairs */
+    long long *binslot;   /* derivative-store offset of each bin */
+    long long *store_need; /* entries the current list needs */
+    int *store_ok;        /* 1 when the store holds the current list */
+    void *dstore;         /* pass-3 derivative numerators (float2 / double2) */
+    long long store_cap;  /* entries allocated */
     unsigned long long *npairs;
     double *bblo, *bbhi;  /* per tile bounding box */
     /* exclusions, CSR over global atoms, partners j > i */
@@ -111,10 +116,15 @@ struct Engine {
      * (before pass 1, after pass 1, after the radii, after pass 2, after pass 3) */
     std::vector<hipEvent_t> tev;
     int tev_cap = 0, tev_used = 0;
+    /* pass-3 derivative store (separate from allocs: it is resized) */
+    void *store_mem = nullptr;
+    int store_mode = 1;        /* 1: used when it fits, 0: always recompute */
+    bool store_frozen = false; /* a captured graph holds the current pointer */
 
     ~Engine()
     {
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        if (store_mem) (void)hipFree(store_mem);
         if (own_stream && stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -202,6 +212,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
                  const double *const *x0);
 int engine_gather(Engine &E, const double *x_dev);
 int engine_nblist(Engine &E);
+int engine_store_reserve(Engine &E);
 int engine_eval(Engine &E, const double *x_dev, double *g_dev);
 
 #endif

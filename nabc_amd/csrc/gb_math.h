@@ -141,50 +141,16 @@ __device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R di
 
 /*
  * Pass-1 descreening of an atom with offset radius ri (ri1i = 1/ri) by a
- * sphere of scaled radius sj at distance dij (r2 = dij^2, dij1i = 1/dij);
- * the amount egb() subtracts from sumi.  skip is set when dij > rgbmax + sj
- * (the pair is dropped, neck included).
+ * sphere of scaled radius sj at distance dij (r2 = dij^2, dij1i = 1/dij):
+ * returns the amount egb() subtracts from sumi and sets dv to the pass-3
+ * derivative numerator datmp (eff.c:2289-2334) of the same pair and
+ * direction; callers that need only one of the two let the compiler drop
+ * the other.  skip is set when dij > rgbmax + sj (the pair is dropped,
+ * neck included, in both passes).
  */
 template <typename R>
-__device__ __forceinline__ R descreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
-                                      R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
-{
-    const DsCase k = ds_case<R>(ri, sj, dij, rgbmax);
-    skip = k.skip;
-    R u1, u2, L;
-    ds_trans<R>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
-    const R sj2 = sj * sj;
-    R v = R(0);
-    if (k.a) {
-        v = R(0.125) * dij1i * (R(1) + R(2) * dij * u2 + rgbmax2i * (r2 - R(4) * rgbmax * dij - sj2) + R(2) * L);
-    } else if (k.b) {
-        const R dij2i = dij1i * dij1i;
-        const R tmpsd = sj2 * dij2i;
-        const R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
-        v = sj * tmpsd * dij2i * dumbo;
-    } else if (k.c || k.e) {
-        v = R(0.5) * (sj * u2 + R(0.5) * dij1i * L) + (k.e ? ri1i : R(0));
-    } else if (k.d) {
-        const R theta = R(0.5) * ri1i * dij1i * (r2 + ri * ri - sj2);
-        v = R(0.25) * (ri1i * (R(2) - theta) - u1 + dij1i * L);
-    }
-    return v;
-}
-
-/* Neck contribution (subtracted from sumi) for neck maximum (pos, val). */
-template <typename R>
-__device__ __forceinline__ R neck_term(R dij, R pos, R val, R neckscale)
-{
-    R mdist = dij - pos;
-    R mdist2 = mdist * mdist;
-    R mdist6 = mdist2 * mdist2 * mdist2;
-    return neckscale * val * gbm<R>::rcp(R(1) + mdist2 + R(0.3) * mdist6);
-}
-
-/* Pass-3 derivative numerator datmp (eff.c:2289-2334), same case split. */
-template <typename R>
-__device__ __forceinline__ R ddescreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
-                                       R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
+__device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
+                                        R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip, R &dv)
 {
     const DsCase k = ds_case<R>(ri, sj, dij, rgbmax);
     skip = k.skip;
@@ -193,34 +159,79 @@ __device__ __forceinline__ R ddescreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
     const R sj2 = sj * sj;
     const R dij2i = dij1i * dij1i;
     const R dij3i = dij2i * dij1i;
-    R v = R(0);
+    R v = R(0), w = R(0);
     if (k.a) {
+        v = R(0.125) * dij1i * (R(1) + R(2) * dij * u2 + rgbmax2i * (r2 - R(4) * rgbmax * dij - sj2) + R(2) * L);
         /* -2 log(rgbmax/(dij - sj)) = 2 L */
-        v = R(0.125) * dij3i * ((r2 + sj2) * (u2 * u2 - rgbmax2i) + R(2) * L);
+        w = R(0.125) * dij3i * ((r2 + sj2) * (u2 * u2 - rgbmax2i) + R(2) * L);
     } else if (k.b) {
         const R tmpsd = sj2 * dij2i;
-        const R dumbo = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
-        v = tmpsd * sj * dij2i * dij2i * dumbo;
+        const R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
+        const R dumbd = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
+        v = sj * tmpsd * dij2i * dumbo;
+        w = tmpsd * sj * dij2i * dij2i * dumbd;
     } else if (k.c) {
-        v = u2 * sj * (R(-0.5) * dij2i + u2) + R(0.25) * dij3i * L;
+        v = R(0.5) * (sj * u2 + R(0.5) * dij1i * L);
+        w = u2 * sj * (R(-0.5) * dij2i + u2) + R(0.25) * dij3i * L;
     } else if (k.d) {
-        v = R(-0.25) * (R(-0.5) * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i + dij1i * u1 * (u1 - dij1i) - dij3i * L);
+        const R theta = R(0.5) * ri1i * dij1i * (r2 + ri * ri - sj2);
+        v = R(0.25) * (ri1i * (R(2) - theta) - u1 + dij1i * L);
+        w = R(-0.25) * (R(-0.5) * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i + dij1i * u1 * (u1 - dij1i) - dij3i * L);
     } else if (k.e) {
-        v = R(-0.5) * (sj * dij2i * u2 - R(2) * sj * u2 * u2 - R(0.5) * dij3i * L);
+        v = R(0.5) * (sj * u2 + R(0.5) * dij1i * L) + ri1i;
+        w = R(-0.5) * (sj * dij2i * u2 - R(2) * sj * u2 * u2 - R(0.5) * dij3i * L);
     }
+    dv = w;
     return v;
 }
 
-/* Derivative of the neck term, divided by dij (eff.c:2336-2360). */
 template <typename R>
-__device__ __forceinline__ R dneck_term(R dij, R pos, R val, R neckscale)
+__device__ __forceinline__ R descreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
+                                      R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
 {
-    R mdist = dij - pos;
-    R mdist2 = mdist * mdist;
-    R mdist6 = mdist2 * mdist2 * mdist2;
-    R t = R(1) + mdist2 + R(0.3) * mdist6;
-    t = t * t * dij;
-    return (R(2) * mdist + R(1.8) * mdist2 * mdist2 * mdist) * val * neckscale * gbm<R>::rcp(t);
+    R dv;
+    return descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip, dv);
+}
+
+template <typename R>
+__device__ __forceinline__ R ddescreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
+                                       R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
+{
+    R dv;
+    (void)descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip, dv);
+    return dv;
+}
+
+/*
+ * Neck correction for neck maximum (pos, val) (eff.c:1699-1708): returns
+ * the amount subtracted from sumi and sets dn to its pass-3 derivative
+ * divided by dij (eff.c:2336-2360); t = 1 + m^2 + 0.3 m^6, m = dij - pos.
+ */
+template <typename R>
+__device__ __forceinline__ R neck_d(R dij, R dij1i, R pos, R val, R neckscale, R &dn)
+{
+    const R mdist = dij - pos;
+    const R mdist2 = mdist * mdist;
+    const R mdist6 = mdist2 * mdist2 * mdist2;
+    const R rt = gbm<R>::rcp(R(1) + mdist2 + R(0.3) * mdist6);
+    const R nv = neckscale * val * rt;
+    dn = (R(2) * mdist + R(1.8) * mdist2 * mdist2 * mdist) * nv * rt * dij1i;
+    return nv;
+}
+
+template <typename R>
+__device__ __forceinline__ R neck_term(R dij, R pos, R val, R neckscale)
+{
+    R dn;
+    return neck_d<R>(dij, R(1), pos, val, neckscale, dn);
+}
+
+template <typename R>
+__device__ __forceinline__ R dneck_term(R dij, R dij1i, R pos, R val, R neckscale)
+{
+    R dn;
+    (void)neck_d<R>(dij, dij1i, pos, val, neckscale, dn);
+    return dn;
 }
 
 #endif

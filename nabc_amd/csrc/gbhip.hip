@@ -49,6 +49,7 @@
 #include "neck_tables.h"
 
 #define PAIR_BLOCKS_MAX 4096
+#define CHAIN_DEPTH 4  /* derivative-store loads in flight per lane in pass 3 */
 
 /* ------------------------------------------------------------------ errors */
 
@@ -286,11 +287,16 @@ __global__ __launch_bounds__(64) void k_binscan(DevPtrs d)
 {
     if (threadIdx.x == 0) {
         int off = 0;
+        long long slots = 0;
         for (int b = 0; b < TILE; b++) {
             d.tbin[TILE + b] = off;
+            d.binslot[b] = slots;
             off += d.tbin[b];
+            slots += (long long)d.tbin[b] * (TILE - b) * TILE;
         }
         *d.nnz = off;
+        *d.store_need = slots;
+        *d.store_ok = d.dstore != nullptr && slots <= d.store_cap;
     }
 }
 
@@ -313,6 +319,11 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
         if (m) d.tord[base[TILE - m] + r] = t;
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(64) void k_setflag(int *flag, int v)
+{
+    if (threadIdx.x == 0) *flag = v;
 }
 
 /*
@@ -366,20 +377,45 @@ template <typename R> __device__ __forceinline__ const R *neckval(const DevPtrs 
 template <> __device__ __forceinline__ const double *neckval<double>(const DevPtrs &d) { return d.neck_val; }
 template <> __device__ __forceinline__ const float *neckval<float>(const DevPtrs &d) { return d.neck_val_f; }
 
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { typedef double2 T; };
+template <> struct Vec2<float> { typedef float2 T; };
+
+/*
+ * Derivative store: pass 1 writes, for every lane-step of every non-empty
+ * tile pair, the two pass-3 derivative numerators (datmp of i by j, datmp
+ * of j by i, zero when the pair is beyond rgbmax + Fsmax), and pass 3 reads
+ * them back instead of recomputing the descreening case split, logarithms
+ * and reciprocals.  Tile pairs are stored in processing order, m * 64
+ * entries each (entry s * 64 + l = step s of lane l); the offset of queue
+ * position q follows from the counting-sort bins.
+ */
+__device__ __forceinline__ long long store_base(const DevPtrs &d, int q, int m)
+{
+    const int b = TILE - m;
+    const int start = d.tbin[TILE + b] - d.tbin[b];
+    return d.binslot[b] + (long long)(q - start) * m * TILE;
+}
+
 /*
  * Pass 1 (eff.c:1566-1733): for every listed pair, the descreening of i by
  * j (added to i's sum) and of j by i (added to j's sum).  The r^2 test
  * against (rgbmax+Fsmax)^2 and the FP64 sums are kept in both variants.
+ * When the derivative store is large enough for the current list, the
+ * pass-3 numerators of both directions are written as well.
  */
 template <typename R>
 __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
 {
+    typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE];
     __shared__ int s_nk[TILE], s_col[TILE];
     __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
+    const bool store = *d.store_ok;
+    R2 *st = (R2 *)d.dstore;
     const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
     const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
     if (p.has_neck) {
@@ -392,7 +428,6 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const int t = d.tord[q];
         const unsigned long long cols = d.tcol[t];
         const int m = __popcll(cols);
-        if (m == 0) continue;
         const int2 ab = d.tpo[t];
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
@@ -403,6 +438,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
+        R2 *sp = store ? st + store_base(d, q, m) + l : nullptr;
         __syncthreads();
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
@@ -417,6 +453,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         double acc = 0.0;
         int k = l % m;
         for (int s = 0; s < m; s++) {
+            R aij = R(0), aji = R(0);
             if ((mr >> s_col[k]) & 1ull) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
                 const double r2d = dx * dx + dy * dy + dz * dz;
@@ -426,18 +463,32 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                     const R dij = r2 * dij1i;
                     const R rj = s_ri[k], sj = s_sj[k];
                     bool skip_ij, skip_ji;
-                    R tij = descreen<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij);
-                    R tji = descreen<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji);
+                    R tij = descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, aij);
+                    R tji = descreen_d<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji, aji);
                     if (p.has_neck) {
                         const int nj = s_nk[k];
                         if (dij < rbi + s_rb[k] + neckcut) {
-                            if (!skip_ij) tij += neck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
-                            if (!skip_ji) tji += neck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                            R dn;
+                            if (!skip_ij) {
+                                tij += neck_d<R>(dij, dij1i, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale, dn);
+                                aij += dn;
+                            }
+                            if (!skip_ji) {
+                                tji += neck_d<R>(dij, dij1i, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale, dn);
+                                aji += dn;
+                            }
                         }
                     }
                     acc -= (double)tij;
                     atomicAdd(&s_acc[k], -(double)tji);
                 }
+            }
+            if (store) {
+                /* every lane, listed pair or not: whole 512-byte rows */
+                R2 v;
+                v.x = aij;
+                v.y = aji;
+                sp[s * TILE] = v;
             }
             k = k + 1 == m ? 0 : k + 1;
         }
@@ -649,7 +700,84 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
  * opposite to j, and from row j the same with i and j swapped; both rows
  * are done here in one visit: F = xij*(datmp_ij*w_i + datmp_ji*w_j),
  * grad_i -= F, grad_j += F, with w = sumdeijda * dR/dsum (0 if frozen).
+ * Two kernels, both launched, exactly one of which does the work, as the
+ * device flag of the current list says: k_chain_st reads datmp from the
+ * derivative store written by pass 1 (memory-bound, few registers, high
+ * occupancy); k_chain recomputes it (the store could not hold the list).
  */
+template <typename R>
+__global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
+{
+    typedef typename Vec2<R>::T R2;
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
+    __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
+    __shared__ R s_w[TILE];
+    __shared__ int s_col[TILE];
+    if (!*d.store_ok) return;
+    const int l = threadIdx.x;
+    const int nnz = *d.nnz;
+    const R2 *st = (const R2 *)d.dstore;
+    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+        const int t = d.tord[q];
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        const int2 ab = d.tpo[t];
+        const int i = ab.x * TILE + l, j = ab.y * TILE + l;
+        const R2 *sp = st + store_base(d, q, m) + l;
+        /* rows of the store are read unconditionally (every lane wrote its
+         * entry) through a window of CHAIN_DEPTH loads in flight */
+        R2 win[CHAIN_DEPTH];
+#pragma unroll
+        for (int u = 0; u < CHAIN_DEPTH; u++)
+            if (u < m) win[u] = sp[u * TILE];
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
+        const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        const bool own = (cols >> l) & 1ull;
+        const int pos = __popcll(cols & ((1ull << l) - 1ull));
+        __syncthreads();
+        if (own) {
+            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
+            s_col[pos] = l;
+            s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
+        }
+        __syncthreads();
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        int k = l % m;
+        for (int s0 = 0; s0 < m; s0 += CHAIN_DEPTH) {
+#pragma unroll
+            for (int u = 0; u < CHAIN_DEPTH; u++) {
+                const int s = s0 + u;
+                if (s < m) {
+                    const R2 a = win[u];
+                    if (s + CHAIN_DEPTH < m) win[u] = sp[(s + CHAIN_DEPTH) * TILE];
+                    if ((mr >> s_col[k]) & 1ull) {
+                        const double f = (double)(a.x * wi + a.y * s_w[k]);
+                        const double gxv = f * (xi - s_x[k]), gyv = f * (yi - s_y[k]), gzv = f * (zi - s_z[k]);
+                        fxi -= gxv; fyi -= gyv; fzi -= gzv;
+                        atomicAdd(&s_fx[k], gxv);
+                        atomicAdd(&s_fy[k], gyv);
+                        atomicAdd(&s_fz[k], gzv);
+                    }
+                    k = k + 1 == m ? 0 : k + 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
+            unsafeAtomicAdd(&d.gx[i], fxi);
+            unsafeAtomicAdd(&d.gy[i], fyi);
+            unsafeAtomicAdd(&d.gz[i], fzi);
+        }
+        if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
+            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+        }
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
 {
@@ -658,6 +786,7 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE], s_w[TILE];
     __shared__ int s_nk[TILE], s_col[TILE];
     __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
+    if (*d.store_ok) return;
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
@@ -672,7 +801,6 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
         const int t = d.tord[q];
         const unsigned long long cols = d.tcol[t];
         const int m = __popcll(cols);
-        if (m == 0) continue;
         const int2 ab = d.tpo[t];
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
@@ -713,8 +841,8 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
                     if (p.has_neck) {
                         const int nj = s_nk[k];
                         if (dij < rbi + s_rb[k] + neckcut) {
-                            if (!skip_ij) aij += dneck_term<R>(dij, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
-                            if (!skip_ji) aji += dneck_term<R>(dij, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
+                            if (!skip_ij) aij += dneck_term<R>(dij, dij1i, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale);
+                            if (!skip_ji) aji += dneck_term<R>(dij, dij1i, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale);
                         }
                     }
                     const double f = (double)(aij * wi + aji * s_w[k]);
@@ -996,6 +1124,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
         (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
         (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
+        (rc = E.dalloc(&D.binslot, TILE)) || (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
         (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
         (rc = E.dalloc(&D.bbhi, 3 * (size_t)NT)) || (rc = E.dalloc(&E.x_dev, 3 * (size_t)E.natoms)) ||
@@ -1004,6 +1133,10 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         return rc;
     HIPCHK(hipMemsetAsync(D.ntp, 0, sizeof(int), E.stream));
     HIPCHK(hipMemsetAsync(D.nnz, 0, sizeof(int), E.stream));
+    HIPCHK(hipMemsetAsync(D.store_ok, 0, sizeof(int), E.stream));
+    HIPCHK(hipMemsetAsync(D.store_need, 0, sizeof(long long), E.stream));
+    D.dstore = nullptr;
+    D.store_cap = 0;
     HIPCHK(hipMemsetAsync(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms, E.stream));
     if (x0) {
         for (int m = 0; m < nmol; m++)
@@ -1051,6 +1184,71 @@ int engine_nblist(Engine &E)
     return 0;
 }
 
+/*
+ * Size the pass-3 derivative store for the list just built (synchronises the
+ * engine stream).  Grows it to 1.25x the need when it is too small, unless a
+ * captured graph holds the current pointer or the device lacks the memory;
+ * in those cases the list is evaluated with pass 3 recomputing the
+ * derivatives (the device flag written at list build says which).
+ */
+int engine_store_reserve(Engine &E)
+{
+    long long need = 0;
+    HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (E.store_mode == 0 || E.store_frozen) return 0;
+    if (E.d.dstore && need <= E.d.store_cap) return 0;
+    const size_t esz = E.precision == GBHIP_FP64 ? 2 * sizeof(double) : 2 * sizeof(float);
+    const long long cap = need + need / 4 + (long long)TILE * TILE;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    if (E.store_mem) {
+        HIPCHK(hipStreamSynchronize(E.stream));
+        (void)hipFree(E.store_mem);
+        E.store_mem = nullptr;
+        E.d.dstore = nullptr;
+        E.d.store_cap = 0;
+    }
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess || (size_t)cap * esz > freeb / 2) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, (size_t)cap * esz) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    E.store_mem = p;
+    E.d.dstore = p;
+    E.d.store_cap = cap;
+    hipLaunchKernelGGL(k_setflag, dim3(1), dim3(64), 0, E.stream, E.d.store_ok, 1);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
+static int set_store_mode(Engine &E, int mode)
+{
+    if (mode != 0 && mode != 1) return seterr(GBHIP_EINVAL, "derivative store mode must be 0 or 1");
+    if (E.store_frozen && mode != E.store_mode)
+        return seterr(GBHIP_ESTATE, "derivative store mode cannot change while dynamics graphs exist");
+    E.store_mode = mode;
+    if (mode == 0) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        HIPCHK(hipStreamSynchronize(E.stream));
+        if (E.store_mem) (void)hipFree(E.store_mem);
+        E.store_mem = nullptr;
+        E.d.dstore = nullptr;
+        E.d.store_cap = 0;
+        hipLaunchKernelGGL(k_setflag, dim3(1), dim3(64), 0, E.stream, E.d.store_ok, 0);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(E.stream));
+    } else if (E.have_list) {
+        return engine_store_reserve(E);
+    }
+    return 0;
+}
+
 /* one evaluation; coordinates already in E.x_dev or x_dev */
 int engine_eval(Engine &E, const double *x_dev, double *g_dev)
 {
@@ -1076,10 +1274,13 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
     else
         hipLaunchKernelGGL(k_pairs<float>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
-    if (E.precision == GBHIP_FP64)
+    if (E.precision == GBHIP_FP64) {
+        hipLaunchKernelGGL(k_chain_st<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
         hipLaunchKernelGGL(k_chain<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
-    else
+    } else {
+        hipLaunchKernelGGL(k_chain_st<float>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
         hipLaunchKernelGGL(k_chain<float>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (g_dev)
         hipLaunchKernelGGL(k_scatter, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev);
@@ -1181,6 +1382,33 @@ static int autotune(Engine &E, int reps, int *chosen3)
     if (chosen3)
         for (int k = 0; k < 3; k++) chosen3[k] = pick[k];
     return engine_eval(E, E.x_dev, nullptr);
+}
+
+extern "C" int gbhip_set_derivative_store(gbhip_ctx *ctx, int mode)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return set_store_mode(ctx->E, mode);
+}
+
+extern "C" int gbhip_batch_set_derivative_store(gbhip_batch *b, int mode)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return set_store_mode(b->E, mode);
+}
+
+extern "C" int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap, int *in_use)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    Engine &E = ctx->E;
+    long long nd = 0;
+    int ok = 0;
+    HIPCHK(hipMemcpyAsync(&nd, E.d.store_need, sizeof(nd), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipMemcpyAsync(&ok, E.d.store_ok, sizeof(ok), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (need) *need = nd;
+    if (cap) *cap = E.d.store_cap;
+    if (in_use) *in_use = ok;
+    return 0;
 }
 
 extern "C" int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3)
@@ -1314,6 +1542,7 @@ extern "C" int gbhip_nblist(gbhip_ctx *ctx, long long *nentries)
 {
     int rc = gbhip_nblist_async(ctx);
     if (rc) return rc;
+    if ((rc = engine_store_reserve(ctx->E))) return rc;
     long long np = 0;
     if ((rc = gbhip_npairs(ctx, &np))) return rc;
     if (nentries) *nentries = 2 * np;
@@ -1450,6 +1679,7 @@ extern "C" int gbhip_batch_create(int nmol, const gbhip_topology *tops, const gb
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     int rc = engine_build(b->E, nmol, tops, opt, x0);
     if (!rc) rc = engine_nblist(b->E);
+    if (!rc) rc = engine_store_reserve(b->E);
     if (!rc && hipStreamSynchronize(b->E.stream) != hipSuccess) rc = seterr(GBHIP_EHIP, "pair-list build failed");
     if (rc) {
         delete b;

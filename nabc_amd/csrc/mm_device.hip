@@ -547,6 +547,8 @@ static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t 
     const int saved_cap = E.tev_cap;
     E.tev_cap = 0; /* no timing events inside a graph */
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    /* the graph keeps the derivative-store pointer it was captured with */
+    E.store_frozen = true;
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
     int rc = md_step_launch(c, rebuild);
     hipError_t e2 = hipStreamEndCapture(E.stream, g);
@@ -620,16 +622,19 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
                        o->tempi, o->zerov, (o->tempi > 0.0 || o->zerov) ? 0 : 1);
     hipLaunchKernelGGL(k_md_init_done, dim3(1), dim3(1), 0, E.stream, M);
     LAUNCHCHK();
-    /* step 0: list build and forces (md() calls mme with iter 0) */
-    if ((rc = engine_nblist(E))) return rc;
-    if ((rc = mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
-    hipLaunchKernelGGL(k_md_accel0, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, ctx->B.frozen,
-                       (const double *)E.g_dev);
-    LAUNCHCHK();
+    /* graphs of an earlier run hold stale launch arguments */
     if (M.g_step) { (void)hipGraphExecDestroy(M.g_step); M.g_step = nullptr; }
     if (M.g_rebuild) { (void)hipGraphExecDestroy(M.g_rebuild); M.g_rebuild = nullptr; }
     if (M.gr_step) { (void)hipGraphDestroy(M.gr_step); M.gr_step = nullptr; }
     if (M.gr_rebuild) { (void)hipGraphDestroy(M.gr_rebuild); M.gr_rebuild = nullptr; }
+    E.store_frozen = false;
+    /* step 0: list build and forces (md() calls mme with iter 0) */
+    if ((rc = engine_nblist(E))) return rc;
+    if ((rc = engine_store_reserve(E))) return rc;
+    if ((rc = mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
+    hipLaunchKernelGGL(k_md_accel0, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, ctx->B.frozen,
+                       (const double *)E.g_dev);
+    LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(E.stream));
     M.ready = true;
     return 0;

@@ -52,7 +52,7 @@ double job_mme(double *x, double *f, int *iter)
     if (*iter == -3) return 0.0;
     if (gbhip_set_coords(J->ctx, x) != 0) { J->fail = 1; return 1e300; }
     if (!J->have_list || (*iter > 0 && *iter % J->nsnb == 0) || *iter == -4) {
-        if (gbhip_nblist_async(J->ctx) != 0) { J->fail = 1; return 1e300; }
+        if (gbhip_nblist(J->ctx, nullptr) != 0) { J->fail = 1; return 1e300; }
         J->have_list = true;
     }
     if (gbhip_mme(J->ctx, f, J->ene) != 0) { J->fail = 1; return 1e300; }
@@ -199,7 +199,11 @@ extern "C" size_t gbhip_footprint_bytes(int natom, int md_steps)
     size_t b = 0;
     b += NS * (6 * 4 + 7 * 8 + 4 * 4 + 11 * 8);          /* per-slot constants and work arrays */
     b += NT * (4 * 4 + 8 + 2 * 24);                       /* tile arrays */
-    b += cap * (4 + 8 + 2 * TILE_ATOMS * 8);              /* candidates, tile pairs, bit matrices */
+    b += cap * (4 + 8 + 8 + 4 + 8 + 4 + 2 * TILE_ATOMS * 8); /* candidates, tile pairs (both orders, flags,
+                                                              column sets, queue), bit matrices */
+    /* pass-3 derivative store: at most every tile pair full, 16 bytes per
+     * entry (FP64), 1.25x headroom */
+    b += (cap * TILE_ATOMS * TILE_ATOMS + TILE_ATOMS * TILE_ATOMS) * 20;
     b += (size_t)natom * (4 + 4 + 12 + 2 * 24 + 8);       /* inverse order, exclusions, x, g, scratch */
     b += (size_t)natom * 220;                              /* bonded / 1-4 tables (chain topology bound) */
     b += (size_t)natom * 4 * 24 + 16 * 8 * (size_t)(md_steps > 0 ? md_steps : 0); /* dynamics state + log */

@@ -187,3 +187,53 @@ def test_cg_trajectory_on_gpu(gpu):
         assert abs(ene[2] + ene[8] - w["elect"]) <= 0.0051
     it = na.C.c_int(-3)
     na.lib.mme(xc.ctypes.data_as(na.dp), xc.ctypes.data_as(na.dp), na.C.byref(it))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_derivative_store_off_matches_oracle(gpu, gen_dir, precision):
+    """Pass 3 recomputing the derivatives (store disabled) gives the same
+    answers as with the pass-1 store."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 11, gen_dir)
+    ctx, orc, _ = _compare(na, parm, arr, x, gb=8, cut=12.0, rgbmax=10.0, kappa=0.127315,
+                           precision=precision)
+    need, cap, used = ctx.store_stats()
+    assert used and 0 < need <= cap
+    e1 = ctx.egb()
+    ctx.set_derivative_store(0)
+    ctx.nblist()
+    assert ctx.store_stats()[2] is False
+    e0 = ctx.egb()
+    r = orc.egb(x)
+    tol_e = FP64_E if precision == 0 else MIX_E
+    tol_g = FP64_G if precision == 0 else MIX_G
+    assert rel(e0[0], r[0]) <= tol_e and rel_rms(e0[3], r[3]) <= tol_g
+    # same list, same pair arithmetic: the two paths agree far below the bar
+    assert rel(e0[0], e1[0]) <= 1e-12 and rel_rms(e0[3], e1[3]) <= (1e-12 if precision == 0 else 1e-6)
+
+
+def test_derivative_store_outgrown_by_async_rebuild(gpu, gen_dir):
+    """An asynchronous rebuild whose list no longer fits the store falls back
+    to recomputation (and stays correct); the next synchronous build grows
+    the store and uses it again."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 5, gen_dir)
+    ctx = na.GBContext(arr, x, gb=5, cut=10.0, rgbmax=10.0, kappa=0.1)
+    ctx.nblist()
+    need0, cap0, used0 = ctx.store_stats()
+    assert used0
+    xc = x * 0.8  # denser: about twice the pairs
+    ctx.set_coords(xc)
+    ctx.nblist_async()
+    need1, cap1, used1 = ctx.store_stats()
+    assert need1 > cap0 and cap1 == cap0 and not used1
+    orc = oracle.Oracle(parm, arr["natom"], gb=5, cut=10.0, rgbmax=10.0, kappa=0.1)
+    orc.nblist(xc)
+    r = orc.egb(xc)
+    e = ctx.egb()
+    assert rel(e[0], r[0]) <= FP64_E and rel(e[2], r[2]) <= FP64_E and rel_rms(e[3], r[3]) <= FP64_G
+    ctx.nblist()
+    need2, cap2, used2 = ctx.store_stats()
+    assert used2 and cap2 >= need2 == need1
+    e = ctx.egb()
+    assert rel(e[0], r[0]) <= FP64_E and rel_rms(e[3], r[3]) <= FP64_G
