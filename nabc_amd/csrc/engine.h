@@ -69,7 +69,8 @@ struct DevPtrs {
     int *tord;            /* non-empty tile pairs, most non-empty columns first */
     int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
     int *nnz;             /* number of non-empty tile pairs */
-    long long *binslot;   /* derivative-store offset of each bin */
+    int *tnp;             /* listed pairs per tile pair */
+    long long *tstart;    /* derivative-store offset per queue position */
     long long *store_need; /* entries the current list needs */
     int *store_ok;        /* 1 when the store holds the current list */
     void *dstore;         /* pass-3 derivative numerators (float2 / double2) */

@@ -222,7 +222,10 @@ __global__ __launch_bounds__(64) void k_masks(DevPtrs d, GBPar p)
         }
         d.mask[(size_t)t * TILE + l] = w;
         d.excl[(size_t)t * TILE + l] = 0ull;
+        int np = __popcll(w);
+        for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
         if (l == 0) {
+            d.tnp[t] = np;
             d.tpo[t] = flip ? make_int2(ij.y, ij.x) : ij;
             d.tflip[t] = flip;
             d.tcol[t] = flip ? rows : cols;
@@ -287,16 +290,11 @@ __global__ __launch_bounds__(64) void k_binscan(DevPtrs d)
 {
     if (threadIdx.x == 0) {
         int off = 0;
-        long long slots = 0;
         for (int b = 0; b < TILE; b++) {
             d.tbin[TILE + b] = off;
-            d.binslot[b] = slots;
             off += d.tbin[b];
-            slots += (long long)d.tbin[b] * (TILE - b) * TILE;
         }
         *d.nnz = off;
-        *d.store_need = slots;
-        *d.store_ok = d.dstore != nullptr && slots <= d.store_cap;
     }
 }
 
@@ -318,6 +316,38 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
         __syncthreads();
         if (m) d.tord[base[TILE - m] + r] = t;
         __syncthreads();
+    }
+}
+
+/*
+ * Derivative-store offsets: exclusive prefix sum of the listed-pair counts
+ * in processing order (one block), the store size the list needs, and the
+ * flag saying whether the store allocated holds it.
+ */
+__global__ __launch_bounds__(1024) void k_qscan(DevPtrs d)
+{
+    __shared__ long long s_part[1024];
+    const int n = *d.nnz, t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    long long sum = 0;
+    for (int q = b; q < e; q++) sum += d.tnp[d.tord[q]];
+    s_part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    long long run = s_part[t] - sum;
+    for (int q = b; q < e; q++) {
+        d.tstart[q] = run;
+        run += d.tnp[d.tord[q]];
+    }
+    if (t == 1023) {
+        *d.store_need = s_part[1023];
+        *d.store_ok = d.dstore != nullptr && s_part[1023] <= d.store_cap;
     }
 }
 
@@ -382,19 +412,18 @@ template <> struct Vec2<double> { typedef double2 T; };
 template <> struct Vec2<float> { typedef float2 T; };
 
 /*
- * Derivative store: pass 1 writes, for every lane-step of every non-empty
+ * Derivative store: pass 1 writes, for every listed pair of every non-empty
  * tile pair, the two pass-3 derivative numerators (datmp of i by j, datmp
- * of j by i, zero when the pair is beyond rgbmax + Fsmax), and pass 3 reads
+ * of j by i; zero when the pair is beyond rgbmax + Fsmax), and pass 3 reads
  * them back instead of recomputing the descreening case split, logarithms
- * and reciprocals.  Tile pairs are stored in processing order, m * 64
- * entries each (entry s * 64 + l = step s of lane l); the offset of queue
- * position q follows from the counting-sort bins.
+ * and reciprocals.  Tile pairs are stored in processing order from
+ * tstart[q]; within a tile pair, step by step, the entries of the lanes
+ * holding a listed pair at that step, packed in lane order (position =
+ * entries of earlier steps + listed lanes below this one at this step).
  */
-__device__ __forceinline__ long long store_base(const DevPtrs &d, int q, int m)
+__device__ __forceinline__ int lanes_below(unsigned long long bal)
 {
-    const int b = TILE - m;
-    const int start = d.tbin[TILE + b] - d.tbin[b];
-    return d.binslot[b] + (long long)(q - start) * m * TILE;
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
 }
 
 /*
@@ -438,7 +467,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
-        R2 *sp = store ? st + store_base(d, q, m) + l : nullptr;
+        R2 *sp = store ? st + d.tstart[q] : nullptr;
         __syncthreads();
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
@@ -454,7 +483,8 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         int k = l % m;
         for (int s = 0; s < m; s++) {
             R aij = R(0), aji = R(0);
-            if ((mr >> s_col[k]) & 1ull) {
+            const bool listed = (mr >> s_col[k]) & 1ull;
+            if (listed) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
                 const double r2d = dx * dx + dy * dy + dz * dz;
                 if (r2d <= rp2) {
@@ -484,11 +514,14 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                 }
             }
             if (store) {
-                /* every lane, listed pair or not: whole 512-byte rows */
-                R2 v;
-                v.x = aij;
-                v.y = aji;
-                sp[s * TILE] = v;
+                const unsigned long long bal = __ballot(listed);
+                if (listed) {
+                    R2 v;
+                    v.x = aij;
+                    v.y = aji;
+                    sp[lanes_below(bal)] = v;
+                }
+                sp += __popcll(bal);
             }
             k = k + 1 == m ? 0 : k + 1;
         }
@@ -723,13 +756,7 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         const int m = __popcll(cols);
         const int2 ab = d.tpo[t];
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
-        const R2 *sp = st + store_base(d, q, m) + l;
-        /* rows of the store are read unconditionally (every lane wrote its
-         * entry) through a window of CHAIN_DEPTH loads in flight */
-        R2 win[CHAIN_DEPTH];
-#pragma unroll
-        for (int u = 0; u < CHAIN_DEPTH; u++)
-            if (u < m) win[u] = sp[u * TILE];
+        const R2 *sp = st + d.tstart[q];
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
@@ -743,16 +770,42 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
         }
         __syncthreads();
+        /* the packed entries of the next CHAIN_DEPTH steps are in flight
+         * while a step is processed: kp/sp walk ahead of k */
+        const int k0 = l % m;
+        int kp = k0;
+        R2 win[CHAIN_DEPTH];
+        bool act[CHAIN_DEPTH];
+#pragma unroll
+        for (int u = 0; u < CHAIN_DEPTH; u++) {
+            act[u] = false;
+            if (u < m) {
+                const bool a = (mr >> s_col[kp]) & 1ull;
+                const unsigned long long bal = __ballot(a);
+                if (a) win[u] = sp[lanes_below(bal)];
+                act[u] = a;
+                sp += __popcll(bal);
+                kp = kp + 1 == m ? 0 : kp + 1;
+            }
+        }
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        int k = l % m;
+        int k = k0;
         for (int s0 = 0; s0 < m; s0 += CHAIN_DEPTH) {
 #pragma unroll
             for (int u = 0; u < CHAIN_DEPTH; u++) {
                 const int s = s0 + u;
                 if (s < m) {
                     const R2 a = win[u];
-                    if (s + CHAIN_DEPTH < m) win[u] = sp[(s + CHAIN_DEPTH) * TILE];
-                    if ((mr >> s_col[k]) & 1ull) {
+                    const bool on = act[u];
+                    if (s + CHAIN_DEPTH < m) {
+                        const bool an = (mr >> s_col[kp]) & 1ull;
+                        const unsigned long long bal = __ballot(an);
+                        if (an) win[u] = sp[lanes_below(bal)];
+                        act[u] = an;
+                        sp += __popcll(bal);
+                        kp = kp + 1 == m ? 0 : kp + 1;
+                    }
+                    if (on) {
                         const double f = (double)(a.x * wi + a.y * s_w[k]);
                         const double gxv = f * (xi - s_x[k]), gyv = f * (yi - s_y[k]), gzv = f * (zi - s_z[k]);
                         fxi -= gxv; fyi -= gyv; fzi -= gzv;
@@ -1124,7 +1177,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
         (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
         (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
-        (rc = E.dalloc(&D.binslot, TILE)) || (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
+        (rc = E.dalloc(&D.tnp, (size_t)cap)) || (rc = E.dalloc(&D.tstart, (size_t)cap)) ||
+        (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
         (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
         (rc = E.dalloc(&D.bbhi, 3 * (size_t)NT)) || (rc = E.dalloc(&E.x_dev, 3 * (size_t)E.natoms)) ||
@@ -1177,6 +1231,7 @@ int engine_nblist(Engine &E)
     hipLaunchKernelGGL(k_bincount, dim3(nsb), dim3(256), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_binscan, dim3(1), dim3(64), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_binscatter, dim3(nsb), dim3(256), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_qscan, dim3(1), dim3(1024), 0, E.stream, E.d);
     if (E.natoms > 0)
         hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
     LAUNCHCHK();
