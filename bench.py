@@ -31,10 +31,28 @@ CUT = 25.0
 RGBMAX = 25.0
 KAPPA = 0.127315
 NSNB = 20
-# algorithmic flops per unordered pair and pass for gb 7/8: pass 1 two
-# directions x (24 + 10 neck), pass 2 68, pass 3 two directions x (37 + 15)
-FLOPS = {"born": 68.0, "pairs": 68.0, "chain": 104.0}
+# Algorithmic flops per listed (unordered) pair, from the reference's
+# expressions on the most common (Taylor) branch with +,-,x = 1, FMA = 2 and
+# each sqrt/1/x/exp/log = 1: pass 1 F1 = 24 per ordered pair (+10 neck),
+# pass 2 F2 = 68 per unordered pair, pass 3 F3 = 37 per ordered pair (+15
+# neck), i.e. W = 240 per pair for gb 7/8.  Split along this build's kernel
+# boundaries: k_born does pass 1 and the datmp part of pass 3 (2 x (28 +
+# 15)), k_chain_st the pass-3 force application (2 x 9), k_pairs pass 2.
+FLOPS = {"born": 2 * (24 + 10) + 2 * (28 + 15), "pairs": 68.0, "chain": 2 * 9.0}
+W_PER_PAIR = 240.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
+# HBM bytes per launch of the pair kernels at this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the copy-kernel
+# calibration (profiles/gb_25k_igb8_hbm_traffic.md)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gb_25k_igb8_traffic.json")
+
+
+def measured_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as fp:
+            return json.load(fp)["kernels"].get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def sha256(path):
@@ -268,6 +286,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop")
@@ -293,6 +313,15 @@ def main():
     import numpy as np
     import nabc_amd as na
 
+    if args.copy_probe > 0:
+        import torch
+        a = torch.ones(args.copy_probe * 262144, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     with tempfile.TemporaryDirectory() as d:
         top, rst = os.path.join(d, "c3.prmtop"), os.path.join(d, "c3.rst7")
         na.generate(NATOM, SEED, top, rst)
@@ -312,11 +341,15 @@ def main():
     P = mixed["npairs"]
     dom = max(("born", "pairs", "chain"), key=lambda k: mixed["pass_ms"][k])
     achieved = FLOPS[dom] * P / (mixed["pass_ms"][dom] * 1e-3) / 1e12
+    kname = {"born": "k_born<float>", "pairs": "k_pairs<float>", "chain": "k_chain_st<float>"}[dom]
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK["mixed"], unit="TFLOP/s",
-                    frac=round(achieved / PEAK["mixed"], 4), traffic=None, kernel="k_%s<float>" % dom,
+                    frac=round(achieved / PEAK["mixed"], 4), traffic=measured_traffic(kname), kernel=kname,
                     kernel_ms=round(mixed["pass_ms"][dom], 4))
     dom64 = max(("born", "pairs", "chain"), key=lambda k: fp64["pass_ms"][k])
     a64 = FLOPS[dom64] * P / (fp64["pass_ms"][dom64] * 1e-3) / 1e12
+    # whole-evaluation figure: W = 240 P flops per evaluation at the measured rate
+    roofline["eval_achieved"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12, 3)
+    roofline["eval_frac"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12 / PEAK["mixed"], 4)
 
     cpu = None
     if not args.no_cpu:
@@ -343,7 +376,8 @@ def main():
                    "derivative_store": {"mixed": mixed["store"], "fp64": fp64["store"]}},
         "fp64": {"value": round(fp64["evals_per_s"], 2), "ms_per_step": round(fp64["ms_per_eval"], 4),
                  "pass_ms": {k: round(v, 4) for k, v in fp64["pass_ms"].items()},
-                 "dominant": "k_%s<double>" % dom64, "achieved_tflops": round(a64, 3),
+                 "dominant": {"born": "k_born<double>", "pairs": "k_pairs<double>",
+                              "chain": "k_chain_st<double>"}[dom64], "achieved_tflops": round(a64, 3),
                  "frac": round(a64 / PEAK["fp64"], 4)},
         "mixed_vs_fp64": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
         "mixed_pass_ms": {k: round(v, 4) for k, v in mixed["pass_ms"].items()},

@@ -70,6 +70,7 @@ struct DevPtrs {
     int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
     int *nnz;             /* number of non-empty tile pairs */
     int *tnp;             /* listed pairs per tile pair */
+    int *tnq;             /* the same in processing order */
     long long *tstart;    /* derivative-store offset per queue position */
     long long *store_need; /* entries the current list needs */
     int *store_ok;        /* 1 when the store holds the current list */

@@ -49,7 +49,7 @@
 #include "neck_tables.h"
 
 #define PAIR_BLOCKS_MAX 4096
-#define CHAIN_DEPTH 4  /* derivative-store loads in flight per lane in pass 3 */
+#define CHAIN_DEPTH 8  /* derivative-store loads in flight per lane in pass 3 */
 
 /* ------------------------------------------------------------------ errors */
 
@@ -314,7 +314,11 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
         if (threadIdx.x < TILE && h[threadIdx.x])
             base[threadIdx.x] = atomicAdd(&d.tbin[TILE + threadIdx.x], h[threadIdx.x]);
         __syncthreads();
-        if (m) d.tord[base[TILE - m] + r] = t;
+        if (m) {
+            const int q = base[TILE - m] + r;
+            d.tord[q] = t;
+            d.tnq[q] = d.tnp[t];
+        }
         __syncthreads();
     }
 }
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_qscan(DevPtrs d)
     const int per = (n + 1023) / 1024;
     const int b = min(n, t * per), e = min(n, b + per);
     long long sum = 0;
-    for (int q = b; q < e; q++) sum += d.tnp[d.tord[q]];
+    for (int q = b; q < e; q++) sum += d.tnq[q];
     s_part[t] = sum;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(1024) void k_qscan(DevPtrs d)
     long long run = s_part[t] - sum;
     for (int q = b; q < e; q++) {
         d.tstart[q] = run;
-        run += d.tnp[d.tord[q]];
+        run += d.tnq[q];
     }
     if (t == 1023) {
         *d.store_need = s_part[1023];
@@ -1177,7 +1181,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
         (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
         (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
-        (rc = E.dalloc(&D.tnp, (size_t)cap)) || (rc = E.dalloc(&D.tstart, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tnp, (size_t)cap)) || (rc = E.dalloc(&D.tnq, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tstart, (size_t)cap)) ||
         (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
         (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
