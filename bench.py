@@ -148,6 +148,7 @@ def bench_batch(args):
     for name, prec in (("fp64", na.GBHIP_FP64), ("mixed", na.GBHIP_MIXED)):
         b = na.Batch([(a, x) for _, a, x in systems], gb=5, precision=prec)
         npairs, natoms, ntp = b.stats()
+        launch = b.autotune(3) if not args.no_tune else None
         for _ in range(args.warmup):
             b.eval_device()
         ene, grads = b.eval()
@@ -161,7 +162,7 @@ def bench_batch(args):
         dt = (t1 - t0) / (args.steps + 1)
         res[name] = dict(batch_ms=1e3 * dt, molecules_per_s=nmol / dt, pairs_per_s=npairs / dt,
                          pass_ms={k: round(v / n, 4) for k, v in zip(("born", "radii", "pairs", "chain"), ms)},
-                         ene=ene, grads=grads, npairs=npairs, natoms=natoms, tilepairs=ntp)
+                         ene=ene, grads=grads, npairs=npairs, natoms=natoms, tilepairs=ntp, launch=launch)
         b.close()
     # per-instance check against the oracle
     worst = dict(fp64_e=0.0, fp64_g=0.0, mixed_e=0.0, mixed_g=0.0)
@@ -187,9 +188,13 @@ def bench_batch(args):
            "pass_ms": {k: res[k]["pass_ms"] for k in res},
            "npairs": res["mixed"]["npairs"], "natoms": res["mixed"]["natoms"],
            "tile_pairs": res["mixed"]["tilepairs"], "nmol": nmol,
+           "launch_blocks": {k: res[k]["launch"] for k in res},
            "worst_instance_error_vs_oracle": worst, "generate_s": round(t_gen, 1),
            "oracle_check_s": round(t_check, 1)}
     print(json.dumps(out))
+
+
+_MME_FN = None
 
 
 def bench_md(args):
@@ -197,17 +202,41 @@ def bench_md(args):
     hipGraph launch (bonded + 1-4 + GB on the device)."""
     import numpy as np
     import nabc_amd as na
+    global _MME_FN
+    _MME_FN = na.MMEFUNC(lambda xp, fp, itp: na.lib.mme(xp, fp, itp))
     natom, opts = {"c1": (600, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
                    "c2": (2500, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
                    "c3": (25000, "gb=8, cut=25.0, rgbmax=25.0, kappa=0.127315, nsnb=20")}[args.md]
     parm, arr, x0 = na.build_system(natom, SEED)
-    out = {"metric": "MD steps/s (%s, %d atoms)" % (args.md, natom), "unit": "steps/s"}
+    # the generated chain is strained (about 9 kcal/mol/atom above its
+    # relaxed energy), so dynamics starts from a conjugate-gradient minimum
+    # found through the same library (conjgrad over the device mme)
+    na.lib.mm_reset_options()
+    assert na.lib.mm_options(("%s, ntpr=100000000" % opts).encode()) == 0
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+    x0 = np.ascontiguousarray(x0.copy())
+    n3, fmin = na.C.c_int(3 * natom), na.C.c_double()
+    rms, dfp, mx = na.C.c_double(0.1), na.C.c_double(10.0), na.C.c_int(args.md_min)
+    t0 = time.perf_counter()
+    cg_ret = na.lib.conjgrad(x0.ctypes.data_as(na.dp), na.C.byref(n3), na.C.byref(fmin), _MME_FN,
+                             na.C.byref(rms), na.C.byref(dfp), na.C.byref(mx))
+    t_min = time.perf_counter() - t0
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+    out = {"metric": "MD steps/s (%s, %d atoms)" % (args.md, natom), "unit": "steps/s",
+           "start": {"cg_iterations_max": args.md_min, "cg_return": cg_ret, "energy": fmin.value,
+                     "seconds": round(t_min, 2)}}
     for name, prec in (("fp64", "fp64"), ("mixed", "mixed")):
         na.lib.mm_reset_options()
         assert na.lib.mm_options(("%s, gbprec=%s, dt=0.001, temp0=300, tempi=300, gamma_ln=5.0, ntpr=1000000, "
                                   "ntpr_md=1000000" % (opts, prec)).encode()) == 0
         assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
         ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+        if not args.no_tune:
+            # launch shapes tuned on the starting list, before the step graphs are captured
+            assert na.lib.gbhip_set_coords(ctx, x0.ctypes.data_as(na.dp)) == 0
+            assert na.lib.gbhip_nblist(ctx, None) == 0
+            assert na.lib.gbhip_autotune(ctx, 3, None) == 0
         masses = arr["mass"]
         o = na.MDOptions(dt=0.001, temp0=300.0, tempi=300.0, gamma_ln=5.0, tautp=999999.0, vlimit=10.0,
                          boltz2=9.93595e-4, nsnb=int(opts.split("nsnb=")[1]), zerov=0, seed=SEED)
@@ -298,6 +327,7 @@ def main():
     ap.add_argument("--nmol", type=int, default=1024)
     ap.add_argument("--md", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--md-steps", type=int, default=5000)
+    ap.add_argument("--md-min", type=int, default=500, help="CG iterations before dynamics")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

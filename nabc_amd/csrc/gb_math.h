@@ -101,7 +101,9 @@ template <> __device__ __forceinline__ double rsqrt_pair<float>(double r2)
  * Every case but B needs one logarithm and at most two reciprocals, so they
  * are computed once, for every lane, before the case split: a wavefront
  * whose lanes fall into several cases then issues one log and two rcp per
- * pair instead of one set per case present.  The arguments per case are
+ * step instead of one set per case present (measured: cheaper than guarding
+ * them with a branch that all-Taylor steps would skip).  The arguments per
+ * case are
  *   u1 = 1/(dij + sj)                       (C, D, E)
  *   u2 = 1/(dij - sj) for A, 1/(r2 - sj2)   (A, C, E)
  *   L  = log((dij - sj)/rgbmax)             A
