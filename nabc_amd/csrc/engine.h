@@ -43,6 +43,14 @@ int seterr(int code, const char *fmt, ...);
 
 /* ------------------------------------------------------------ device data */
 
+/* per queue position: the tile pair's oriented tiles, index, non-empty
+ * columns and derivative-store offset (one scalar load per visit) */
+struct QMeta {
+    int a, b, t, pad;
+    unsigned long long cols;
+    long long start;
+};
+
 struct DevPtrs {
     /* per slot */
     const int *perm;      /* slot -> global atom index, -1 = padding */
@@ -71,7 +79,7 @@ struct DevPtrs {
     int *nnz;             /* number of non-empty tile pairs */
     int *tnp;             /* listed pairs per tile pair */
     int *tnq;             /* the same in processing order */
-    long long *tstart;    /* derivative-store offset per queue position */
+    QMeta *qm;            /* per queue position, see QMeta */
     long long *store_need; /* entries the current list needs */
     int *store_ok;        /* 1 when the store holds the current list */
     void *dstore;         /* pass-3 derivative numerators (float2 / double2) */

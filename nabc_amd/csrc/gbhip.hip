@@ -318,6 +318,14 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
             const int q = base[TILE - m] + r;
             d.tord[q] = t;
             d.tnq[q] = d.tnp[t];
+            QMeta mq;
+            mq.a = d.tpo[t].x;
+            mq.b = d.tpo[t].y;
+            mq.t = t;
+            mq.pad = 0;
+            mq.cols = d.tcol[t];
+            mq.start = 0;
+            d.qm[q] = mq;
         }
         __syncthreads();
     }
@@ -346,12 +354,13 @@ __global__ __launch_bounds__(1024) void k_qscan(DevPtrs d)
     }
     long long run = s_part[t] - sum;
     for (int q = b; q < e; q++) {
-        d.tstart[q] = run;
+        d.qm[q].start = run;
         run += d.tnq[q];
     }
     if (t == 1023) {
         *d.store_need = s_part[1023];
-        *d.store_ok = d.dstore != nullptr && s_part[1023] <= d.store_cap;
+        /* strictly less: pass 3 may read one entry past a list's last */
+        *d.store_ok = d.dstore != nullptr && s_part[1023] < d.store_cap;
     }
 }
 
@@ -421,7 +430,7 @@ template <> struct Vec2<float> { typedef float2 T; };
  * of j by i; zero when the pair is beyond rgbmax + Fsmax), and pass 3 reads
  * them back instead of recomputing the descreening case split, logarithms
  * and reciprocals.  Tile pairs are stored in processing order from
- * tstart[q]; within a tile pair, step by step, the entries of the lanes
+ * its queue offset; within a tile pair, step by step, the entries of the lanes
  * holding a listed pair at that step, packed in lane order (position =
  * entries of earlier steps + listed lanes below this one at this step).
  */
@@ -457,11 +466,16 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             nval[k] = neckval<R>(d)[k];
         }
     }
+    QMeta nxt;
+    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
     for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
-        const int t = d.tord[q];
-        const unsigned long long cols = d.tcol[t];
+        /* this visit's metadata was loaded one visit ahead */
+        const QMeta mq = nxt;
+        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        const int t = mq.t;
+        const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
-        const int2 ab = d.tpo[t];
+        const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
@@ -471,7 +485,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
-        R2 *sp = store ? st + d.tstart[q] : nullptr;
+        R2 *sp = store ? st + mq.start : nullptr;
         __syncthreads();
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
@@ -605,12 +619,16 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     const bool screened = p.kappa != 0.0;
     double epol = 0.0, eel = 0.0, evdw = 0.0;
     int cur_mol = -1;
+    QMeta nxt;
+    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
     for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
-        const int t = d.tord[q];
-        const unsigned long long cols = d.tcol[t];
+        /* this visit's metadata was loaded one visit ahead */
+        const QMeta mq = nxt;
+        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        const int t = mq.t;
+        const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
-        if (m == 0) continue;
-        const int2 ab = d.tpo[t];
+        const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const int mol = d.slot_mol[i];
         if (mol != cur_mol) {
@@ -754,13 +772,18 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R2 *st = (const R2 *)d.dstore;
+    QMeta nxt;
+    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
     for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
-        const int t = d.tord[q];
-        const unsigned long long cols = d.tcol[t];
+        /* this visit's metadata was loaded one visit ahead */
+        const QMeta mq = nxt;
+        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        const int t = mq.t;
+        const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
-        const int2 ab = d.tpo[t];
+        const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
-        const R2 *sp = st + d.tstart[q];
+        const R2 *sp = st + mq.start;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
@@ -784,9 +807,13 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         for (int u = 0; u < CHAIN_DEPTH; u++) {
             act[u] = false;
             if (u < m) {
+                /* every lane loads (lanes without a pair read the step's
+                 * first entry, in bounds: the store holds more than the
+                 * list needs), so no branch separates a load from its use
+                 * and the loads stay in flight */
                 const bool a = (mr >> s_col[kp]) & 1ull;
                 const unsigned long long bal = __ballot(a);
-                if (a) win[u] = sp[lanes_below(bal)];
+                win[u] = sp[a ? lanes_below(bal) : 0];
                 act[u] = a;
                 sp += __popcll(bal);
                 kp = kp + 1 == m ? 0 : kp + 1;
@@ -804,7 +831,7 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                     if (s + CHAIN_DEPTH < m) {
                         const bool an = (mr >> s_col[kp]) & 1ull;
                         const unsigned long long bal = __ballot(an);
-                        if (an) win[u] = sp[lanes_below(bal)];
+                        win[u] = sp[an ? lanes_below(bal) : 0];
                         act[u] = an;
                         sp += __popcll(bal);
                         kp = kp + 1 == m ? 0 : kp + 1;
@@ -854,11 +881,16 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             nval[k] = neckval<R>(d)[k];
         }
     }
+    QMeta nxt;
+    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
     for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
-        const int t = d.tord[q];
-        const unsigned long long cols = d.tcol[t];
+        /* this visit's metadata was loaded one visit ahead */
+        const QMeta mq = nxt;
+        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        const int t = mq.t;
+        const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
-        const int2 ab = d.tpo[t];
+        const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
@@ -1182,7 +1214,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
         (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
         (rc = E.dalloc(&D.tnp, (size_t)cap)) || (rc = E.dalloc(&D.tnq, (size_t)cap)) ||
-        (rc = E.dalloc(&D.tstart, (size_t)cap)) ||
+        (rc = E.dalloc(&D.qm, (size_t)cap)) ||
         (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
         (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
@@ -1257,7 +1289,7 @@ int engine_store_reserve(Engine &E)
     HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     if (E.store_mode == 0 || E.store_frozen) return 0;
-    if (E.d.dstore && need <= E.d.store_cap) return 0;
+    if (E.d.dstore && need < E.d.store_cap) return 0;
     const size_t esz = E.precision == GBHIP_FP64 ? 2 * sizeof(double) : 2 * sizeof(float);
     const long long cap = need + need / 4 + (long long)TILE * TILE;
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
