@@ -194,6 +194,57 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def bench_accuracy(args):
+    """Per-term accuracy of the full mme() (bonded + 1-4 + GB) through the
+    host API, FP64 and mixed kernels, against the oracle's mme34 on the same
+    coordinates and list: C1 (600 atoms, igb=5, no cutoff), C2 (2,500,
+    igb=5, no cutoff), C3 (25,000, igb=8, 25 A, 0.15 M salt)."""
+    import numpy as np
+    import nabc_amd as na
+    import oracle
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    names = {0: "total", 1: "enb", 2: "eel", 3: "bond", 4: "angle", 5: "dihedral", 7: "enb14",
+             8: "eel14", 10: "egb"}
+    cfgs = {"c1": (600, 5, 999.0, 999.0, 0.0), "c2": (2500, 5, 999.0, 999.0, 0.0),
+            "c3": (25000, 8, 25.0, 25.0, 0.127315)}
+    out = {"metric": "mme() per-term relative error vs the oracle (FP64 mme34)", "unit": "relative",
+           "configs": {}}
+    for cfg, (natom, gb, cut, rgbmax, kappa) in cfgs.items():
+        parm, arr, x = na.build_system(natom, SEED)
+        x = np.ascontiguousarray(x)
+        orc = oracle.Oracle(parm, natom, gb=gb, cut=cut, rgbmax=rgbmax, kappa=kappa)
+        r_ene, r_frms, r_g = orc.mme(x)
+        orc.close()
+        res = {}
+        for prec in ("fp64", "mixed"):
+            na.lib.mm_reset_options()
+            assert na.lib.mm_options(("gb=%d, cut=%g, rgbmax=%g, kappa=%g, diel=C, gbprec=%s, ntpr=100000000" % (
+                gb, cut, rgbmax, kappa, prec)).encode()) == 0
+            assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+            f = np.zeros(3 * natom)
+            it = na.C.c_int(-2)
+            na.lib.mme(x.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+            ene = np.zeros(13)
+            frms = na.C.c_double()
+            na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(frms))
+            # relative to the term, with a 1 kcal/mol floor (the generated
+            # bonds sit at Req, so the bond energy is ~1e-26)
+            terms = {names[k]: abs(ene[k] - r_ene[k]) / max(abs(r_ene[k]), 1.0) for k in names}
+            res[prec] = {"terms": terms,
+                         "grad_rel_rms": float(np.sqrt(np.mean((f - r_g) ** 2) / np.mean(r_g ** 2))),
+                         "grad_max_component_abs": float(np.max(np.abs(f - r_g))),
+                         "grad_max_component_rel_to_max": float(np.max(np.abs(f - r_g)) / np.max(np.abs(r_g))),
+                         "frms_rel": abs(frms.value - r_frms) / r_frms}
+            it = na.C.c_int(-3)
+            na.lib.mme(x.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+        out["configs"][cfg] = {"natom": natom, "gb": gb, "cut": cut, "kappa": kappa,
+                               "oracle_terms": {names[k]: float(r_ene[k]) for k in names}, **res}
+    worst = max(v for c in out["configs"].values() for p in ("fp64", "mixed") for v in c[p]["terms"].values())
+    out["value"] = worst
+    out["worst_mixed_grad_rel_rms"] = max(c["mixed"]["grad_rel_rms"] for c in out["configs"].values())
+    print(json.dumps(out))
+
+
 _MME_FN = None
 
 
@@ -204,6 +255,8 @@ def bench_md(args):
     import nabc_amd as na
     global _MME_FN
     _MME_FN = na.MMEFUNC(lambda xp, fp, itp: na.lib.mme(xp, fp, itp))
+    if args.md_steps <= 0:
+        args.md_steps = {"c1": 1000, "c2": 10000, "c3": 5000}[args.md]
     natom, opts = {"c1": (600, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
                    "c2": (2500, "gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, nsnb=10"),
                    "c3": (25000, "gb=8, cut=25.0, rgbmax=25.0, kappa=0.127315, nsnb=20")}[args.md]
@@ -317,16 +370,18 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve"],
+    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
-                         "md: dynamics (see --md); serve: request loop")
+                         "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
+                         "errors against the oracle at C1/C2/C3")
     ap.add_argument("--jobs", type=int, default=200)
     ap.add_argument("--rate", type=float, default=10.0)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--hbm-gb", type=float, default=64.0)
     ap.add_argument("--nmol", type=int, default=1024)
     ap.add_argument("--md", default="c3", choices=["c1", "c2", "c3"])
-    ap.add_argument("--md-steps", type=int, default=5000)
+    ap.add_argument("--md-steps", type=int, default=0,
+                    help="MD steps (default per config: c1 1000, c2 10000, c3 5000)")
     ap.add_argument("--md-min", type=int, default=500, help="CG iterations before dynamics")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -339,6 +394,8 @@ def main():
         return bench_md(args)
     if args.config == "serve":
         return bench_serve(args)
+    if args.config == "accuracy":
+        return bench_accuracy(args)
 
     import numpy as np
     import nabc_amd as na
@@ -381,9 +438,23 @@ def main():
     roofline["eval_achieved"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12, 3)
     roofline["eval_frac"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12 / PEAK["mixed"], 4)
 
-    cpu = None
+    cpu, vs_oracle = None, None
+    gmax = float(np.max(np.abs(fp64["grad"])))
+    acc["grad_max_component_abs"] = float(np.max(np.abs(mixed["grad"] - fp64["grad"])))
+    acc["grad_max_component_rel_to_max"] = acc["grad_max_component_abs"] / gmax
     if not args.no_cpu:
-        cpu, _ = cpu_baseline(parm, arr["natom"], x)
+        cpu, ref = cpu_baseline(parm, arr["natom"], x)
+        # both GPU variants against the oracle's FP64 egb() on the same list
+        vs_oracle = {}
+        r_epol, r_evdw, r_eel, r_g = ref[0], ref[1], ref[2], ref[3]
+        for name, res in (("fp64", fp64), ("mixed", mixed)):
+            e = res["energies"]
+            vs_oracle[name] = {
+                "epol": abs(e["epol"] - r_epol) / abs(r_epol), "evdw": abs(e["evdw"] - r_evdw) / abs(r_evdw),
+                "eelec": abs(e["eelec"] - r_eel) / abs(r_eel),
+                "grad_rel_rms": float(np.sqrt(np.mean((res["grad"] - r_g) ** 2) / np.mean(r_g ** 2))),
+                "grad_max_component_abs": float(np.max(np.abs(res["grad"] - r_g))),
+                "grad_max_component_rel_to_max": float(np.max(np.abs(res["grad"] - r_g)) / np.max(np.abs(r_g)))}
 
     out = {
         "metric": "GB energy+grad evals/s, mixed vs FP64, at 25k atoms",
@@ -413,7 +484,9 @@ def main():
         "mixed_pass_ms": {k: round(v, 4) for k, v in mixed["pass_ms"].items()},
         "nblist_ms": round(mixed["nblist_ms"], 4),
         "mixed_error_vs_fp64": {"epol": acc["epol"], "evdw": acc["evdw"], "eelec": acc["eelec"],
-                                "grad_rel_rms": gerr},
+                                "grad_rel_rms": gerr, "grad_max_component_abs": acc["grad_max_component_abs"],
+                                "grad_max_component_rel_to_max": acc["grad_max_component_rel_to_max"]},
+        "error_vs_oracle": vs_oracle,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
