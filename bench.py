@@ -338,7 +338,7 @@ def bench_serve(args):
     opts = ("gb=5, cut=20.0, rgbmax=20.0, nsnb=10, dt=0.001, gamma_ln=5.0, temp0=300, tempi=300, "
             "ntpr=100000000")
     res = na.serve(jobs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), precision=na.GBHIP_MIXED,
-                   check_fp64=True, mm_opts=opts, cg_rmsgrad=1e-4, cg_dfpred=10.0)
+                   check_fp64=True, mm_opts=opts, cg_rmsgrad=1e-4, cg_dfpred=10.0, progress=20)
     ok = [r for r in res if r["status"] >= -3]
     lat = np.array([r["latency_s"] for r in ok])
     span = max(r["end_s"] for r in ok) - min(j["arrival_s"] for j in jobs)

@@ -37,6 +37,7 @@ typedef struct {
     int check_fp64;          /* re-evaluate each final state in FP64 and in the job precision */
     double cg_rmsgrad;       /* conjgrad rmsgrad argument */
     double cg_dfpred;        /* conjgrad dfpred argument */
+    int progress;            /* > 0: a line on stderr every `progress` completed jobs */
 } gbserve_config;
 
 typedef struct {

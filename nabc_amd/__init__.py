@@ -99,7 +99,8 @@ class ServeJob(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
-                ("check_fp64", C.c_int), ("cg_rmsgrad", C.c_double), ("cg_dfpred", C.c_double)]
+                ("check_fp64", C.c_int), ("cg_rmsgrad", C.c_double), ("cg_dfpred", C.c_double),
+                ("progress", C.c_int)]
 
 
 class ServeResult(C.Structure):
@@ -186,7 +187,7 @@ _sig("gbserve_run", C.c_int, [C.POINTER(ServeJob), C.c_int, C.POINTER(ServeConfi
 
 
 def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=False,
-          mm_opts="gb=5, cut=20.0, rgbmax=20.0", cg_rmsgrad=1e-4, cg_dfpred=10.0):
+          mm_opts="gb=5, cut=20.0, rgbmax=20.0", cg_rmsgrad=1e-4, cg_dfpred=10.0, progress=0):
     """Run jobs through the serving loop.
 
     jobs: list of dicts with prm (PARMSTRUCT_T pointer), x, kind
@@ -200,7 +201,7 @@ def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=
         keep.append(x)
         arr[k] = ServeJob(C.cast(j["prm"], C.c_void_p), _p(x), j["kind"], j["nsteps"],
                           j.get("seed", k + 1), j["arrival_s"])
-    cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred)
+    cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred, progress)
     res = (ServeResult * n)()
     rc = lib.gbserve_run(arr, n, C.byref(cfg), mm_opts.encode() if mm_opts else None, res)
     if rc != 0:

@@ -179,14 +179,19 @@ void worker(Server *S, int w)
             k = S->assigned[w];
         }
         run_job(*S, w, k);
+        int done;
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->assigned[w] = -1;
             S->mem_in_use -= S->res[k].footprint;
             S->active--;
-            S->done++;
+            done = ++S->done;
         }
         S->cv.notify_all();
+        if (S->cfg.progress > 0 && done % S->cfg.progress == 0) {
+            std::fprintf(stderr, "gbserve: %d of %d jobs done at %.2f s\n", done, S->njobs, now_s(S->t0));
+            std::fflush(stderr);
+        }
     }
 }
 
