@@ -277,6 +277,7 @@ def bench_md(args):
     it = na.C.c_int(-3)
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
     out = {"metric": "MD steps/s (%s, %d atoms)" % (args.md, natom), "unit": "steps/s",
+           "step_launch": "stream" if args.md_no_graph else "hipGraph",
            "start": {"cg_iterations_max": args.md_min, "cg_return": cg_ret, "energy": fmin.value,
                      "seconds": round(t_min, 2)}}
     for name, prec in (("fp64", "fp64"), ("mixed", "mixed")):
@@ -295,10 +296,11 @@ def bench_md(args):
                          boltz2=9.93595e-4, nsnb=int(opts.split("nsnb=")[1]), zerov=0, seed=SEED)
         assert na.lib.gbhip_md_init(ctx, masses.ctypes.data_as(na.dp), x0.ctypes.data_as(na.dp), None,
                                     na.C.byref(o), args.md_steps) == 0
-        assert na.lib.gbhip_md_run(ctx, 20, 1) == 0
+        use_graph = 0 if args.md_no_graph else 1
+        assert na.lib.gbhip_md_run(ctx, 20, use_graph) == 0
         assert na.lib.gbhip_md_state(ctx, None, None, None) == 0
         t0 = time.perf_counter()
-        assert na.lib.gbhip_md_run(ctx, args.md_steps, 1) == 0
+        assert na.lib.gbhip_md_run(ctx, args.md_steps, use_graph) == 0
         assert na.lib.gbhip_md_state(ctx, None, None, None) == 0
         t1 = time.perf_counter()
         rows = np.zeros((args.md_steps, 16))
@@ -383,6 +385,8 @@ def main():
     ap.add_argument("--md-steps", type=int, default=0,
                     help="MD steps (default per config: c1 1000, c2 10000, c3 5000)")
     ap.add_argument("--md-min", type=int, default=500, help="CG iterations before dynamics")
+    ap.add_argument("--md-no-graph", action="store_true",
+                    help="launch each step's kernels on the stream instead of replaying the captured graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

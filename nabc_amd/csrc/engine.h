@@ -183,6 +183,7 @@ struct BondedDev {
     const double *q = nullptr;
     const int *frozen = nullptr;  /* original order, always allocated */
     double *ene = nullptr;        /* 13 doubles, mme34 layout */
+    double *gpart = nullptr;      /* per-block partial sums of |grad|^2 */
 };
 
 /* device-resident dynamics state (mm_device.hip) */

@@ -2,25 +2,30 @@
  * mm_device.hip -- device-resident mme(): bonded and 1-4 terms next to the
  * GB engine, so one energy + gradient never leaves the GPU.
  *
- *   k_bond    ebond, eff.c:82-168    E = Rk (r - Req)^2
- *   k_angle   eangl, eff.c:168-316   E = Tk (theta - Teq)^2, sin clamp 1e-3
- *   k_dihed   ephi,  eff.c:316-627   Fourier terms; a series (Pn < 0) is
+ *   d_bond    ebond, eff.c:82-168    E = Rk (r - Req)^2
+ *   d_angle   eangl, eff.c:168-316   E = Tk (theta - Teq)^2, sin clamp 1e-3
+ *   d_dihed   ephi,  eff.c:316-627   Fourier terms; a series (Pn < 0) is
  *             expanded on the host into one entry per term
- *   k_nb14    nbond(N14=1), eff.c:896-1331 (diel C or R), scaled by the
+ *   d_nb14    nbond(N14=1), eff.c:896-1331 (diel C or R), scaled by the
  *             per-pair 1/SCEE and 1/SCNB from the prmtop
- * One thread per term, FP64, gradient added with FP64 atomics into the AoS
+ * all four in one launch (k_bonded, a block range per term type).  One
+ * thread per term, FP64, gradient added with FP64 atomics into the AoS
  * gradient in original atom order; energies reduced per block.
  * The assembled energy vector follows mme34's ene[] (eff.c:2617-2874);
- * frozen atoms get a zero gradient (eff.c:2826-2834).
+ * frozen atoms get a zero gradient (eff.c:2826-2834); the rms gradient is
+ * reduced over a grid of blocks, then summed in a fixed order (k_gnorm,
+ * k_finish).
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "engine.h"
 
 #define MM_THREADS 256
+#define GNORM_PARTS 1024 /* blocks of the gradient-norm reduction */
 
 /* sum v over the block and add it to *dst (one atomic per block) */
 __device__ __forceinline__ void block_add(double v, double *dst)
@@ -44,9 +49,9 @@ __global__ __launch_bounds__(MM_THREADS) void k_fill(double *p, long long n, dou
     if (k < n) p[k] = v;
 }
 
-__global__ __launch_bounds__(MM_THREADS) void k_bond(BondedDev B, const double *__restrict__ x, double *g)
+__device__ __forceinline__ void d_bond(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
-    const int t = blockIdx.x * MM_THREADS + threadIdx.x;
+    const int t = blk * MM_THREADS + threadIdx.x;
     double e = 0.0;
     if (t < B.nb) {
         const int i = 3 * B.b_i[t], j = 3 * B.b_j[t];
@@ -66,9 +71,9 @@ __global__ __launch_bounds__(MM_THREADS) void k_bond(BondedDev B, const double *
     block_add(e, &B.ene[3]);
 }
 
-__global__ __launch_bounds__(MM_THREADS) void k_angle(BondedDev B, const double *__restrict__ x, double *g)
+__device__ __forceinline__ void d_angle(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
-    const int t = blockIdx.x * MM_THREADS + threadIdx.x;
+    const int t = blk * MM_THREADS + threadIdx.x;
     double e = 0.0;
     if (t < B.na) {
         const int i1 = 3 * B.a_i[t], i2 = 3 * B.a_j[t], i3 = 3 * B.a_k[t];
@@ -112,9 +117,9 @@ __global__ __launch_bounds__(MM_THREADS) void k_angle(BondedDev B, const double 
  * cos(phi) = uv / sqrt(uu vv) with uu = |a|^2|b|^2 - (a.b)^2,
  * vv = |b|^2|c|^2 - (b.c)^2, uv = (a.b)(b.c) - (a.c)|b|^2.
  */
-__global__ __launch_bounds__(MM_THREADS) void k_dihed(BondedDev B, const double *__restrict__ x, double *g)
+__device__ __forceinline__ void d_dihed(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
-    const int t = blockIdx.x * MM_THREADS + threadIdx.x;
+    const int t = blk * MM_THREADS + threadIdx.x;
     double e = 0.0;
     if (t < B.nd) {
         const int i1 = 3 * B.d_i[t], i2 = 3 * B.d_j[t], i3 = 3 * B.d_k[t], i4 = 3 * B.d_l[t];
@@ -193,9 +198,9 @@ __global__ __launch_bounds__(MM_THREADS) void k_dihed(BondedDev B, const double 
     block_add(e, &B.ene[5]);
 }
 
-__global__ __launch_bounds__(MM_THREADS) void k_nb14(BondedDev B, const double *__restrict__ x, double *g)
+__device__ __forceinline__ void d_nb14(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
-    const int t = blockIdx.x * MM_THREADS + threadIdx.x;
+    const int t = blk * MM_THREADS + threadIdx.x;
     double ev = 0.0, ee = 0.0;
     if (t < B.n14) {
         const int i = 3 * B.p_i[t], j = 3 * B.p_j[t];
@@ -236,13 +241,37 @@ __global__ __launch_bounds__(MM_THREADS) void k_nb14(BondedDev B, const double *
     block_add(ee, &B.ene[8]);
 }
 
-/* zero frozen gradients, copy the GB terms in, total and rms gradient */
-__global__ __launch_bounds__(MM_THREADS) void k_finish(BondedDev B, const double *gbene, double *g, int natom,
-                                                       double *ene_out)
+/* all bonded and 1-4 terms in one launch: consecutive block ranges per term
+ * type (block-uniform branches, so block_add's barriers are reached by the
+ * whole block) */
+__global__ __launch_bounds__(MM_THREADS) void k_bonded(BondedDev B, const double *__restrict__ x, double *g,
+                                                       int4 nblk)
+{
+    int b = blockIdx.x;
+    if (b < nblk.x) { d_bond(B, x, g, b); return; }
+    b -= nblk.x;
+    if (b < nblk.y) { d_angle(B, x, g, b); return; }
+    b -= nblk.y;
+    if (b < nblk.z) { d_dihed(B, x, g, b); return; }
+    b -= nblk.z;
+    d_nb14(B, x, g, b);
+}
+
+/* gradient and the 14 energy slots cleared in one launch */
+__global__ __launch_bounds__(MM_THREADS) void k_zero_mme(double *g, long long n3, double *ene)
+{
+    const long long k = (long long)blockIdx.x * MM_THREADS + threadIdx.x;
+    if (k < n3) g[k] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x < 14) ene[threadIdx.x] = 0.0;
+}
+
+/* zero frozen gradients; per-block sums of squared gradient components,
+ * written per block so the total is summed in a fixed order */
+__global__ __launch_bounds__(MM_THREADS) void k_gnorm(BondedDev B, double *g, int natom, double *part)
 {
     __shared__ double s_red[MM_THREADS / 64];
     double s = 0.0;
-    for (int a = threadIdx.x; a < natom; a += MM_THREADS) {
+    for (int a = blockIdx.x * MM_THREADS + threadIdx.x; a < natom; a += gridDim.x * MM_THREADS) {
         if (B.frozen[a]) {
             g[3 * a] = 0.0; g[3 * a + 1] = 0.0; g[3 * a + 2] = 0.0;
         } else {
@@ -253,8 +282,21 @@ __global__ __launch_bounds__(MM_THREADS) void k_finish(BondedDev B, const double
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double tot = 0.0, e[13];
-        for (int k = 0; k < MM_THREADS / 64; k++) tot += s_red[k];
+        double t = 0.0;
+        for (int k = 0; k < MM_THREADS / 64; k++) t += s_red[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+/* copy the GB terms in, total energy and rms gradient (one wavefront) */
+__global__ __launch_bounds__(64) void k_finish(BondedDev B, const double *gbene, const double *part, int nparts,
+                                              int natom, double *ene_out)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += 64) s += part[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (threadIdx.x == 0) {
+        double e[13];
         for (int k = 0; k < 13; k++) e[k] = B.ene[k];
         e[10] = gbene[0];
         e[1] = gbene[1];
@@ -262,7 +304,7 @@ __global__ __launch_bounds__(MM_THREADS) void k_finish(BondedDev B, const double
         e[0] = 0.0;
         for (int k = 1; k <= 12; k++) e[0] += e[k];
         for (int k = 0; k < 13; k++) ene_out[k] = e[k];
-        ene_out[13] = sqrt(tot / (3.0 * natom));
+        ene_out[13] = sqrt(s / (3.0 * natom));
     }
 }
 
@@ -312,6 +354,7 @@ extern "C" int gbhip_set_bonded(gbhip_ctx *ctx, const gbhip_bonded *b)
         (rc = up_d(&B.p_b, b->p14_b, B.n14)) || (rc = up_d(&B.q, b->charge, n)))
         return rc;
     if (!B.ene && (rc = E.dalloc(&B.ene, 14))) return rc;
+    if (!B.gpart && (rc = E.dalloc(&B.gpart, GNORM_PARTS))) return rc;
     if (!B.frozen) {
         std::vector<int> fz(n, 0);
         for (int s = 0; s < E.nslots; s++) {
@@ -333,17 +376,18 @@ int mme_device(gbhip_ctx *ctx, const double *x_dev, double *d_grad, double *d_en
     BondedDev &B = ctx->B;
     if (!B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
     const long long n3 = 3LL * E.natoms;
-    hipLaunchKernelGGL(k_fill, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, d_grad, n3, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(MM_THREADS), 0, E.stream, B.ene, 14LL, 0.0);
-    if (B.nb) hipLaunchKernelGGL(k_bond, dim3(grid_of(B.nb)), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad);
-    if (B.na) hipLaunchKernelGGL(k_angle, dim3(grid_of(B.na)), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad);
-    if (B.nd) hipLaunchKernelGGL(k_dihed, dim3(grid_of(B.nd)), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad);
-    if (B.n14) hipLaunchKernelGGL(k_nb14, dim3(grid_of(B.n14)), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad);
+    hipLaunchKernelGGL(k_zero_mme, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, d_grad, n3, B.ene);
+    const int4 nblk = make_int4(grid_of(B.nb), grid_of(B.na), grid_of(B.nd), grid_of(B.n14));
+    const int nbonded = nblk.x + nblk.y + nblk.z + nblk.w;
+    if (nbonded)
+        hipLaunchKernelGGL(k_bonded, dim3(nbonded), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad, nblk);
     LAUNCHCHK();
     int rc = engine_eval(E, x_dev, d_grad);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(MM_THREADS), 0, E.stream, B, (const double *)E.d.ene, d_grad,
-                       E.natoms, d_ene14);
+    const int nparts = std::min(GNORM_PARTS, std::max(1, grid_of(E.natoms)));
+    hipLaunchKernelGGL(k_gnorm, dim3(nparts), dim3(MM_THREADS), 0, E.stream, B, d_grad, E.natoms, B.gpart);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, E.stream, B, (const double *)E.d.ene,
+                       (const double *)B.gpart, nparts, E.natoms, d_ene14);
     LAUNCHCHK();
     return 0;
 }
