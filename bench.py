@@ -149,6 +149,7 @@ def bench_batch(args):
         b = na.Batch([(a, x) for _, a, x in systems], gb=5, precision=prec)
         npairs, natoms, ntp = b.stats()
         launch = b.autotune(3) if not args.no_tune else None
+        store_used = b.store_stats()[2]
         for _ in range(args.warmup):
             b.eval_device()
         ene, grads = b.eval()
@@ -162,7 +163,8 @@ def bench_batch(args):
         dt = (t1 - t0) / (args.steps + 1)
         res[name] = dict(batch_ms=1e3 * dt, molecules_per_s=nmol / dt, pairs_per_s=npairs / dt,
                          pass_ms={k: round(v / n, 4) for k, v in zip(("born", "radii", "pairs", "chain"), ms)},
-                         ene=ene, grads=grads, npairs=npairs, natoms=natoms, tilepairs=ntp, launch=launch)
+                         ene=ene, grads=grads, npairs=npairs, natoms=natoms, tilepairs=ntp, launch=launch,
+                         store=store_used)
         b.close()
     # per-instance check against the oracle
     worst = dict(fp64_e=0.0, fp64_g=0.0, mixed_e=0.0, mixed_g=0.0)
@@ -189,6 +191,7 @@ def bench_batch(args):
            "npairs": res["mixed"]["npairs"], "natoms": res["mixed"]["natoms"],
            "tile_pairs": res["mixed"]["tilepairs"], "nmol": nmol,
            "launch_blocks": {k: res[k]["launch"] for k in res},
+           "derivative_store_in_use": {k: res[k]["store"] for k in res},
            "worst_instance_error_vs_oracle": worst, "generate_s": round(t_gen, 1),
            "oracle_check_s": round(t_check, 1)}
     print(json.dumps(out))

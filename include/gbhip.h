@@ -161,9 +161,11 @@ int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
  * Launch shape of the three persistent pair kernels (pass 1, pass 2,
  * pass 3): the number of one-wavefront workgroups each is launched with.
  * gbhip_set_launch takes blocks3[3] (0 keeps a value); gbhip_autotune times
- * reps evaluations on the current list for each candidate size (4 to 40
- * workgroups per CU) and keeps the fastest per pass, writing the choice to
- * chosen3 (may be NULL).  MD graphs captured earlier keep the shapes they
+ * reps evaluations on the current list with and without the derivative
+ * store (gbhip_set_derivative_store; the faster is kept, so it may switch
+ * the store off), then for each candidate size (4 to 40 workgroups per CU),
+ * and keeps the fastest per pass, writing the choice to chosen3 (may be
+ * NULL).  MD graphs captured earlier keep the shapes they
  * were captured with.  No counterpart in the reference (egb() has no launch
  * configuration); nblist must have run.
  */
@@ -314,8 +316,9 @@ int gbhip_batch_timing_stop(gbhip_batch *b, double *ms4, int *nevals);
 /* Launch-shape autotuning of a batch (as gbhip_autotune). */
 int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3);
 
-/* Derivative-store mode of a batch (as gbhip_set_derivative_store). */
+/* Derivative-store mode and statistics of a batch (as for a context). */
 int gbhip_batch_set_derivative_store(gbhip_batch *b, int mode);
+int gbhip_batch_store_stats(gbhip_batch *b, long long *need, long long *cap, int *in_use);
 
 /* Total unordered pairs and atoms of the batch. */
 int gbhip_batch_stats(gbhip_batch *b, long long *npairs, long long *natoms,

@@ -153,6 +153,7 @@ _sig("gbhip_set_launch", C.c_int, [vp, ip])
 _sig("gbhip_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
+_sig("gbhip_batch_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_get_launch", C.c_int, [vp, ip])
 _sig("gbhip_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
@@ -558,6 +559,11 @@ class Batch:
 
     def set_derivative_store(self, mode):
         _check(lib.gbhip_batch_set_derivative_store(self.h, mode))
+
+    def store_stats(self):
+        need, cap, ok = C.c_longlong(), C.c_longlong(), C.c_int()
+        _check(lib.gbhip_batch_store_stats(self.h, C.byref(need), C.byref(cap), C.byref(ok)))
+        return need.value, cap.value, bool(ok.value)
 
     def autotune(self, reps=3):
         b = np.zeros(3, np.int32)

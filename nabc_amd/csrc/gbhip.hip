@@ -1526,34 +1526,64 @@ static int set_launch(Engine &E, const int *blocks3)
 }
 
 /*
- * Time whole evaluations on the current list and coordinates for each
- * candidate persistent-grid size (workgroups per CU: 4 .. 40) and keep, per
- * pass, the fastest.  The candidates differ in how many wavefronts share a
- * CU (latency hiding against LDS and register limits) and in the tail of
- * the tile-pair loop.  Leaves the engine evaluated at the current
+ * Time whole evaluations on the current list and coordinates: first with
+ * and without the pass-3 derivative store (keeping the faster), then for
+ * each candidate persistent-grid size (workgroups per CU: 4 .. 40), keeping
+ * per pass the fastest.  The grid candidates differ in how many wavefronts
+ * share a CU (latency hiding against LDS and register limits) and in the
+ * tail of the tile-pair loop.  Leaves the engine evaluated at the current
  * coordinates; replaces any timing session in progress.
  */
+/* summed pass times (ms) of reps evaluations at the current settings, after
+ * one untimed evaluation */
+static int time_evals(Engine &E, int reps, double *ms3)
+{
+    int rc;
+    if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
+    if ((rc = timing_start(E, reps))) return rc;
+    for (int r = 0; r < reps; r++)
+        if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
+    double ms4[4];
+    int n = 0;
+    if ((rc = timing_stop(E, ms4, &n))) return rc;
+    ms3[0] = ms4[0];
+    ms3[1] = ms4[2];
+    ms3[2] = ms4[3];
+    return 0;
+}
+
 static int autotune(Engine &E, int reps, int *chosen3)
 {
     static const int per_cu[] = {4, 8, 12, 16, 20, 24, 32, 40};
     const int ncand = (int)(sizeof(per_cu) / sizeof(per_cu[0]));
     if (!E.have_list) return seterr(GBHIP_ESTATE, "autotune needs a pair list (call nblist first)");
     if (reps < 1 || reps > 1000) return seterr(GBHIP_EINVAL, "reps out of range");
+    int rc;
+    double ms[3];
+    /* 1. derivative store or recomputation in pass 3, at the current launch
+     * shapes: the store saves pass 3 the descreening arithmetic but costs
+     * 8 (mixed) or 16 (FP64) bytes per pair written and read, which does not
+     * pay where that arithmetic is cheap (FP32, no rgbmax shell) */
+    if (E.store_mode == 1 && !E.store_frozen && E.d.dstore) {
+        int ok = 0;
+        HIPCHK(hipMemcpyAsync(&ok, E.d.store_ok, sizeof(ok), hipMemcpyDeviceToHost, E.stream));
+        HIPCHK(hipStreamSynchronize(E.stream));
+        if (ok) {
+            if ((rc = time_evals(E, reps, ms))) return rc;
+            const double t_on = ms[0] + ms[1] + ms[2];
+            if ((rc = set_store_mode(E, 0))) return rc;
+            if ((rc = time_evals(E, reps, ms))) return rc;
+            const double t_off = ms[0] + ms[1] + ms[2];
+            if (t_on <= t_off && (rc = set_store_mode(E, 1))) return rc;
+        }
+    }
+    /* 2. persistent-grid size per pass */
     double best[3] = {1e300, 1e300, 1e300};
     int pick[3] = {E.pass_blocks[0], E.pass_blocks[1], E.pass_blocks[2]};
-    int rc;
     for (int c = 0; c < ncand; c++) {
         const int nb = std::max(1, E.ncu * per_cu[c]);
         for (int k = 0; k < 3; k++) E.pass_blocks[k] = nb;
-        /* one untimed evaluation at the new shape, then reps timed */
-        if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
-        if ((rc = timing_start(E, reps))) return rc;
-        for (int r = 0; r < reps; r++)
-            if ((rc = engine_eval(E, E.x_dev, nullptr))) return rc;
-        double ms4[4];
-        int n = 0;
-        if ((rc = timing_stop(E, ms4, &n))) return rc;
-        const double ms[3] = {ms4[0], ms4[2], ms4[3]};
+        if ((rc = time_evals(E, reps, ms))) return rc;
         for (int k = 0; k < 3; k++)
             if (ms[k] < best[k]) {
                 best[k] = ms[k];
@@ -1578,10 +1608,8 @@ extern "C" int gbhip_batch_set_derivative_store(gbhip_batch *b, int mode)
     return set_store_mode(b->E, mode);
 }
 
-extern "C" int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap, int *in_use)
+static int store_stats(Engine &E, long long *need, long long *cap, int *in_use)
 {
-    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
-    Engine &E = ctx->E;
     long long nd = 0;
     int ok = 0;
     HIPCHK(hipMemcpyAsync(&nd, E.d.store_need, sizeof(nd), hipMemcpyDeviceToHost, E.stream));
@@ -1591,6 +1619,18 @@ extern "C" int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap
     if (cap) *cap = E.d.store_cap;
     if (in_use) *in_use = ok;
     return 0;
+}
+
+extern "C" int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap, int *in_use)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return store_stats(ctx->E, need, cap, in_use);
+}
+
+extern "C" int gbhip_batch_store_stats(gbhip_batch *b, long long *need, long long *cap, int *in_use)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return store_stats(b->E, need, cap, in_use);
 }
 
 extern "C" int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3)
