@@ -158,16 +158,17 @@ int gbhip_timing_start(gbhip_ctx *ctx, int max_evals);
 int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
 
 /*
- * Launch shape of the three persistent pair kernels (pass 1, pass 2,
- * pass 3): the number of one-wavefront workgroups each is launched with.
+ * Launch shape of the three pair kernels (pass 1, pass 2, pass 3): the
+ * number of one-wavefront workgroups each is launched with (1 to 65536);
+ * each workgroup takes every G-th tile pair of the cost-sorted list.
  * gbhip_set_launch takes blocks3[3] (0 keeps a value); gbhip_autotune times
  * reps evaluations on the current list with and without the derivative
  * store (gbhip_set_derivative_store; the faster is kept, so it may switch
- * the store off), then for each candidate size (4 to 40 workgroups per CU),
- * and keeps the fastest per pass, writing the choice to chosen3 (may be
- * NULL).  MD graphs captured earlier keep the shapes they
- * were captured with.  No counterpart in the reference (egb() has no launch
- * configuration); nblist must have run.
+ * the store off), then for each candidate size (4 to 128 workgroups per CU,
+ * and one per non-empty tile pair), and keeps the fastest per pass, writing
+ * the choice to chosen3 (may be NULL).  MD graphs captured earlier keep the
+ * shapes they were captured with.  No counterpart in the reference (egb()
+ * has no launch configuration); nblist must have run.
  */
 int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3);
 

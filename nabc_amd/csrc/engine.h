@@ -14,6 +14,10 @@
 #include "gbhip.h"
 
 #define TILE 64
+/* energy partial sums per (molecule, term): workgroups add into slot
+ * blockIdx mod ENE_SLOTS, so that tens of thousands of workgroups of one
+ * molecule do not queue on one address */
+#define ENE_SLOTS 64
 
 /*
  * Held while a stream is captured into a graph and while contexts allocate
@@ -62,6 +66,7 @@ struct DevPtrs {
     double *xs, *ys, *zs;
     double *sumi, *reff, *psi, *dfac, *sumdeijda, *gx, *gy, *gz;
     double *ene;          /* 3 per molecule: epol, evdw, eelec */
+    double *eslot;        /* ENE_SLOTS partial sums per ene entry (k_enesum adds them up) */
     /* tiles and tile pairs */
     int ntiles;
     const int *tile_mol;

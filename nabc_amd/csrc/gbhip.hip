@@ -387,7 +387,6 @@ __device__ __forceinline__ int snake_item(int r)
 __global__ __launch_bounds__(256) void k_gather(DevPtrs d, GBPar p, const double *__restrict__ x)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s < 3 * p.nmol) d.ene[s] = 0.0;
     if (s >= p.nslots) return;
     const int a = d.perm[s];
     if (a >= 0) {
@@ -595,7 +594,8 @@ __global__ __launch_bounds__(64) void k_radii(DevPtrs d, GBPar p)
     }
     /* a wave is one tile, so one molecule */
     for (int off = 32; off > 0; off >>= 1) eself += __shfl_xor(eself, off);
-    if (threadIdx.x == 0 && eself != 0.0) unsafeAtomicAdd(&d.ene[3 * d.slot_mol[s]], eself);
+    if (threadIdx.x == 0 && eself != 0.0)
+        unsafeAtomicAdd(&d.eslot[3 * (size_t)d.slot_mol[s] * ENE_SLOTS + (blockIdx.x & (ENE_SLOTS - 1))], eself);
 }
 
 /* one pair's pass-2 contributions, zeros when the lane has no listed pair */
@@ -657,6 +657,34 @@ __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, doub
  * to both atoms, and for non-excluded pairs Coulomb and 12-6 LJ; the direct
  * pair force de*xij goes to i (+) and j (-).
  */
+/* one workgroup's pass-2 energies of molecule mol into its slots */
+__device__ __forceinline__ void ene_flush(const DevPtrs &d, int mol, double epol, double evdw, double eel)
+{
+    double *e = d.eslot + 3 * (size_t)mol * ENE_SLOTS + (blockIdx.x & (ENE_SLOTS - 1));
+    unsafeAtomicAdd(e, epol);
+    unsafeAtomicAdd(e + ENE_SLOTS, evdw);
+    unsafeAtomicAdd(e + 2 * ENE_SLOTS, eel);
+}
+
+/*
+ * ene[s] = sum of its ENE_SLOTS partials in slot order, which are cleared for
+ * the next evaluation (they start at zero from allocation).  One thread per
+ * entry, after pass 2.
+ */
+__global__ __launch_bounds__(64) void k_enesum(DevPtrs d, int n)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= n) return;
+    double *e = d.eslot + (size_t)s * ENE_SLOTS;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < ENE_SLOTS; k++) {
+        acc += e[k];
+        e[k] = 0.0;
+    }
+    d.ene[s] = acc;
+}
+
 template <typename R>
 __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
 {
@@ -691,11 +719,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     eel += __shfl_xor(eel, off);
                     evdw += __shfl_xor(evdw, off);
                 }
-                if (l == 0) {
-                    unsafeAtomicAdd(&d.ene[3 * cur_mol], epol);
-                    unsafeAtomicAdd(&d.ene[3 * cur_mol + 1], evdw);
-                    unsafeAtomicAdd(&d.ene[3 * cur_mol + 2], eel);
-                }
+                if (l == 0) ene_flush(d, cur_mol, epol, evdw, eel);
                 epol = eel = evdw = 0.0;
             }
             cur_mol = mol;
@@ -831,11 +855,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             eel += __shfl_xor(eel, off);
             evdw += __shfl_xor(evdw, off);
         }
-        if (l == 0) {
-            unsafeAtomicAdd(&d.ene[3 * cur_mol], epol);
-            unsafeAtomicAdd(&d.ene[3 * cur_mol + 1], evdw);
-            unsafeAtomicAdd(&d.ene[3 * cur_mol + 2], eel);
-        }
+        if (l == 0) ene_flush(d, cur_mol, epol, evdw, eel);
     }
 }
 
@@ -1298,6 +1318,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.sumi, NS)) || (rc = E.dalloc(&D.reff, NS)) || (rc = E.dalloc(&D.psi, NS)) ||
         (rc = E.dalloc(&D.dfac, NS)) || (rc = E.dalloc(&D.sumdeijda, NS)) || (rc = E.dalloc(&D.gx, NS)) ||
         (rc = E.dalloc(&D.gy, NS)) || (rc = E.dalloc(&D.gz, NS)) || (rc = E.dalloc(&D.ene, 3 * (size_t)nmol)) ||
+        (rc = E.dalloc(&D.eslot, 3 * (size_t)nmol * ENE_SLOTS)) ||
         (rc = E.dalloc(&D.cand, (size_t)cap)) || (rc = E.dalloc(&D.rowcnt, NT)) ||
         (rc = E.dalloc(&D.rowstart, NT)) || (rc = E.dalloc(&D.ntp, 1)) || (rc = E.dalloc(&D.tp, (size_t)cap)) ||
         (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
@@ -1313,6 +1334,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&E.ene14_dev, 14)))
         return rc;
     HIPCHK(hipMemsetAsync(D.ntp, 0, sizeof(int), E.stream));
+    HIPCHK(hipMemsetAsync(D.eslot, 0, 3 * (size_t)nmol * ENE_SLOTS * sizeof(double), E.stream));
+    HIPCHK(hipMemsetAsync(D.ene, 0, 3 * (size_t)nmol * sizeof(double), E.stream));
     HIPCHK(hipMemsetAsync(D.nnz, 0, sizeof(int), E.stream));
     HIPCHK(hipMemsetAsync(D.store_ok, 0, sizeof(int), E.stream));
     HIPCHK(hipMemsetAsync(D.store_need, 0, sizeof(long long), E.stream));
@@ -1455,6 +1478,7 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
         hipLaunchKernelGGL(k_pairs<double>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
     else
         hipLaunchKernelGGL(k_pairs<float>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
+    hipLaunchKernelGGL(k_enesum, dim3((3 * E.nmol + 63) / 64), dim3(64), 0, E.stream, E.d, 3 * E.nmol);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
     if (E.precision == GBHIP_FP64) {
         hipLaunchKernelGGL(k_chain_st<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
@@ -1515,10 +1539,12 @@ static int timing_stop(Engine &E, double *ms4, int *nevals)
 
 /* ------------------------------------------------------------ launch shapes */
 
+#define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
+
 static int set_launch(Engine &E, const int *blocks3)
 {
     for (int k = 0; k < 3; k++) {
-        if (blocks3[k] < 0 || blocks3[k] > 64 * 1024) return seterr(GBHIP_EINVAL, "launch size out of range");
+        if (blocks3[k] < 0 || blocks3[k] > MAX_LAUNCH) return seterr(GBHIP_EINVAL, "launch size out of range");
     }
     for (int k = 0; k < 3; k++)
         if (blocks3[k] > 0) E.pass_blocks[k] = blocks3[k];
@@ -1528,10 +1554,14 @@ static int set_launch(Engine &E, const int *blocks3)
 /*
  * Time whole evaluations on the current list and coordinates: first with
  * and without the pass-3 derivative store (keeping the faster), then for
- * each candidate persistent-grid size (workgroups per CU: 4 .. 40), keeping
- * per pass the fastest.  The grid candidates differ in how many wavefronts
- * share a CU (latency hiding against LDS and register limits) and in the
- * tail of the tile-pair loop.  Leaves the engine evaluated at the current
+ * each candidate grid size (workgroups per CU: 4 .. 128, and one workgroup
+ * per non-empty tile pair), keeping per pass the fastest.  Up to the number
+ * the CUs hold at once the grid is persistent and the candidates differ in
+ * latency hiding; beyond it the hardware hands later workgroups to CUs as
+ * earlier ones finish, which balances the uneven tile-pair costs better than
+ * the static deal (measured at 25k atoms: pass 1 0.32 ms at 40 per CU, 0.26
+ * ms at one tile pair per workgroup) but repeats per-workgroup set-up (pass 2
+ * prefers a persistent grid).  Leaves the engine evaluated at the current
  * coordinates; replaces any timing session in progress.
  */
 /* summed pass times (ms) of reps evaluations at the current settings, after
@@ -1554,12 +1584,20 @@ static int time_evals(Engine &E, int reps, double *ms3)
 
 static int autotune(Engine &E, int reps, int *chosen3)
 {
-    static const int per_cu[] = {4, 8, 12, 16, 20, 24, 32, 40};
-    const int ncand = (int)(sizeof(per_cu) / sizeof(per_cu[0]));
+    static const int per_cu[] = {4, 8, 16, 24, 32, 40, 64, 96, 128};
+    const int nper = (int)(sizeof(per_cu) / sizeof(per_cu[0]));
     if (!E.have_list) return seterr(GBHIP_ESTATE, "autotune needs a pair list (call nblist first)");
     if (reps < 1 || reps > 1000) return seterr(GBHIP_EINVAL, "reps out of range");
     int rc;
     double ms[3];
+    int nnz = 0;
+    HIPCHK(hipMemcpyAsync(&nnz, E.d.nnz, sizeof(nnz), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    std::vector<int> cand;
+    for (int c = 0; c < nper; c++) cand.push_back(std::min(MAX_LAUNCH, std::max(1, E.ncu * per_cu[c])));
+    cand.push_back(std::min(MAX_LAUNCH, std::max(1, nnz)));
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     /* 1. derivative store or recomputation in pass 3, at the current launch
      * shapes: the store saves pass 3 the descreening arithmetic but costs
      * 8 (mixed) or 16 (FP64) bytes per pair written and read, which does not
@@ -1580,8 +1618,7 @@ static int autotune(Engine &E, int reps, int *chosen3)
     /* 2. persistent-grid size per pass */
     double best[3] = {1e300, 1e300, 1e300};
     int pick[3] = {E.pass_blocks[0], E.pass_blocks[1], E.pass_blocks[2]};
-    for (int c = 0; c < ncand; c++) {
-        const int nb = std::max(1, E.ncu * per_cu[c]);
+    for (const int nb : cand) {
         for (int k = 0; k < 3; k++) E.pass_blocks[k] = nb;
         if ((rc = time_evals(E, reps, ms))) return rc;
         for (int k = 0; k < 3; k++)
