@@ -62,13 +62,17 @@ def sha256(path):
     return h.hexdigest()
 
 
-def run_gpu(na, arr, x, precision, steps, warmup, tune=True):
+def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None):
     ctx = na.GBContext(arr, x, gb=GB, cut=CUT, rgbmax=RGBMAX, kappa=KAPPA, precision=precision)
     ctx.nblist()
     npairs = ctx.npairs()
     ntiles, ntp = ctx.tile_stats()
     # launch-shape autotuning is set-up, outside the timed region
-    launch = ctx.autotune(5) if tune else ctx.launch()
+    if fixed_launch:
+        ctx.set_launch(*fixed_launch)
+        launch = ctx.launch()
+    else:
+        launch = ctx.autotune(5) if tune else ctx.launch()
     store_need, store_cap, store_used = ctx.store_stats()
     for k in range(warmup):
         if k % NSNB == 0:
@@ -382,6 +386,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
+    ap.add_argument("--launch", default="", metavar="B1,B2,B3",
+                    help="fixed workgroup counts of passes 1-3 (both precisions), no autotuning; "
+                         "reproduces a tuned run's shapes under a profiler")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy"],
@@ -433,8 +440,11 @@ def main():
         arr = na.parm_arrays(parm)
         x, _ = na.read_rst7(rst, arr["natom"])
 
-    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune)
-    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune)
+    fixed = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
+    if fixed is not None and len(fixed) != 3:
+        raise SystemExit("--launch takes three workgroup counts")
+    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune, fixed)
+    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune, fixed)
 
     # accuracy of the mixed kernels against the FP64 kernels (same list)
     acc = {k: abs(mixed["energies"][k] - fp64["energies"][k]) / abs(fp64["energies"][k])

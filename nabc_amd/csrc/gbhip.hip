@@ -49,6 +49,7 @@
 #include "neck_tables.h"
 
 #define PAIR_BLOCKS_MAX 4096
+#define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
 #define CHAIN_DEPTH 8  /* derivative-store loads in flight per lane in pass 3 */
 
 /* ------------------------------------------------------------------ errors */
@@ -267,6 +268,40 @@ __global__ __launch_bounds__(256) void k_excl(DevPtrs d)
 }
 
 /*
+ * Membership and exclusion words in step order.  Every evaluation pass has
+ * lane l of a tile pair with m non-empty columns visit compacted column
+ * (l mod m + s) mod m at step s; bit s of the lane's words is rewritten to
+ * hold that column's bit, so the passes test (w >> s) & 1 without looking
+ * the column up.  Bits of empty columns hold no listed pair and are dropped.
+ */
+__global__ __launch_bounds__(64) void k_steporder(DevPtrs d)
+{
+    __shared__ int s_pos[TILE];
+    const int l = threadIdx.x;
+    const int ntp = *d.ntp;
+    for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        if (m == 0) continue;
+        __syncthreads();
+        if ((cols >> l) & 1ull) s_pos[__popcll(cols & ((1ull << l) - 1ull))] = l;
+        __syncthreads();
+        const size_t o = (size_t)t * TILE + l;
+        const unsigned long long w = d.mask[o], e = d.excl[o];
+        unsigned long long sw = 0ull, se = 0ull;
+        int k = l % m;
+        for (int st = 0; st < m; st++) {
+            const int c = s_pos[k];
+            sw |= ((w >> c) & 1ull) << st;
+            se |= ((e >> c) & 1ull) << st;
+            k = k + 1 == m ? 0 : k + 1;
+        }
+        d.mask[o] = sw;
+        d.excl[o] = se;
+    }
+}
+
+/*
  * Processing order of the evaluation passes: the non-empty tile pairs sorted
  * by their number of non-empty columns (the step count of a visit), largest
  * first, by a counting sort over 64 bins.  Workgroups take tile pairs from a
@@ -451,7 +486,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
     typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE];
-    __shared__ int s_nk[TILE], s_col[TILE];
+    __shared__ int s_nk[TILE];
     __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
@@ -492,7 +527,6 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             s_ri[pos] = rj; s_ri1i[pos] = R(1) / rj;
             s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
             s_nk[pos] = d.neck[j];
-            s_col[pos] = l;
             s_acc[pos] = 0.0;
         }
         __syncthreads();
@@ -500,7 +534,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         int k = l % m;
         for (int s = 0; s < m; s++) {
             R aij = R(0), aji = R(0);
-            const bool listed = (mr >> s_col[k]) & 1ull;
+            const bool listed = (mr >> s) & 1ull;
             if (listed) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
                 const double r2d = dx * dx + dy * dy + dz * dz;
@@ -692,7 +726,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE], s_sd[TILE];
     __shared__ double s_q[TILE];
     __shared__ R s_r[TILE];
-    __shared__ int s_lj[TILE], s_col[TILE];
+    __shared__ int s_lj[TILE];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R epsinv = (R)(1.0 / p.epsext);
@@ -737,7 +771,6 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
             s_q[pos] = d.q[j]; s_r[pos] = (R)d.reff[j];
             s_lj[pos] = d.ljcol[j];
-            s_col[pos] = l;
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0; s_sd[pos] = 0.0;
         }
         __syncthreads();
@@ -758,7 +791,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
                     const int kq = kk2[u];
-                    const int b = s_col[kq];
+                    const int b = (s + u) & 63;
                     const bool act = (u == 0 || two) && ((mr >> b) & 1ull);
                     const bool coul = act && !((e >> b) & 1ull);
                     pair_terms<R>(P[u], act, coul, xi, yi, zi, qi, ri, s_x[kq], s_y[kq], s_z[kq], s_q[kq], s_r[kq],
@@ -780,7 +813,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             }
         } else {
             for (int s = 0; s < m; s++) {
-                const int b = s_col[k];
+                const int b = s;
                 if ((mr >> b) & 1ull) {
                     const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
                     const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
@@ -877,7 +910,6 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
     __shared__ R s_w[TILE];
-    __shared__ int s_col[TILE];
     if (!*d.store_ok) return;
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
@@ -903,14 +935,12 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
             s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
-            s_col[pos] = l;
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
         }
         __syncthreads();
         /* the packed entries of the next CHAIN_DEPTH steps are in flight
-         * while a step is processed: kp/sp walk ahead of k */
+         * while a step is processed: sp walks ahead of k */
         const int k0 = l % m;
-        int kp = k0;
         R2 win[CHAIN_DEPTH];
         bool act[CHAIN_DEPTH];
 #pragma unroll
@@ -921,12 +951,11 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                  * first entry, in bounds: the store holds more than the
                  * list needs), so no branch separates a load from its use
                  * and the loads stay in flight */
-                const bool a = (mr >> s_col[kp]) & 1ull;
+                const bool a = (mr >> u) & 1ull;
                 const unsigned long long bal = __ballot(a);
                 win[u] = sp[a ? lanes_below(bal) : 0];
                 act[u] = a;
                 sp += __popcll(bal);
-                kp = kp + 1 == m ? 0 : kp + 1;
             }
         }
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
@@ -939,12 +968,11 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                     const R2 a = win[u];
                     const bool on = act[u];
                     if (s + CHAIN_DEPTH < m) {
-                        const bool an = (mr >> s_col[kp]) & 1ull;
+                        const bool an = (mr >> (s + CHAIN_DEPTH)) & 1ull;
                         const unsigned long long bal = __ballot(an);
                         win[u] = sp[an ? lanes_below(bal) : 0];
                         act[u] = an;
                         sp += __popcll(bal);
-                        kp = kp + 1 == m ? 0 : kp + 1;
                     }
                     if (on) {
                         const double f = (double)(a.x * wi + a.y * s_w[k]);
@@ -978,7 +1006,7 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE], s_w[TILE];
-    __shared__ int s_nk[TILE], s_col[TILE];
+    __shared__ int s_nk[TILE];
     __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     if (*d.store_ok) return;
     const int l = threadIdx.x;
@@ -1019,14 +1047,13 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
             s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
             s_nk[pos] = d.neck[j];
-            s_col[pos] = l;
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
         }
         __syncthreads();
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
         int k = l % m;
         for (int s = 0; s < m; s++) {
-            if ((mr >> s_col[k]) & 1ull) {
+            if ((mr >> s) & 1ull) {
                 const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
                 const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
                 if (r2d <= rp2) {
@@ -1083,15 +1110,25 @@ __global__ __launch_bounds__(256) void k_scatter(DevPtrs d, GBPar p, double *__r
 __global__ __launch_bounds__(64) void k_listpairs(DevPtrs d, int *pi, int *pj, unsigned long long *cnt,
                                                   long long cap)
 {
+    __shared__ int s_pos[TILE];
     const int l = threadIdx.x;
     const int ntp = *d.ntp;
     for (int t = blockIdx.x; t < ntp; t += gridDim.x) {
         const int2 ij = d.tpo[t];
+        const unsigned long long cols = d.tcol[t];
+        const int m = __popcll(cols);
+        if (m == 0) continue;
+        __syncthreads();
+        if ((cols >> l) & 1ull) s_pos[__popcll(cols & ((1ull << l) - 1ull))] = l;
+        __syncthreads();
+        /* the word is in step order: bit s is column (l mod m + s) mod m */
         unsigned long long w = d.mask[(size_t)t * TILE + l];
         const int a = d.perm[ij.x * TILE + l];
+        const int k0 = l % m;
         while (w) {
-            const int b = __ffsll((unsigned long long)w) - 1;
+            const int st = __ffsll((unsigned long long)w) - 1;
             w &= w - 1ull;
+            const int b = s_pos[k0 + st < m ? k0 + st : k0 + st - m];
             const unsigned long long k = atomicAdd(cnt, 1ull);
             if ((long long)k < cap) {
                 pi[k] = a;
@@ -1355,7 +1392,13 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     E.pair_blocks = (int)std::min<long long>(std::max<long long>(1, cap), (long long)ncu * 16);
     E.pair_blocks = std::min(E.pair_blocks, PAIR_BLOCKS_MAX * 4);
     E.ncu = ncu;
-    for (int k = 0; k < 3; k++) E.pass_blocks[k] = E.pair_blocks;
+    /* evaluation grids until autotuned (measured at 25k atoms, 25 A): pass 1
+     * fastest with one workgroup per tile pair, pass 2 near 64 per CU, pass 3
+     * near 40 per CU */
+    const long long cap1 = std::max<long long>(1, cap);
+    E.pass_blocks[0] = (int)std::min<long long>(cap1, MAX_LAUNCH);
+    E.pass_blocks[1] = (int)std::min<long long>(cap1, (long long)ncu * 64);
+    E.pass_blocks[2] = (int)std::min<long long>(cap1, (long long)ncu * 40);
     return 0;
 }
 
@@ -1384,6 +1427,7 @@ int engine_nblist(Engine &E)
     hipLaunchKernelGGL(k_qscan, dim3(1), dim3(1024), 0, E.stream, E.d);
     if (E.natoms > 0)
         hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_steporder, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d);
     LAUNCHCHK();
     E.have_list = true;
     return 0;
@@ -1538,8 +1582,6 @@ static int timing_stop(Engine &E, double *ms4, int *nevals)
 }
 
 /* ------------------------------------------------------------ launch shapes */
-
-#define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
 
 static int set_launch(Engine &E, const int *blocks3)
 {
