@@ -463,6 +463,11 @@ def main():
     # whole-evaluation figure: W = 240 P flops per evaluation at the measured rate
     roofline["eval_achieved"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12, 3)
     roofline["eval_frac"] = round(W_PER_PAIR * P * mixed["evals_per_s"] / 1e12 / PEAK["mixed"], 4)
+    # the ceiling this build can reach: independent FMA chains compiled like
+    # the pair kernels (no packed FP32), measured on this device
+    probe32, probe64 = na.valu_peak(na.GBHIP_MIXED), na.valu_peak(na.GBHIP_FP64)
+    roofline["valu_probe_tflops"] = {"fp32": round(probe32, 2), "fp64": round(probe64, 2)}
+    roofline["frac_of_probe"] = round(achieved / probe32, 4)
 
     cpu, vs_oracle = None, None
     gmax = float(np.max(np.abs(fp64["grad"])))

@@ -158,6 +158,16 @@ int gbhip_timing_start(gbhip_ctx *ctx, int max_evals);
 int gbhip_timing_stop(gbhip_ctx *ctx, double *ms4, int *nevals);
 
 /*
+ * Measured VALU ceiling of the current device for this build's code: a
+ * grid of independent FMA chains (2 flops each) in FP64 (GBHIP_FP64) or
+ * FP32 (GBHIP_MIXED), compiled like the pair kernels (no packed FP32),
+ * best of three timed launches on the default stream; writes TFLOP/s.
+ * Synchronises the device.  No counterpart in the reference (a
+ * measurement helper for the roofline the benchmark reports).
+ */
+int gbhip_valu_peak(int precision, double *tflops);
+
+/*
  * Launch shape of the three pair kernels (pass 1, pass 2, pass 3): the
  * number of one-wavefront workgroups each is launched with (1 to 65536);
  * each workgroup takes every G-th tile pair of the cost-sorted list.

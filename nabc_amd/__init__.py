@@ -150,6 +150,7 @@ _sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_set_launch", C.c_int, [vp, ip])
+_sig("gbhip_valu_peak", C.c_int, [C.c_int, dp])
 _sig("gbhip_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
@@ -230,6 +231,13 @@ def _p(a, t=dp):
 
 def device_count():
     return lib.gbhip_device_count()
+
+
+def valu_peak(precision):
+    """Measured VALU TFLOP/s of independent FMA chains (GBHIP_FP64 or GBHIP_MIXED = FP32)."""
+    t = C.c_double()
+    _check(lib.gbhip_valu_peak(int(precision), C.byref(t)))
+    return t.value
 
 
 # ---------------------------------------------------------------- prmtop / rst7

@@ -1739,6 +1739,65 @@ extern "C" int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3)
 
 /* ---------------------------------------------------------- single context */
 
+/*
+ * VALU peak probe: eight independent FMA chains per lane (no dependency
+ * stalls at full occupancy), 2 flops per FMA; the result is stored so the
+ * chains are not removed.  Built with the same flags as the pair kernels
+ * (packed FP32 off), so it measures the ceiling those kernels can reach.
+ */
+template <typename R>
+__global__ __launch_bounds__(256) void k_valu_probe(R *out, int iters, R a, R b)
+{
+    R x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = (R)(threadIdx.x + u) * R(1e-3);
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = fma(x[u], a, b);
+    }
+    R sum = R(0);
+#pragma unroll
+    for (int u = 0; u < 8; u++) sum += x[u];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+extern "C" int gbhip_valu_peak(int precision, double *tflops)
+{
+    if (!tflops) return seterr(GBHIP_EINVAL, "NULL argument");
+    if (precision != GBHIP_FP64 && precision != GBHIP_MIXED) return seterr(GBHIP_EINVAL, "bad precision");
+    int dev = 0, ncu = 256;
+    HIPCHK(hipGetDevice(&dev));
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int blocks = ncu * 8, iters = 8192;
+    void *out = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    HIPCHK(hipMalloc(&out, (size_t)blocks * 256 * sizeof(double)));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; rep++) {
+        HIPCHK(hipEventRecord(e0, nullptr));
+        if (precision == GBHIP_FP64)
+            hipLaunchKernelGGL(k_valu_probe<double>, dim3(blocks), dim3(256), 0, nullptr, (double *)out, iters,
+                               0.9999999, 1e-7);
+        else
+            hipLaunchKernelGGL(k_valu_probe<float>, dim3(blocks), dim3(256), 0, nullptr, (float *)out, iters,
+                               0.9999f, 1e-4f);
+        HIPCHK(hipEventRecord(e1, nullptr));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0 && ms < best) best = ms; /* the first launch warms up */
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipFree(out));
+    const double flops = 2.0 * 8.0 * iters * (double)blocks * 256.0;
+    *tflops = flops / (best * 1e-3) / 1e12;
+    return 0;
+}
+
 extern "C" int gbhip_timing_start(gbhip_ctx *ctx, int max_evals)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");

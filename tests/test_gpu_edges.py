@@ -88,3 +88,15 @@ def test_batches_with_tiny_and_single_molecules(gpu, precision):
             assert rel_rms(grads[k], r[3]) <= gtol
             o.close()
         b.close()
+
+
+def test_valu_probe_in_range(gpu):
+    """The VALU ceiling probe gives a plausible MI355X-class rate (the
+    published vector peaks are 78.6 TFLOP/s FP64 and 157.3 FP32 with packed
+    FP32, which this build does not emit) and rejects a bad precision."""
+    na = gpu
+    for prec in (na.GBHIP_FP64, na.GBHIP_MIXED):
+        t = na.valu_peak(prec)
+        assert 5.0 < t < 170.0, (prec, t)
+    with pytest.raises(RuntimeError):
+        na.valu_peak(7)
