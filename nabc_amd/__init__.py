@@ -479,7 +479,7 @@ class GBContext:
         return need.value, cap.value, bool(ok.value)
 
     def autotune(self, reps=5):
-        """Pick the fastest persistent-grid size per pass; returns the choice."""
+        """Pick the fastest grid size per pass (and derivative store on/off); returns the grid sizes."""
         b = np.zeros(3, np.int32)
         _check(lib.gbhip_autotune(self.h, reps, _p(b, ip)))
         return tuple(int(v) for v in b)

@@ -110,7 +110,7 @@ struct Engine {
     int nmol = 0, natoms = 0, nslots = 0, ntiles = 0;
     long long tp_cap = 0;
     int precision = GBHIP_FP64;
-    int pair_blocks = 0;       /* default workgroups of the persistent pair kernels */
+    int pair_blocks = 0;       /* workgroups of the list-build kernels (16 per CU, at most the tile pairs) */
     int pass_blocks[3] = {0, 0, 0}; /* per pass: born, pairs, chain */
     int ncu = 0;
     GBPar par{};

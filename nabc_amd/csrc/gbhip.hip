@@ -31,8 +31,9 @@
  * list build so that the side with fewer non-empty atoms is the one
  * stepped.  j-side sums are accumulated in LDS (ds_add_f64) and flushed
  * with one FP64 atomic per atom per tile pair.  Tile pairs are processed in
- * descending step count, dealt to the persistent grid in boustrophedon
- * rounds.
+ * descending step count, dealt to the grid in boustrophedon rounds (the
+ * autotuner picks per pass between a persistent grid and up to one
+ * workgroup per tile pair, which the hardware balances as workgroups end).
  */
 #include <hip/hip_runtime.h>
 
@@ -405,11 +406,12 @@ __global__ __launch_bounds__(64) void k_setflag(int *flag, int v)
 }
 
 /*
- * Static assignment of the sorted tile pairs to the persistent grid, in
+ * Static assignment of the sorted tile pairs to a grid of G workgroups, in
  * boustrophedon rounds: round r gives workgroup b the item r*G + b for even
  * r and r*G + G-1-b for odd r.  Over a descending cost order this pairs each
  * workgroup's expensive items with cheap ones (longest-processing-time
- * balance) without a shared counter.
+ * balance) without a shared counter; with G at least the number of items
+ * every workgroup has one item or none.
  */
 __device__ __forceinline__ int snake_item(int r)
 {
@@ -1657,7 +1659,7 @@ static int autotune(Engine &E, int reps, int *chosen3)
             if (t_on <= t_off && (rc = set_store_mode(E, 1))) return rc;
         }
     }
-    /* 2. persistent-grid size per pass */
+    /* 2. grid size per pass */
     double best[3] = {1e300, 1e300, 1e300};
     int pick[3] = {E.pass_blocks[0], E.pass_blocks[1], E.pass_blocks[2]};
     for (const int nb : cand) {
