@@ -61,7 +61,7 @@ struct DevPtrs {
     const int *slot_mol;  /* slot -> molecule */
     const double *q, *rb, *ri0, *sj, *ga, *gbb, *gg;
     const double *rgbpsmax2;  /* per molecule: (rgbmax + Fsmax)^2 */
-    const float *q_f, *ri0_f, *sj_f, *rb_f;
+    const float *ri0_f, *sj_f, *rb_f;
     const int *neck, *frz, *ljrow, *ljcol;
     double *xs, *ys, *zs;
     double *sumi, *reff, *psi, *dfac, *sumdeijda, *gx, *gy, *gz;

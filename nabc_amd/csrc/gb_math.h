@@ -3,13 +3,15 @@
  * the mixed-precision kernels (template parameter R is double or float).
  *
  * Formulas follow egb() in src/sff/eff.c:
- *   descreen()   pass 1 term, eff.c:1663-1697 (Schaefer-Froemmel appendix,
- *                Taylor series for d > 4 s, smooth rgbmax switch)
- *   neck_term()  gb 7/8 neck correction, eff.c:1699-1708
- *   ddescreen()  pass 3 derivative numerator, eff.c:2289-2334
- *   dneck_term() derivative of the neck term, eff.c:2336-2360
- * The value returned by descreen() is the amount egb() subtracts from sumi;
- * ddescreen() returns datmp before the multiplication by sumdeijda.
+ *   descreen_d() pass 1 term, eff.c:1663-1697 (Schaefer-Froemmel appendix,
+ *                Taylor series for d > 4 s, smooth rgbmax switch), and its
+ *                pass 3 derivative numerator, eff.c:2289-2334
+ *   neck_d()     gb 7/8 neck correction, eff.c:1699-1708, and its
+ *                derivative, eff.c:2336-2360
+ *   ddescreen(), dneck_term(): the derivative parts alone (pass 3 when it
+ *                recomputes them)
+ * The value returned by descreen_d() is the amount egb() subtracts from
+ * sumi; the derivative is datmp before the multiplication by sumdeijda.
  */
 #ifndef NABC_AMD_GB_MATH_H
 #define NABC_AMD_GB_MATH_H
@@ -196,14 +198,6 @@ __device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i
 }
 
 template <typename R>
-__device__ __forceinline__ R descreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
-                                      R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
-{
-    R dv;
-    return descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip, dv);
-}
-
-template <typename R>
 __device__ __forceinline__ R ddescreen(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
                                        R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip)
 {
@@ -227,13 +221,6 @@ __device__ __forceinline__ R neck_d(R dij, R dij1i, R pos, R val, R neckscale, R
     const R nv = neckscale * val * rt;
     dn = (R(2) * mdist + R(1.8) * mdist2 * mdist2 * mdist) * nv * rt * dij1i;
     return nv;
-}
-
-template <typename R>
-__device__ __forceinline__ R neck_term(R dij, R pos, R val, R neckscale)
-{
-    R dn;
-    return neck_d<R>(dij, R(1), pos, val, neckscale, dn);
 }
 
 template <typename R>

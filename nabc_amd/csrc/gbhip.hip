@@ -437,9 +437,6 @@ __global__ __launch_bounds__(256) void k_gather(DevPtrs d, GBPar p, const double
     d.gx[s] = 0.0; d.gy[s] = 0.0; d.gz[s] = 0.0;
 }
 
-template <typename R> __device__ __forceinline__ R ldq(const DevPtrs &d, int s);
-template <> __device__ __forceinline__ double ldq<double>(const DevPtrs &d, int s) { return d.q[s]; }
-template <> __device__ __forceinline__ float ldq<float>(const DevPtrs &d, int s) { return d.q_f[s]; }
 template <typename R> __device__ __forceinline__ R ldri0(const DevPtrs &d, int s);
 template <> __device__ __forceinline__ double ldri0<double>(const DevPtrs &d, int s) { return d.ri0[s]; }
 template <> __device__ __forceinline__ float ldri0<float>(const DevPtrs &d, int s) { return d.ri0_f[s]; }
@@ -1343,11 +1340,11 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.upload(&D.rgbpsmax2, rgbpsmax2)))
         return rc;
     {
-        std::vector<float> qf(q.begin(), q.end()), rf(ri0.begin(), ri0.end()), sf(sj.begin(), sj.end()),
+        std::vector<float> rf(ri0.begin(), ri0.end()), sf(sj.begin(), sj.end()),
             bf(rb.begin(), rb.end());
         std::vector<double> np(NECK_POS, NECK_POS + NECK_N * NECK_N), nv(NECK_VAL, NECK_VAL + NECK_N * NECK_N);
         std::vector<float> npf(np.begin(), np.end()), nvf(nv.begin(), nv.end());
-        if ((rc = E.upload(&D.q_f, qf)) || (rc = E.upload(&D.ri0_f, rf)) || (rc = E.upload(&D.sj_f, sf)) ||
+        if ((rc = E.upload(&D.ri0_f, rf)) || (rc = E.upload(&D.sj_f, sf)) ||
             (rc = E.upload(&D.rb_f, bf)) ||
             (rc = E.upload(&D.neck_pos, np)) || (rc = E.upload(&D.neck_val, nv)) ||
             (rc = E.upload(&D.neck_pos_f, npf)) || (rc = E.upload(&D.neck_val_f, nvf)))
