@@ -65,6 +65,21 @@ size_t gbhip_footprint_bytes(int natom, int md_steps);
 int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_config *cfg,
                 const char *mm_opts, gbserve_result *res);
 
+/*
+ * The same admission and completion loop with the work of a job done by a
+ * caller's function instead of the GPU job: run(job, slot, user, &energy)
+ * is called on the slot's worker thread (slots threads at once) and returns
+ * the job status (0 or a positive code for success, negative for failure).
+ * Admission by arrival time, free slot and gbhip_footprint_bytes against
+ * cfg->hbm_budget is unchanged; precision and check_fp64 are ignored and
+ * no HIP call is made.  For schedulers that run other work per job and for
+ * testing the loop itself without a device.  No counterpart in the
+ * reference.
+ */
+typedef int (*gbserve_runner)(const gbserve_job *job, int slot, void *user, double *energy);
+int gbserve_run_custom(const gbserve_job *jobs, int njobs, const gbserve_config *cfg, gbserve_runner run,
+                       void *user, gbserve_result *res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,6 +186,9 @@ _sig("mme_context", vp, [dp])
 _sig("gbhip_footprint_bytes", C.c_size_t, [C.c_int, C.c_int])
 _sig("gbserve_run", C.c_int, [C.POINTER(ServeJob), C.c_int, C.POINTER(ServeConfig), C.c_char_p,
                               C.POINTER(ServeResult)])
+SERVE_RUNNER = C.CFUNCTYPE(C.c_int, C.POINTER(ServeJob), C.c_int, C.c_void_p, C.POINTER(C.c_double))
+_sig("gbserve_run_custom", C.c_int, [C.POINTER(ServeJob), C.c_int, C.POINTER(ServeConfig), SERVE_RUNNER,
+                                     C.c_void_p, C.POINTER(ServeResult)])
 
 
 def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=False,
@@ -206,6 +209,39 @@ def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=
     cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred, progress)
     res = (ServeResult * n)()
     rc = lib.gbserve_run(arr, n, C.byref(cfg), mm_opts.encode() if mm_opts else None, res)
+    if rc != 0:
+        raise GBHipError(lib.mm_get_error().decode(errors="replace"))
+    return [{f[0]: getattr(r, f[0]) for f in ServeResult._fields_} for r in res]
+
+
+def serve_custom(jobs, run, slots=8, hbm_budget=64 << 30):
+    """The serving loop's admission and completion logic with run(k, slot)
+    doing job k's work on the slot's worker thread (returns (status,
+    energy)); no device work.  jobs as for serve(); results likewise."""
+    n = len(jobs)
+    arr = (ServeJob * n)()
+    keep = []
+    for k, j in enumerate(jobs):
+        x = np.ascontiguousarray(j.get("x", np.zeros(3)), dtype=np.float64)
+        keep.append(x)
+        arr[k] = ServeJob(C.cast(j["prm"], C.c_void_p), _p(x), j["kind"], j["nsteps"],
+                          j.get("seed", k + 1), j["arrival_s"])
+    base = C.addressof(arr)
+    size = C.sizeof(ServeJob)
+
+    def tramp(job, slot, user, energy):
+        k = (C.addressof(job.contents) - base) // size
+        try:
+            st, e = run(k, slot)
+        except Exception:  # a Python error becomes a failed job, not a crash
+            return -99
+        energy[0] = e
+        return st
+
+    cb = SERVE_RUNNER(tramp)
+    cfg = ServeConfig(slots, hbm_budget, GBHIP_MIXED, 0, 0.0, 0.0, 0)
+    res = (ServeResult * n)()
+    rc = lib.gbserve_run_custom(arr, n, C.byref(cfg), cb, None, res)
     if rc != 0:
         raise GBHipError(lib.mm_get_error().decode(errors="replace"))
     return [{f[0]: getattr(r, f[0]) for f in ServeResult._fields_} for r in res]

@@ -76,6 +76,8 @@ struct Server {
     std::vector<int> assigned;     /* per worker: job index or -1 */
     std::vector<hipStream_t> streams;
     std::vector<std::vector<double>> xfinal; /* per job, kept for the precision check */
+    gbserve_runner runner = nullptr; /* caller's job function (gbserve_run_custom) */
+    void *user = nullptr;
     int done = 0;
     bool stop = false;
     size_t mem_in_use = 0;
@@ -178,7 +180,17 @@ void worker(Server *S, int w)
             if (S->assigned[w] < 0) return;
             k = S->assigned[w];
         }
-        run_job(*S, w, k);
+        if (S->runner) {
+            const gbserve_job &jb = S->jobs[k];
+            gbserve_result &r = S->res[k];
+            double energy = 0.0;
+            r.status = S->runner(&jb, w, S->user, &energy);
+            r.energy = energy;
+            r.end_s = now_s(S->t0);
+            r.latency_s = r.end_s - jb.arrival_s;
+        } else {
+            run_job(*S, w, k);
+        }
         int done;
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -217,8 +229,8 @@ extern "C" size_t gbhip_footprint_bytes(int natom, int md_steps)
     return b;
 }
 
-extern "C" int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_config *cfg,
-                           const char *mm_opts, gbserve_result *res)
+static int serve_loop(const gbserve_job *jobs, int njobs, const gbserve_config *cfg, const char *mm_opts,
+                      gbserve_result *res, gbserve_runner runner, void *user)
 {
     if (!jobs || !res || !cfg || njobs < 0 || cfg->slots < 1) {
         nabc_seterr("gbserve_run: bad arguments");
@@ -229,7 +241,10 @@ extern "C" int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_con
     S.res = res;
     S.njobs = njobs;
     S.cfg = *cfg;
-    {
+    S.runner = runner;
+    S.user = user;
+    if (runner) S.cfg.check_fp64 = 0;
+    if (!runner) {
         /* options for the jobs: current mm_options state plus the given string */
         nabc_opts_t saved = nabc_opts;
         if (mm_opts && mm_options((char *)mm_opts) != 0) {
@@ -252,7 +267,7 @@ extern "C" int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_con
     S.assigned.assign(cfg->slots, -1);
     S.xfinal.assign(njobs, std::vector<double>());
     S.streams.assign(cfg->slots, nullptr);
-    for (int w = 0; w < cfg->slots; w++) {
+    for (int w = 0; w < cfg->slots && !runner; w++) {
         if (hipStreamCreateWithFlags(&S.streams[w], hipStreamNonBlocking) != hipSuccess) {
             nabc_seterr("gbserve_run: stream creation failed");
             for (int v = 0; v < w; v++) (void)hipStreamDestroy(S.streams[v]);
@@ -304,9 +319,26 @@ extern "C" int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_con
     }
     S.cv.notify_all();
     for (auto &t : threads) t.join();
-    if (cfg->check_fp64)
+    if (S.cfg.check_fp64)
         for (int k = 0; k < njobs; k++) check_job(S, k);
-    for (hipStream_t s : S.streams) (void)hipStreamDestroy(s);
+    for (hipStream_t s : S.streams)
+        if (s) (void)hipStreamDestroy(s);
     (void)rejected;
     return 0;
+}
+
+extern "C" int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_config *cfg,
+                           const char *mm_opts, gbserve_result *res)
+{
+    return serve_loop(jobs, njobs, cfg, mm_opts, res, nullptr, nullptr);
+}
+
+extern "C" int gbserve_run_custom(const gbserve_job *jobs, int njobs, const gbserve_config *cfg,
+                                  gbserve_runner run, void *user, gbserve_result *res)
+{
+    if (!run) {
+        nabc_seterr("gbserve_run_custom: NULL runner");
+        return -1;
+    }
+    return serve_loop(jobs, njobs, cfg, nullptr, res, run, user);
 }
