@@ -96,6 +96,7 @@ struct DevPtrs {
     int natoms;
     /* LJ dense tables, concatenated per molecule */
     const double2 *ljab;  /* (A, B) = (cn1, cn2) per type pair */
+    const int2 *mol_lj;   /* per molecule: offset of its ntypes x ntypes block in ljab, ntypes */
     const double *neck_pos, *neck_val;
     const float *neck_pos_f, *neck_val_f;
 };

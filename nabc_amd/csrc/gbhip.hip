@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -51,6 +52,7 @@
 
 #define PAIR_BLOCKS_MAX 4096
 #define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
+#define LJ_LDS 256 /* LJ table entries pass 2 keeps in LDS */
 #define CHAIN_DEPTH 8  /* derivative-store loads in flight per lane in pass 3 */
 
 /* ------------------------------------------------------------------ errors */
@@ -724,6 +726,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE], s_sd[TILE];
     __shared__ double s_q[TILE];
+    /* the molecule's LJ (A, B) table when it has at most LJ_LDS entries
+     * (16 types): a per-pair global load leaves the dependency chain */
+    __shared__ double2 s_tab[LJ_LDS];
     __shared__ R s_r[TILE];
     __shared__ int s_lj[TILE];
     const int l = threadIdx.x;
@@ -732,7 +737,8 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     const R kks = (R)(-GB_KSCALE * p.kappa);
     const bool screened = p.kappa != 0.0;
     double epol = 0.0, eel = 0.0, evdw = 0.0;
-    int cur_mol = -1;
+    int cur_mol = -1, lj0 = 0;
+    bool tab_lds = false;
     QMeta nxt;
     if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
     for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
@@ -756,11 +762,19 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                 epol = eel = evdw = 0.0;
             }
             cur_mol = mol;
+            const int2 mlj = d.mol_lj[mol];
+            lj0 = mlj.x;
+            tab_lds = mlj.y * mlj.y <= LJ_LDS;
+            if (tab_lds) {
+                __syncthreads(); /* the previous molecule's table is no longer read */
+                for (int k = l; k < mlj.y * mlj.y; k += TILE) s_tab[k] = d.ljab[lj0 + k];
+            }
         }
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double qi = d.q[i];
         const R ri = (R)d.reff[i];
         const int row = d.ljrow[i];
+        const double2 *gtab = d.ljab;
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const unsigned long long e = d.excl[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
@@ -774,6 +788,15 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         }
         __syncthreads();
         double fxi = 0.0, fyi = 0.0, fzi = 0.0, sdi = 0.0;
+        /* the column sweep, instantiated for the LJ table in LDS and in
+         * global memory and chosen per visit (a per-pair choice compiles to
+         * one generic-address load, slower than either) */
+        auto sweep = [&](auto in_lds) {
+        constexpr bool TAB = decltype(in_lds)::value;
+        auto lj = [&](int kk) -> double2 {
+            if constexpr (TAB) return s_tab[kk - lj0];
+            else return gtab[kk];
+        };
         int k = l % m;
         if constexpr (sizeof(R) == sizeof(float)) {
             /* mixed: two steps per iteration, branch-free, so the two pairs'
@@ -794,7 +817,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     const bool act = (u == 0 || two) && ((mr >> b) & 1ull);
                     const bool coul = act && !((e >> b) & 1ull);
                     pair_terms<R>(P[u], act, coul, xi, yi, zi, qi, ri, s_x[kq], s_y[kq], s_z[kq], s_q[kq], s_r[kq],
-                                  d.ljab[row + s_lj[kq]], epsinv, kks, screened);
+                                  lj(row + s_lj[kq]), epsinv, kks, screened);
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
@@ -852,7 +875,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                         ded -= el * r2inv;
                         const double r6inv = r2inv * r2inv * r2inv;
                         const int kk = row + s_lj[k];
-                        const double2 ab = d.ljab[kk];
+                        const double2 ab = lj(kk);
                         const double f6 = ab.y * r6inv;
                         const double f12 = ab.x * r6inv * r6inv;
                         evdw += f12 - f6;
@@ -867,6 +890,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                 k = k + 1 == m ? 0 : k + 1;
             }
         }
+        };
+        if (tab_lds) sweep(std::true_type{});
+        else sweep(std::false_type{});
         __syncthreads();
         if (mr) {
             unsafeAtomicAdd(&d.gx[i], fxi);
@@ -1253,6 +1279,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     std::vector<int> tile_mol(NT), tile_end(NT);
     std::vector<long long> rowbase(NT);
     std::vector<double2> ljab;
+    std::vector<int2> mol_lj(nmol);
     std::vector<double> rgbpsmax2(nmol);
     std::vector<int> ex_ptr(E.natoms + 1, 0), ex_j;
     long long cap = 0;
@@ -1263,6 +1290,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         std::vector<int> order;
         tile_order(n, x0 ? x0[m] : nullptr, order);
         const int lj0 = (int)ljab.size(), nt = t.ntypes;
+        mol_lj[m] = make_int2(lj0, nt);
         rgbpsmax2[m] = (o->rgbmax + t.fsmax) * (o->rgbmax + t.fsmax);
         for (int a = 0; a < nt; a++)
             for (int b = 0; b < nt; b++) {
@@ -1336,7 +1364,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.upload(&D.tile_mol, tile_mol)) || (rc = E.upload(&D.tile_end, tile_end)) ||
         (rc = E.upload(&D.rowbase, rowbase)) || (rc = E.upload(&D.ex_ptr, ex_ptr)) ||
         (rc = E.upload(&D.ex_j, ex_j)) || (rc = E.upload(&D.inv, inv)) ||
-        (rc = E.upload(&D.ljab, ljab)) ||
+        (rc = E.upload(&D.ljab, ljab)) || (rc = E.upload(&D.mol_lj, mol_lj)) ||
         (rc = E.upload(&D.rgbpsmax2, rgbpsmax2)))
         return rc;
     {

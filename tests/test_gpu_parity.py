@@ -212,6 +212,47 @@ def test_derivative_store_off_matches_oracle(gpu, gen_dir, precision):
     assert rel(e0[0], e1[0]) <= 1e-12 and rel_rms(e0[3], e1[3]) <= (1e-12 if precision == 0 else 1e-6)
 
 
+def _split_types(arr, copies):
+    """The same topology with every LJ atom type split into `copies`
+    identical types (atom a takes copy a mod copies): same coefficients for
+    every pair, a table copies^2 times larger."""
+    nt = int(arr["ntypes"])
+    t = np.arange(nt * copies) % nt
+    out = dict(arr)
+    out["ntypes"] = nt * copies
+    out["iac"] = (np.asarray(arr["iac"]) + nt * (np.arange(arr["natom"]) % copies)).astype(np.int32)
+    out["cno"] = np.asarray(arr["cno"]).reshape(nt, nt)[t[:, None], t[None, :]].ravel().astype(np.int32)
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lj_table_lds_and_global_agree(gpu, gen_dir, precision):
+    """Pass 2 reads the LJ coefficients from LDS when the molecule has at
+    most 16 types and from global memory otherwise: the 5-type generated
+    system and the same system with 20 identical-coefficient types (400
+    table entries) give the same energies and gradient (FP64 sums aside),
+    and both match the oracle on the 5-type topology."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 11, gen_dir)
+    big = _split_types(arr, 4)
+    assert big["ntypes"] ** 2 > 256 >= arr["ntypes"] ** 2
+    out = []
+    for a in (arr, big):
+        ctx = na.GBContext(a, x, gb=5, cut=14.0, rgbmax=12.0, kappa=0.1, precision=precision)
+        ctx.nblist()
+        out.append(ctx.egb())
+        ctx.close()
+    for c in range(3):
+        assert rel(out[1][c], out[0][c]) <= 1e-12, c
+    assert rel_rms(out[1][3], out[0][3]) <= 1e-12
+    orc = oracle.Oracle(parm, arr["natom"], gb=5, cut=14.0, rgbmax=12.0, kappa=0.1)
+    orc.nblist(x)
+    r = orc.egb(x)
+    tol_e = FP64_E if precision == 0 else MIX_E
+    assert rel(out[1][1], r[1]) <= tol_e and rel(out[1][2], r[2]) <= tol_e
+    orc.close()
+
+
 def test_derivative_store_outgrown_by_async_rebuild(gpu, gen_dir):
     """An asynchronous rebuild whose list no longer fits the store falls back
     to recomputation (and stays correct); the next synchronous build grows
