@@ -487,7 +487,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
     typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE];
-    __shared__ int s_nk[TILE];
+    __shared__ int s_nk[TILE]; /* neck index n | (n * NECK_N) << 8 */
     __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
-        const int ni = d.neck[i];
+        const int ni = d.neck[i], niN = ni * NECK_N;
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             const R rj = ldri0<R>(d, j);
             s_ri[pos] = rj; s_ri1i[pos] = R(1) / rj;
             s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
-            s_nk[pos] = d.neck[j];
+            s_nk[pos] = d.neck[j] | (d.neck[j] * NECK_N) << 8;
             s_acc[pos] = 0.0;
         }
         __syncthreads();
@@ -548,15 +548,15 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                     R tij = descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, aij);
                     R tji = descreen_d<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji, aji);
                     if (p.has_neck) {
-                        const int nj = s_nk[k];
+                        const int nk = s_nk[k], nij = niN + (nk & 255), nji = (nk >> 8) + ni;
                         if (dij < rbi + s_rb[k] + neckcut) {
                             R dn;
                             if (!skip_ij) {
-                                tij += neck_d<R>(dij, dij1i, npos[ni * NECK_N + nj], nval[ni * NECK_N + nj], neckscale, dn);
+                                tij += neck_d<R>(dij, dij1i, npos[nij], nval[nij], neckscale, dn);
                                 aij += dn;
                             }
                             if (!skip_ji) {
-                                tji += neck_d<R>(dij, dij1i, npos[nj * NECK_N + ni], nval[nj * NECK_N + ni], neckscale, dn);
+                                tji += neck_d<R>(dij, dij1i, npos[nji], nval[nji], neckscale, dn);
                                 aji += dn;
                             }
                         }
