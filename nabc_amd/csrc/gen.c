@@ -19,8 +19,12 @@
  *   - partial charges ~ N(0, 0.35) e, shifted to a net charge of zero and
  *     stored in the prmtop unit (e * 18.2223);
  *   - 5 Lennard-Jones types (one per class), Lorentz-Berthelot combination
- *     of (Rmin/2, epsilon) = H (1.00, 0.02), O (1.25, 0.15), N (1.30, 0.15),
- *     C (1.35, 0.10), S (1.45, 0.25);
+ *     of (Rmin/2, epsilon) = H (1.487, 0.0157), O (1.25, 0.15),
+ *     N (1.30, 0.15), C (1.35, 0.10), S (1.45, 0.25); the hydrogen values are
+ *     those of aliphatic hydrogen in common biomolecular force fields: with
+ *     a smaller wall, pairs of opposite partial charges pull together below
+ *     1 A (the Coulomb attraction of two 0.35 e charges outweighs a
+ *     (1.0, 0.02) wall there) and dynamics bounces them off the r^-12 wall;
  *   - bonds (Rk 300, Req 1.5), angles (Tk 40, Teq 115 deg) and dihedrals
  *     from chain neighbours; three dihedral types, one of them a two-term
  *     series (Pn = -3 then 1) so the multi-term path is exercised;
@@ -264,8 +268,8 @@ static const double cls_radius[5] = {1.2, 1.5, 1.55, 1.7, 1.8};
 static const double cls_screen[5] = {0.85, 0.85, 0.79, 0.72, 0.96};
 static const int cls_atnum[5] = {1, 8, 7, 6, 16};
 static const double cls_mass[5] = {1.008, 16.00, 14.01, 12.01, 32.06};
-static const double cls_rmin2[5] = {1.00, 1.25, 1.30, 1.35, 1.45};
-static const double cls_eps[5] = {0.02, 0.15, 0.15, 0.10, 0.25};
+static const double cls_rmin2[5] = {1.487, 1.25, 1.30, 1.35, 1.45};
+static const double cls_eps[5] = {0.0157, 0.15, 0.15, 0.10, 0.25};
 
 int gbgen_build(int natom, unsigned long long seed, PARMSTRUCT_T **prm_out, double *x_out)
 {
