@@ -229,7 +229,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
                  const double *const *x0);
 int engine_gather(Engine &E, const double *x_dev);
 int engine_nblist(Engine &E);
-int engine_store_reserve(Engine &E);
+int engine_store_reserve(Engine &E, bool upper = false);
 int engine_eval(Engine &E, const double *x_dev, double *g_dev);
 
 #endif

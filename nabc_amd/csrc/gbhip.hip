@@ -1465,17 +1465,22 @@ int engine_nblist(Engine &E)
  * engine stream).  Grows it to 1.25x the need when it is too small, unless a
  * captured graph holds the current pointer or the device lacks the memory;
  * in those cases the list is evaluated with pass 3 recomputing the
- * derivatives (the device flag written at list build says which).
+ * derivatives (the device flag written at list build says which).  With
+ * upper (before dynamics graphs are captured) it is sized for the largest
+ * list the molecules can have, every tile pair full, when that takes at
+ * most a quarter of the free device memory (25k atoms: 2.5 GB mixed, 5 GB
+ * FP64), else as above.
  */
-int engine_store_reserve(Engine &E)
+int engine_store_reserve(Engine &E, bool upper)
 {
     long long need = 0;
     HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     if (E.store_mode == 0 || E.store_frozen) return 0;
-    if (E.d.dstore && need < E.d.store_cap) return 0;
+    const long long full = E.tp_cap * TILE * TILE + (long long)TILE * TILE;
+    if (E.d.dstore && need < E.d.store_cap && (!upper || E.d.store_cap >= full)) return 0;
     const size_t esz = E.precision == GBHIP_FP64 ? 2 * sizeof(double) : 2 * sizeof(float);
-    const long long cap = need + need / 4 + (long long)TILE * TILE;
+    long long cap = need + need / 4 + (long long)TILE * TILE;
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     if (E.store_mem) {
         HIPCHK(hipStreamSynchronize(E.stream));
@@ -1485,7 +1490,12 @@ int engine_store_reserve(Engine &E)
         E.d.store_cap = 0;
     }
     size_t freeb = 0, totb = 0;
-    if (hipMemGetInfo(&freeb, &totb) != hipSuccess || (size_t)cap * esz > freeb / 2) {
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (upper && (size_t)full * esz <= freeb / 4) cap = full;
+    if ((size_t)cap * esz > freeb / 2) {
         (void)hipGetLastError();
         return 0;
     }

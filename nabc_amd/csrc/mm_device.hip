@@ -674,7 +674,9 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
     E.store_frozen = false;
     /* step 0: list build and forces (md() calls mme with iter 0) */
     if ((rc = engine_nblist(E))) return rc;
-    if ((rc = engine_store_reserve(E))) return rc;
+    /* the step graphs freeze the store: size it for any list this molecule
+     * can have, so that no rebuild during the run falls back to recomputing */
+    if ((rc = engine_store_reserve(E, true))) return rc;
     if ((rc = mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
     hipLaunchKernelGGL(k_md_accel0, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, ctx->B.frozen,
                        (const double *)E.g_dev);

@@ -93,3 +93,30 @@ def test_langevin_temperature(gpu, gen_dir):
     assert 270.0 < temp < 330.0, temp
     it = na.C.c_int(-3)
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_md_store_holds_any_list(gpu, gen_dir):
+    """The step graphs freeze the pass-3 derivative store, so gbhip_md_init
+    sizes it for the largest list the molecule can have (every tile pair
+    full): the store stays in use after the run, whatever the list grew to."""
+    na = gpu
+    parm, arr, x0 = generated_system(2500, 23, gen_dir)
+    _init(na, parm, "gb=5, cut=12.0, rgbmax=12.0, ntpr=100000")
+    ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+    assert ctx
+    o = na.MDOptions(dt=0.001, temp0=300.0, tempi=300.0, gamma_ln=5.0, tautp=999999.0, vlimit=10.0,
+                     boltz2=9.93595e-4, nsnb=10, zerov=0, seed=3)
+    masses = na.parm_arrays(parm)["mass"]
+    assert na.lib.gbhip_md_init(ctx, masses.ctypes.data_as(na.dp), x0.ctypes.data_as(na.dp), None,
+                                na.C.byref(o), 50) == 0, na.lib.gbhip_last_error()
+    nt, ntp = na.C.c_int(), na.C.c_int()
+    assert na.lib.gbhip_tile_stats(ctx, na.C.byref(nt), na.C.byref(ntp)) == 0
+    need, cap, used = na.C.c_longlong(), na.C.c_longlong(), na.C.c_int()
+    assert na.lib.gbhip_store_stats(ctx, na.C.byref(need), na.C.byref(cap), na.C.byref(used)) == 0
+    full = nt.value * (nt.value + 1) // 2 * 64 * 64
+    assert used.value == 1 and cap.value >= full > need.value
+    assert na.lib.gbhip_md_run(ctx, 50, 1) == 0, na.lib.gbhip_last_error()
+    assert na.lib.gbhip_store_stats(ctx, na.C.byref(need), na.C.byref(cap), na.C.byref(used)) == 0
+    assert used.value == 1
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
