@@ -1488,6 +1488,11 @@ int engine_store_reserve(Engine &E, bool upper)
         E.store_mem = nullptr;
         E.d.dstore = nullptr;
         E.d.store_cap = 0;
+        /* the flag the list build set may name the store just freed: clear
+         * it until a new store is in place */
+        hipLaunchKernelGGL(k_setflag, dim3(1), dim3(64), 0, E.stream, E.d.store_ok, 0);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(E.stream));
     }
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) {
