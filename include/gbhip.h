@@ -132,7 +132,10 @@ int gbhip_get_sumdeijda(gbhip_ctx *ctx, double *sumdeijda);
 
 /* ---- device-resident use (no host copies; asynchronous on the stream) ---- */
 
-/* Stream for all work of this context (a hipStream_t; NULL = default). */
+/* Stream for all later work of this context (a hipStream_t; NULL =
+ * default); waits for the work queued on the previous stream first.
+ * Dynamics graphs keep running on whichever stream is current when
+ * gbhip_md_run is called. */
 int gbhip_set_stream(gbhip_ctx *ctx, void *stream);
 
 /* Coordinates from device memory d_x[3*natom] (AoS, FP64). */

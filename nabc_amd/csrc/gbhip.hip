@@ -1903,6 +1903,9 @@ extern "C" int gbhip_ctx_destroy(gbhip_ctx *ctx)
 extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    /* work already queued on the old stream finishes before any on the new
+     * one can start (the context's calls assume one ordered stream) */
+    if ((hipStream_t)stream != ctx->E.stream) HIPCHK(hipStreamSynchronize(ctx->E.stream));
     ctx->E.stream = (hipStream_t)stream;
     return 0;
 }
