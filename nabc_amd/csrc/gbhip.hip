@@ -1905,7 +1905,13 @@ extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
     /* work already queued on the old stream finishes before any on the new
      * one can start (the context's calls assume one ordered stream) */
-    if ((hipStream_t)stream != ctx->E.stream) HIPCHK(hipStreamSynchronize(ctx->E.stream));
+    if ((hipStream_t)stream != ctx->E.stream) {
+        HIPCHK(hipStreamSynchronize(ctx->E.stream));
+        /* a stream the engine created for dynamics goes now; the caller's
+         * stream is never destroyed by the engine */
+        if (ctx->E.own_stream && ctx->E.stream) (void)hipStreamDestroy(ctx->E.stream);
+        ctx->E.own_stream = false;
+    }
     ctx->E.stream = (hipStream_t)stream;
     return 0;
 }

@@ -120,3 +120,34 @@ def test_md_store_holds_any_list(gpu, gen_dir):
     assert used.value == 1
     it = na.C.c_int(-3)
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_set_stream_after_dynamics_keeps_callers_stream(gpu, gen_dir):
+    """gbhip_md_init gives a context without a stream one of its own; a later
+    gbhip_set_stream hands the context the caller's stream (dynamics then
+    replay there) and the engine releases only the stream it created: the
+    caller's stream is still usable after the context is freed."""
+    import torch
+    na = gpu
+    parm, arr, x0 = generated_system(600, 24, gen_dir)
+    _init(na, parm, "gb=5, cut=12.0, rgbmax=12.0, ntpr=100000")
+    ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+    assert ctx
+    o = na.MDOptions(dt=0.001, temp0=300.0, tempi=300.0, gamma_ln=5.0, tautp=999999.0, vlimit=10.0,
+                     boltz2=9.93595e-4, nsnb=10, zerov=0, seed=5)
+    masses = na.parm_arrays(parm)["mass"]
+    assert na.lib.gbhip_md_init(ctx, masses.ctypes.data_as(na.dp), x0.ctypes.data_as(na.dp), None,
+                                na.C.byref(o), 40) == 0, na.lib.gbhip_last_error()
+    assert na.lib.gbhip_md_run(ctx, 10, 1) == 0, na.lib.gbhip_last_error()
+    s = torch.cuda.Stream(device=0)
+    assert na.lib.gbhip_set_stream(ctx, na.C.c_void_p(s.cuda_stream)) == 0, na.lib.gbhip_last_error()
+    assert na.lib.gbhip_md_run(ctx, 10, 1) == 0, na.lib.gbhip_last_error()
+    x = np.zeros_like(x0)
+    assert na.lib.gbhip_md_state(ctx, x.ctypes.data_as(na.dp), None, None) == 0
+    assert np.all(np.isfinite(x)) and np.max(np.abs(x - x0)) > 1e-4
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+    with torch.cuda.stream(s):
+        t = torch.ones(1024, device="cuda:0") * 2.0
+    s.synchronize()
+    assert float(t.sum()) == 2048.0
