@@ -259,6 +259,20 @@ int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14);
 /* Same, device pointers, asynchronous. */
 int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14);
 
+/*
+ * gbhip_mme's device work as one captured hipGraph (on = 1, the default):
+ * captured at the first call and re-captured when a launch argument changes
+ * (a regrown derivative store, new launch shapes, new bonded terms), so each
+ * minimizer step costs one graph launch instead of a dozen kernel launches.
+ * A context without a stream is given one of its own at the first capture.
+ * During a timing session (gbhip_timing_start) evaluations are launched
+ * kernel by kernel.  on = 0 launches kernel by kernel always.  The stats
+ * count captures and graph launches.  No counterpart in the reference
+ * (mme34, eff.c:2442-2947, is host code).
+ */
+int gbhip_set_mme_graph(gbhip_ctx *ctx, int on);
+int gbhip_mme_graph_stats(gbhip_ctx *ctx, long long *captures, long long *replays);
+
 /* ---- dynamics on the device (md(), sff.c:2071-2366) ---- */
 
 typedef struct {

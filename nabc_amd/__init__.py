@@ -161,6 +161,8 @@ _sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
 _sig("gbhip_mme", C.c_int, [vp, dp, dp])
 _sig("gbhip_mme_device", C.c_int, [vp, dp, dp])
+_sig("gbhip_set_mme_graph", C.c_int, [vp, C.c_int])
+_sig("gbhip_mme_graph_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)])
 _sig("gbhip_batch_create", C.c_int, [C.c_int, C.POINTER(Topology), C.POINTER(Options),
                                      C.POINTER(dp), C.POINTER(vp)])
 _sig("gbhip_batch_destroy", C.c_int, [vp])

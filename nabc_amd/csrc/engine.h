@@ -208,12 +208,25 @@ struct MDState {
     hipGraph_t gr_step = nullptr, gr_rebuild = nullptr;
 };
 
+/* captured mme evaluation (mm_device.hip): replayed while the launch
+ * arguments it was captured with (key) are unchanged */
+struct MmeGraph {
+    bool on = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+    long long captures = 0, replays = 0;
+};
+
 struct gbhip_ctx {
     Engine E;
     BondedDev B;
     MDState M;
+    MmeGraph G;
     ~gbhip_ctx()
     {
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.graph) (void)hipGraphDestroy(G.graph);
         if (M.g_step) (void)hipGraphExecDestroy(M.g_step);
         if (M.g_rebuild) (void)hipGraphExecDestroy(M.g_rebuild);
         if (M.gr_step) (void)hipGraphDestroy(M.gr_step);

@@ -398,11 +398,108 @@ extern "C" int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14)
     return mme_device(ctx, ctx->E.x_dev, d_grad, d_ene14);
 }
 
+/*
+ * Everything the launches of mme_device take from the host: device pointers
+ * (the derivative store's included), pair-kernel grids, precision, model
+ * parameters and the bonded term counts.  A captured evaluation is replayed
+ * only while this is unchanged; list contents, coordinates and the store's
+ * in-use flag live on the device and need no re-capture.
+ */
+static std::vector<unsigned char> mme_key(const gbhip_ctx *c)
+{
+    const Engine &E = c->E;
+    std::vector<unsigned char> k;
+    auto put = [&k](const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        k.insert(k.end(), b, b + n);
+    };
+    put(&E.d, sizeof(E.d));
+    put(&E.par, sizeof(E.par));
+    put(E.pass_blocks, sizeof(E.pass_blocks));
+    put(&E.precision, sizeof(E.precision));
+    put(&E.x_dev, sizeof(E.x_dev));
+    put(&E.g_dev, sizeof(E.g_dev));
+    put(&E.ene14_dev, sizeof(E.ene14_dev));
+    put(&c->B, sizeof(c->B));
+    return k;
+}
+
+/*
+ * One mme evaluation as a hipGraph launch on the engine stream: captured at
+ * the first call and again whenever mme_key changes (a regrown derivative
+ * store, new grids, new bonded terms), so a minimizer's calls cost one
+ * launch each instead of a dozen.  A context without a stream gets one of
+ * its own (capture needs one), as gbhip_md_init does.
+ */
+static int mme_graph_launch(gbhip_ctx *c)
+{
+    Engine &E = c->E;
+    MmeGraph &G = c->G;
+    if (!c->B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
+    if (!E.have_list) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
+    if (!E.stream) {
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        E.stream = s;
+        E.own_stream = true;
+    }
+    std::vector<unsigned char> key = mme_key(c);
+    if (!G.exec || key != G.key) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        HIPCHK(hipStreamSynchronize(E.stream));
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.graph) (void)hipGraphDestroy(G.graph);
+        G.exec = nullptr;
+        G.graph = nullptr;
+        G.key.clear();
+        HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+        const int rc = mme_device(c, E.x_dev, E.g_dev, E.ene14_dev);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(E.stream, &g);
+        if (rc || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return rc ? rc : seterr(GBHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            G.exec = nullptr;
+            return seterr(GBHIP_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+        }
+        G.graph = g;
+        G.key.swap(key);
+        G.captures++;
+    }
+    HIPCHK(hipGraphLaunch(G.exec, E.stream));
+    G.replays++;
+    return 0;
+}
+
+extern "C" int gbhip_set_mme_graph(gbhip_ctx *ctx, int on)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (on != 0 && on != 1) return seterr(GBHIP_EINVAL, "mme graph mode must be 0 or 1");
+    ctx->G.on = on != 0;
+    return 0;
+}
+
+extern "C" int gbhip_mme_graph_stats(gbhip_ctx *ctx, long long *captures, long long *replays)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (captures) *captures = ctx->G.captures;
+    if (replays) *replays = ctx->G.replays;
+    return 0;
+}
+
 extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
     Engine &E = ctx->E;
-    int rc = mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
+    /* per-pass timing events cannot sit inside a graph: a timing session
+     * evaluates with plain launches */
+    const bool graph = ctx->G.on && E.tev_cap == 0;
+    int rc = graph ? mme_graph_launch(ctx) : mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
     if (rc) return rc;
     double e[14];
     if (grad)
