@@ -47,6 +47,12 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gb_25k_igb8_traffic.json")
 
 
+def kernel_name(pass_, real, split):
+    k = ("born", "pairs", "chain").index(pass_)
+    base = ("k_born", "k_pairs", "k_chain_st")[k]
+    return "%s<%s, %s>" % (base, real, "true" if split[k] > 1 else "false")
+
+
 def measured_traffic(kernel):
     try:
         with open(TRAFFIC_FILE) as fp:
@@ -74,6 +80,7 @@ def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None):
     else:
         launch = ctx.autotune(5) if tune else ctx.launch()
     store_need, store_cap, store_used = ctx.store_stats()
+    split = ctx.split()
     for k in range(warmup):
         if k % NSNB == 0:
             ctx.nblist_async()
@@ -99,7 +106,7 @@ def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None):
     ctx.close()
     return dict(evals_per_s=steps / (t1 - t0), ms_per_eval=1e3 * (t1 - t0) / steps,
                 pass_ms=dict(born=ms[0] / n, radii=ms[1] / n, pairs=ms[2] / n, chain=ms[3] / n),
-                nblist_ms=1e3 * tl, npairs=npairs, ntiles=ntiles, tilepairs=ntp, launch=launch,
+                nblist_ms=1e3 * tl, npairs=npairs, ntiles=ntiles, tilepairs=ntp, launch=launch, split=split,
                 store=dict(entries=store_need, allocated=store_cap, in_use=store_used),
                 energies=dict(epol=epol, evdw=evdw, eelec=eel), grad=g)
 
@@ -454,7 +461,9 @@ def main():
     P = mixed["npairs"]
     dom = max(("born", "pairs", "chain"), key=lambda k: mixed["pass_ms"][k])
     achieved = FLOPS[dom] * P / (mixed["pass_ms"][dom] * 1e-3) / 1e12
-    kname = {"born": "k_born<float>", "pairs": "k_pairs<float>", "chain": "k_chain_st<float>"}[dom]
+    # the kernels are instantiated with and without the step split (the
+    # profiler names them k_born<float, false> etc.)
+    kname = kernel_name(dom, "float", mixed["split"])
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK["mixed"], unit="TFLOP/s",
                     frac=round(achieved / PEAK["mixed"], 4), traffic=measured_traffic(kname), kernel=kname,
                     kernel_ms=round(mixed["pass_ms"][dom], 4))
@@ -505,11 +514,11 @@ def main():
                    "natom": NATOM, "seed": SEED, "npairs": P, "tiles": mixed["ntiles"],
                    "tile_pairs": mixed["tilepairs"], "sha256": digests,
                    "launch_blocks": {"mixed": mixed["launch"], "fp64": fp64["launch"]},
+                   "step_split": {"mixed": list(mixed["split"]), "fp64": list(fp64["split"])},
                    "derivative_store": {"mixed": mixed["store"], "fp64": fp64["store"]}},
         "fp64": {"value": round(fp64["evals_per_s"], 2), "ms_per_step": round(fp64["ms_per_eval"], 4),
                  "pass_ms": {k: round(v, 4) for k, v in fp64["pass_ms"].items()},
-                 "dominant": {"born": "k_born<double>", "pairs": "k_pairs<double>",
-                              "chain": "k_chain_st<double>"}[dom64], "achieved_tflops": round(a64, 3),
+                 "dominant": kernel_name(dom64, "double", fp64["split"]), "achieved_tflops": round(a64, 3),
                  "frac": round(a64 / PEAK["fp64"], 4)},
         "mixed_vs_fp64": round(mixed["evals_per_s"] / fp64["evals_per_s"], 3),
         "mixed_pass_ms": {k: round(v, 4) for k, v in mixed["pass_ms"].items()},

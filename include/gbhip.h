@@ -173,17 +173,34 @@ int gbhip_valu_peak(int precision, double *tflops);
 /*
  * Launch shape of the three pair kernels (pass 1, pass 2, pass 3): the
  * number of one-wavefront workgroups each is launched with (1 to 65536);
- * each workgroup takes every G-th tile pair of the cost-sorted list.
+ * each workgroup takes every G-th work item (a tile pair, or a chunk of its
+ * steps, see gbhip_set_split) of the cost-sorted list.
  * gbhip_set_launch takes blocks3[3] (0 keeps a value); gbhip_autotune times
  * reps evaluations on the current list with and without the derivative
  * store (gbhip_set_derivative_store; the faster is kept, so it may switch
- * the store off), then for each candidate size (4 to 128 workgroups per CU,
- * and one per non-empty tile pair), and keeps the fastest per pass, writing
+ * the store off), then each step split for small systems, then each
+ * candidate size (4 to 128 workgroups per CU, and one per work item),
+ * and keeps the fastest per pass, writing
  * the choice to chosen3 (may be NULL).  MD graphs captured earlier keep the
  * shapes they were captured with.  No counterpart in the reference (egb()
  * has no launch configuration); nblist must have run.
  */
 int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3);
+
+/*
+ * Step split: a work item of the three pair kernels is one wavefront's sweep
+ * over the steps of a tile pair; with split = 2, 4 or 8 each tile pair's
+ * steps are dealt out in that many chunks, one work item each, so that a
+ * system with few tile pairs still occupies the chip (600 atoms: 55 tile
+ * pairs for 1,024 SIMDs).  split3[3] per pass (pass 1, pass 2, pass 3):
+ * 1, 2, 4, 8, or 0 for the default, which doubles the split, up to 8, while
+ * the pass has fewer than 96 (pass 1) or 4 (passes 2, 3) work items per CU;
+ * large systems run with 1.  gbhip_set_split resets the launch shapes to
+ * their defaults; gbhip_autotune also times each split for small systems.
+ * No counterpart in the reference (a launch-shape choice).
+ */
+int gbhip_set_split(gbhip_ctx *ctx, const int *split3);
+int gbhip_get_split(gbhip_ctx *ctx, int *split3);
 
 /*
  * Pass-3 derivative store.  Pass 1 can write, per listed pair, the two

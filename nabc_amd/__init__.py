@@ -156,6 +156,8 @@ _sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_batch_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_get_launch", C.c_int, [vp, ip])
+_sig("gbhip_set_split", C.c_int, [vp, ip])
+_sig("gbhip_get_split", C.c_int, [vp, ip])
 _sig("gbhip_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
@@ -509,6 +511,18 @@ class GBContext:
     def set_derivative_store(self, mode):
         """1: pass 1 stores the pass-3 derivatives when they fit; 0: never."""
         _check(lib.gbhip_set_derivative_store(self.h, mode))
+
+    def set_split(self, born=0, pairs=0, chain=0):
+        """Step chunks per tile pair for the three pair passes (1, 2, 4, 8;
+        0 = the default for the system's size); resets the launch shapes to
+        their defaults."""
+        b = np.array([born, pairs, chain], np.int32)
+        _check(lib.gbhip_set_split(self.h, _p(b, ip)))
+
+    def split(self):
+        b = np.zeros(3, np.int32)
+        _check(lib.gbhip_get_split(self.h, _p(b, ip)))
+        return tuple(int(v) for v in b)
 
     def store_stats(self):
         """(entries needed by the current list, entries allocated, in use)."""

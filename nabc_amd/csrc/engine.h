@@ -103,6 +103,10 @@ struct DevPtrs {
 
 struct GBPar {
     int gb, has_neck, nslots, nmol;
+    /* per pass (born, pairs, chain): each tile pair's steps are dealt out
+     * in 1 << split_sh chunks, one work item each (more waves than tile
+     * pairs for small systems) */
+    int split_sh[3], pad_;
     double cut2, rgbmax, rgbmax1i, rgbmax2i;
     double neckscale, kappa, epsext;
 };
@@ -136,9 +140,19 @@ struct Engine {
     void *store_mem = nullptr;
     int store_mode = 1;        /* 1: used when it fits, 0: always recompute */
     bool store_frozen = false; /* a captured graph holds the current pointer */
+    /* pinned host staging for coordinates in and gradient/energies out
+     * (3*natoms + 3*natoms + 14 doubles); stage_ev marks the end of the
+     * last copy out of the coordinate part */
+    double *h_stage = nullptr;
+    hipEvent_t stage_ev = nullptr;
 
     ~Engine()
     {
+        if (h_stage) {
+            if (stream) (void)hipStreamSynchronize(stream);
+            (void)hipHostFree(h_stage);
+        }
+        if (stage_ev) (void)hipEventDestroy(stage_ev);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         if (store_mem) (void)hipFree(store_mem);
         if (own_stream && stream) {
@@ -244,5 +258,6 @@ int engine_gather(Engine &E, const double *x_dev);
 int engine_nblist(Engine &E);
 int engine_store_reserve(Engine &E, bool upper = false);
 int engine_eval(Engine &E, const double *x_dev, double *g_dev);
+int engine_stage(Engine &E);
 
 #endif

@@ -54,6 +54,7 @@
 #define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
 #define LJ_LDS 256 /* LJ table entries pass 2 keeps in LDS */
 #define CHAIN_DEPTH 8  /* derivative-store loads in flight per lane in pass 3 */
+#define SPLIT_MAX 8    /* step chunks per tile pair, at most */
 
 /* ------------------------------------------------------------------ errors */
 
@@ -421,6 +422,28 @@ __device__ __forceinline__ int snake_item(int r)
     return r * G + ((r & 1) ? (G - 1 - b) : b);
 }
 
+/*
+ * Work item q of a pair pass: the tile pair at queue position q >> sh, and
+ * of its m steps the chunk c = q mod 2^sh, steps [m c / 2^sh, m (c+1) / 2^sh)
+ * (sh = 0: one item per tile pair, all steps).  Chunks of one tile pair are
+ * consecutive items, so they go to different workgroups.
+ */
+__device__ __forceinline__ void item_steps(int q, int sh, int m, int &sb, int &se)
+{
+    const int c = q & ((1 << sh) - 1);
+    sb = (m * c) >> sh;
+    se = (m * (c + 1)) >> sh;
+}
+
+/* listed pairs of all lanes at the steps before sb (derivative-store
+ * entries that precede the chunk's first step) */
+__device__ __forceinline__ int entries_before(unsigned long long mr, int sb)
+{
+    int n = __popcll(mr & ((1ull << sb) - 1ull));
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    return n;
+}
+
 /* ------------------------------------------------------------ evaluation */
 
 __global__ __launch_bounds__(256) void k_gather(DevPtrs d, GBPar p, const double *__restrict__ x)
@@ -481,7 +504,7 @@ __device__ __forceinline__ int lanes_below(unsigned long long bal)
  * When the derivative store is large enough for the current list, the
  * pass-3 numerators of both directions are written as well.
  */
-template <typename R>
+template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
 {
     typedef typename Vec2<R>::T R2;
@@ -501,15 +524,20 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             nval[k] = neckval<R>(d)[k];
         }
     }
+    /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
+    const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
     QMeta nxt;
-    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
-    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+    if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
+    for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
         /* this visit's metadata was loaded one visit ahead */
         const QMeta mq = nxt;
-        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        if (snake_item(r + 1) < nit) nxt = d.qm[snake_item(r + 1) >> sh];
         const int t = mq.t;
         const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
+        int sb, se;
+        item_steps(q, sh, m, sb, se);
+        if (sb == se) continue;
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
@@ -520,7 +548,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
-        R2 *sp = store ? st + mq.start : nullptr;
+        R2 *sp = store ? st + mq.start + (sb ? entries_before(mr, sb) : 0) : nullptr;
         __syncthreads();
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
@@ -532,8 +560,8 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         }
         __syncthreads();
         double acc = 0.0;
-        int k = l % m;
-        for (int s = 0; s < m; s++) {
+        int k = (l + sb) % m;
+        for (int s = sb; s < se; s++) {
             R aij = R(0), aji = R(0);
             const bool listed = (mr >> s) & 1ull;
             if (listed) {
@@ -720,7 +748,7 @@ __global__ __launch_bounds__(64) void k_enesum(DevPtrs d, int n)
     d.ene[s] = acc;
 }
 
-template <typename R>
+template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
 {
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
@@ -739,15 +767,20 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     double epol = 0.0, eel = 0.0, evdw = 0.0;
     int cur_mol = -1, lj0 = 0;
     bool tab_lds = false;
+    /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
+    const int sh = SPL ? p.split_sh[1] : 0, nit = nnz << sh;
     QMeta nxt;
-    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
-    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+    if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
+    for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
         /* this visit's metadata was loaded one visit ahead */
         const QMeta mq = nxt;
-        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        if (snake_item(r + 1) < nit) nxt = d.qm[snake_item(r + 1) >> sh];
         const int t = mq.t;
         const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
+        int sb, se;
+        item_steps(q, sh, m, sb, se);
+        if (sb == se) continue;
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const int mol = d.slot_mol[i];
@@ -797,7 +830,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             if constexpr (TAB) return s_tab[kk - lj0];
             else return gtab[kk];
         };
-        int k = l % m;
+        int k = (l + sb) % m;
         if constexpr (sizeof(R) == sizeof(float)) {
             /* mixed: two steps per iteration, branch-free, so the two pairs'
              * dependency chains interleave (the kernel is latency-bound at
@@ -805,8 +838,8 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
              * contribute zeros chosen by select, never by multiplication (a
              * lane's own diagonal entry has r = 0).  Measured at 25k atoms:
              * 0.32 -> 0.30 ms; the FP64 kernel is faster as written below */
-            for (int s = 0; s < m; s += 2) {
-                const bool two = s + 1 < m;
+            for (int s = sb; s < se; s += 2) {
+                const bool two = s + 1 < se;
                 const int k1 = k + 1 == m ? 0 : k + 1;
                 const int kk2[2] = {k, k1};
                 PairOut P[2];
@@ -834,7 +867,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                 k = two ? (k1 + 1 == m ? 0 : k1 + 1) : k1;
             }
         } else {
-            for (int s = 0; s < m; s++) {
+            for (int s = sb; s < se; s++) {
                 const int b = s;
                 if ((mr >> b) & 1ull) {
                     const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
@@ -928,7 +961,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
  * derivative store written by pass 1 (memory-bound, few registers, high
  * occupancy); k_chain recomputes it (the store could not hold the list).
  */
-template <typename R>
+template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
 {
     typedef typename Vec2<R>::T R2;
@@ -939,21 +972,26 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R2 *st = (const R2 *)d.dstore;
+    /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
+    const int sh = SPL ? p.split_sh[2] : 0, nit = nnz << sh;
     QMeta nxt;
-    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
-    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+    if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
+    for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
         /* this visit's metadata was loaded one visit ahead */
         const QMeta mq = nxt;
-        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        if (snake_item(r + 1) < nit) nxt = d.qm[snake_item(r + 1) >> sh];
         const int t = mq.t;
         const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
+        int sb, se;
+        item_steps(q, sh, m, sb, se);
+        if (sb == se) continue;
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
-        const R2 *sp = st + mq.start;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        const R2 *sp = st + mq.start + (sb ? entries_before(mr, sb) : 0);
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
         __syncthreads();
@@ -965,18 +1003,18 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         __syncthreads();
         /* the packed entries of the next CHAIN_DEPTH steps are in flight
          * while a step is processed: sp walks ahead of k */
-        const int k0 = l % m;
+        const int k0 = (l + sb) % m;
         R2 win[CHAIN_DEPTH];
         bool act[CHAIN_DEPTH];
 #pragma unroll
         for (int u = 0; u < CHAIN_DEPTH; u++) {
             act[u] = false;
-            if (u < m) {
+            if (sb + u < se) {
                 /* every lane loads (lanes without a pair read the step's
                  * first entry, in bounds: the store holds more than the
                  * list needs), so no branch separates a load from its use
                  * and the loads stay in flight */
-                const bool a = (mr >> u) & 1ull;
+                const bool a = (mr >> (sb + u)) & 1ull;
                 const unsigned long long bal = __ballot(a);
                 win[u] = sp[a ? lanes_below(bal) : 0];
                 act[u] = a;
@@ -985,14 +1023,14 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         }
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
         int k = k0;
-        for (int s0 = 0; s0 < m; s0 += CHAIN_DEPTH) {
+        for (int s0 = sb; s0 < se; s0 += CHAIN_DEPTH) {
 #pragma unroll
             for (int u = 0; u < CHAIN_DEPTH; u++) {
                 const int s = s0 + u;
-                if (s < m) {
+                if (s < se) {
                     const R2 a = win[u];
                     const bool on = act[u];
-                    if (s + CHAIN_DEPTH < m) {
+                    if (s + CHAIN_DEPTH < se) {
                         const bool an = (mr >> (s + CHAIN_DEPTH)) & 1ull;
                         const unsigned long long bal = __ballot(an);
                         win[u] = sp[an ? lanes_below(bal) : 0];
@@ -1025,7 +1063,7 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     }
 }
 
-template <typename R>
+template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
 {
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
@@ -1044,15 +1082,20 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             nval[k] = neckval<R>(d)[k];
         }
     }
+    /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
+    const int sh = SPL ? p.split_sh[2] : 0, nit = nnz << sh;
     QMeta nxt;
-    if (snake_item(0) < nnz) nxt = d.qm[snake_item(0)];
-    for (int r = 0, q = snake_item(0); q < nnz; q = snake_item(++r)) {
+    if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
+    for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
         /* this visit's metadata was loaded one visit ahead */
         const QMeta mq = nxt;
-        if (snake_item(r + 1) < nnz) nxt = d.qm[snake_item(r + 1)];
+        if (snake_item(r + 1) < nit) nxt = d.qm[snake_item(r + 1) >> sh];
         const int t = mq.t;
         const unsigned long long cols = mq.cols;
         const int m = __popcll(cols);
+        int sb, se;
+        item_steps(q, sh, m, sb, se);
+        if (sb == se) continue;
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
@@ -1076,8 +1119,8 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
         }
         __syncthreads();
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        int k = l % m;
-        for (int s = 0; s < m; s++) {
+        int k = (l + sb) % m;
+        for (int s = sb; s < se; s++) {
             if ((mr >> s) & 1ull) {
                 const double dxd = xi - s_x[k], dyd = yi - s_y[k], dzd = zi - s_z[k];
                 const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
@@ -1242,6 +1285,38 @@ static int check_topology(const gbhip_topology *t, int gb)
             return seterr(GBHIP_EINVAL, "Atom %d outside the allowed range of 1-2 Angstroms for igb==%d", i, gb);
     }
     return 0;
+}
+
+/*
+ * Step split for a system with few tile pairs: one work item is one
+ * wavefront's sweep, so a system with fewer tile pairs than the chip has
+ * SIMDs leaves most of them idle, and a pass cannot end before its longest
+ * sweep.  Each tile pair's steps are dealt out in up to SPLIT_MAX chunks
+ * until a pass has split_items_per_cu work items per CU (at most as many as
+ * tile pairs allow): 96 for pass 1, whose sweeps are the longest, 4 for the
+ * others (measured, gb 8, 14 A cutoff: pass 1 gains from splitting up to
+ * 12k atoms, passes 2 and 3 lose beyond 2,500 atoms).
+ */
+static const int split_items_per_cu[3] = {96, 4, 4};
+
+static int default_split(const Engine &E, int pass)
+{
+    int sh = 0;
+    const long long cap = std::max<long long>(1, E.tp_cap);
+    while ((1 << sh) < SPLIT_MAX && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
+    return sh;
+}
+
+/* evaluation grids until autotuned (measured at 25k atoms, 25 A): pass 1
+ * fastest with one workgroup per work item, pass 2 near 64 per CU, pass 3
+ * near 40 per CU */
+static void default_grids(Engine &E)
+{
+    long long items[3];
+    for (int k = 0; k < 3; k++) items[k] = std::max<long long>(1, E.tp_cap) << E.par.split_sh[k];
+    E.pass_blocks[0] = (int)std::min<long long>(items[0], MAX_LAUNCH);
+    E.pass_blocks[1] = (int)std::min<long long>(items[1], (long long)E.ncu * 64);
+    E.pass_blocks[2] = (int)std::min<long long>(items[2], (long long)E.ncu * 40);
 }
 
 int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_options *o,
@@ -1419,13 +1494,8 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     E.pair_blocks = (int)std::min<long long>(std::max<long long>(1, cap), (long long)ncu * 16);
     E.pair_blocks = std::min(E.pair_blocks, PAIR_BLOCKS_MAX * 4);
     E.ncu = ncu;
-    /* evaluation grids until autotuned (measured at 25k atoms, 25 A): pass 1
-     * fastest with one workgroup per tile pair, pass 2 near 64 per CU, pass 3
-     * near 40 per CU */
-    const long long cap1 = std::max<long long>(1, cap);
-    E.pass_blocks[0] = (int)std::min<long long>(cap1, MAX_LAUNCH);
-    E.pass_blocks[1] = (int)std::min<long long>(cap1, (long long)ncu * 64);
-    E.pass_blocks[2] = (int)std::min<long long>(cap1, (long long)ncu * 40);
+    for (int k = 0; k < 3; k++) E.par.split_sh[k] = default_split(E, k);
+    default_grids(E);
     return 0;
 }
 
@@ -1553,25 +1623,34 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
         E.tev_used += 5;
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], E.stream));
+    const bool sp1 = E.par.split_sh[0] > 0, sp2 = E.par.split_sh[1] > 0, sp3 = E.par.split_sh[2] > 0;
     if (E.precision == GBHIP_FP64)
-        hipLaunchKernelGGL(k_born<double>, dim3(E.pass_blocks[0]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp1 ? k_born<double, true> : k_born<double, false>), dim3(E.pass_blocks[0]), dim3(64), 0,
+                           E.stream, E.d, E.par);
     else
-        hipLaunchKernelGGL(k_born<float>, dim3(E.pass_blocks[0]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp1 ? k_born<float, true> : k_born<float, false>), dim3(E.pass_blocks[0]), dim3(64), 0,
+                           E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
     hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
     if (E.precision == GBHIP_FP64)
-        hipLaunchKernelGGL(k_pairs<double>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp2 ? k_pairs<double, true> : k_pairs<double, false>), dim3(E.pass_blocks[1]), dim3(64), 0,
+                           E.stream, E.d, E.par);
     else
-        hipLaunchKernelGGL(k_pairs<float>, dim3(E.pass_blocks[1]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp2 ? k_pairs<float, true> : k_pairs<float, false>), dim3(E.pass_blocks[1]), dim3(64), 0,
+                           E.stream, E.d, E.par);
     hipLaunchKernelGGL(k_enesum, dim3((3 * E.nmol + 63) / 64), dim3(64), 0, E.stream, E.d, 3 * E.nmol);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
     if (E.precision == GBHIP_FP64) {
-        hipLaunchKernelGGL(k_chain_st<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_chain<double>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp3 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
+                           dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp3 ? k_chain<double, true> : k_chain<double, false>), dim3(E.pass_blocks[2]), dim3(64),
+                           0, E.stream, E.d, E.par);
     } else {
-        hipLaunchKernelGGL(k_chain_st<float>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL(k_chain<float>, dim3(E.pass_blocks[2]), dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp3 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
+                           dim3(64), 0, E.stream, E.d, E.par);
+        hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]), dim3(64), 0,
+                           E.stream, E.d, E.par);
     }
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (g_dev)
@@ -1625,6 +1704,28 @@ static int timing_stop(Engine &E, double *ms4, int *nevals)
 
 /* ------------------------------------------------------------ launch shapes */
 
+/* per pass 1, 2, 4 or 8 step chunks per tile pair (0: the default for the
+ * system's size); the grids return to their defaults for the new split */
+static int set_split(Engine &E, const int *split3)
+{
+    int sh[3];
+    for (int k = 0; k < 3; k++) {
+        if (split3[k] == 0) {
+            sh[k] = default_split(E, k);
+        } else {
+            sh[k] = 0;
+            while (sh[k] < 4 && (1 << sh[k]) != split3[k]) sh[k]++;
+            if (sh[k] == 4 || split3[k] > SPLIT_MAX)
+                return seterr(GBHIP_EINVAL, "step split must be 0, 1, 2, 4 or 8");
+        }
+        if ((std::max<long long>(1, E.tp_cap) << sh[k]) > (1LL << 30))
+            return seterr(GBHIP_EINVAL, "too many tile pairs for a step split of %d", 1 << sh[k]);
+    }
+    for (int k = 0; k < 3; k++) E.par.split_sh[k] = sh[k];
+    default_grids(E);
+    return 0;
+}
+
 static int set_launch(Engine &E, const int *blocks3)
 {
     for (int k = 0; k < 3; k++) {
@@ -1637,9 +1738,11 @@ static int set_launch(Engine &E, const int *blocks3)
 
 /*
  * Time whole evaluations on the current list and coordinates: first with
- * and without the pass-3 derivative store (keeping the faster), then for
- * each candidate grid size (workgroups per CU: 4 .. 128, and one workgroup
- * per non-empty tile pair), keeping per pass the fastest.  Up to the number
+ * and without the pass-3 derivative store (keeping the faster), then, for a
+ * system with few tile pairs, each step split (1, 2, 4, 8; the fastest per
+ * pass is kept), then for each candidate
+ * grid size (workgroups per CU: 4 .. 128, and one workgroup per work item),
+ * keeping per pass the fastest.  Up to the number
  * the CUs hold at once the grid is persistent and the candidates differ in
  * latency hiding; beyond it the hardware hands later workgroups to CUs as
  * earlier ones finish, which balances the uneven tile-pair costs better than
@@ -1677,11 +1780,6 @@ static int autotune(Engine &E, int reps, int *chosen3)
     int nnz = 0;
     HIPCHK(hipMemcpyAsync(&nnz, E.d.nnz, sizeof(nnz), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
-    std::vector<int> cand;
-    for (int c = 0; c < nper; c++) cand.push_back(std::min(MAX_LAUNCH, std::max(1, E.ncu * per_cu[c])));
-    cand.push_back(std::min(MAX_LAUNCH, std::max(1, nnz)));
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     /* 1. derivative store or recomputation in pass 3, at the current launch
      * shapes: the store saves pass 3 the descreening arithmetic but costs
      * 8 (mixed) or 16 (FP64) bytes per pair written and read, which does not
@@ -1699,7 +1797,30 @@ static int autotune(Engine &E, int reps, int *chosen3)
             if (t_on <= t_off && (rc = set_store_mode(E, 1))) return rc;
         }
     }
-    /* 2. grid size per pass */
+    /* 2. step split per pass, for systems with few tile pairs (each at one
+     * workgroup per work item, the grid that suits short work lists) */
+    if (E.tp_cap < (long long)split_items_per_cu[0] * E.ncu) {
+        double tbest[3] = {1e300, 1e300, 1e300};
+        int spbest[3] = {1 << E.par.split_sh[0], 1 << E.par.split_sh[1], 1 << E.par.split_sh[2]};
+        for (int sh = 0; (1 << sh) <= SPLIT_MAX; sh++) {
+            const int sp3[3] = {1 << sh, 1 << sh, 1 << sh};
+            if ((rc = set_split(E, sp3))) return rc;
+            for (int k = 0; k < 3; k++) E.pass_blocks[k] = std::min(MAX_LAUNCH, std::max(1, nnz << sh));
+            if ((rc = time_evals(E, reps, ms))) return rc;
+            for (int k = 0; k < 3; k++)
+                if (ms[k] < tbest[k]) {
+                    tbest[k] = ms[k];
+                    spbest[k] = 1 << sh;
+                }
+        }
+        if ((rc = set_split(E, spbest))) return rc;
+    }
+    std::vector<int> cand;
+    for (int c = 0; c < nper; c++) cand.push_back(std::min(MAX_LAUNCH, std::max(1, E.ncu * per_cu[c])));
+    for (int k = 0; k < 3; k++) cand.push_back(std::min(MAX_LAUNCH, std::max(1, nnz << E.par.split_sh[k])));
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    /* 3. grid size per pass */
     double best[3] = {1e300, 1e300, 1e300};
     int pick[3] = {E.pass_blocks[0], E.pass_blocks[1], E.pass_blocks[2]};
     for (const int nb : cand) {
@@ -1764,6 +1885,19 @@ extern "C" int gbhip_get_launch(gbhip_ctx *ctx, int *blocks3)
 {
     if (!ctx || !blocks3) return seterr(GBHIP_EINVAL, "NULL argument");
     for (int k = 0; k < 3; k++) blocks3[k] = ctx->E.pass_blocks[k];
+    return 0;
+}
+
+extern "C" int gbhip_set_split(gbhip_ctx *ctx, const int *split3)
+{
+    if (!ctx || !split3) return seterr(GBHIP_EINVAL, "NULL argument");
+    return set_split(ctx->E, split3);
+}
+
+extern "C" int gbhip_get_split(gbhip_ctx *ctx, int *split3)
+{
+    if (!ctx || !split3) return seterr(GBHIP_EINVAL, "NULL argument");
+    for (int k = 0; k < 3; k++) split3[k] = 1 << ctx->E.par.split_sh[k];
     return 0;
 }
 
@@ -1916,11 +2050,42 @@ extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
     return 0;
 }
 
+/*
+ * Pinned host staging (allocated at first use): copies between pinned
+ * memory and the device are plain DMA transfers, where copies from pageable
+ * memory go through the runtime's bounce buffers (measured at 600 atoms:
+ * about 30 us per copy, most of a minimizer step's host time).
+ */
+int engine_stage(Engine &E)
+{
+    if (E.h_stage) return 0;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    void *p = nullptr;
+    if (hipHostMalloc(&p, sizeof(double) * (6 * (size_t)E.natoms + 14), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return seterr(GBHIP_ENOMEM, "hipHostMalloc of the staging buffer failed");
+    }
+    if (hipEventCreateWithFlags(&E.stage_ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(p);
+        E.stage_ev = nullptr;
+        return seterr(GBHIP_EHIP, "hipEventCreate failed");
+    }
+    E.h_stage = (double *)p;
+    return 0;
+}
+
 extern "C" int gbhip_set_coords(gbhip_ctx *ctx, const double *x)
 {
     if (!ctx || !x) return seterr(GBHIP_EINVAL, "NULL argument");
     Engine &E = ctx->E;
-    HIPCHK(hipMemcpyAsync(E.x_dev, x, 3 * sizeof(double) * (size_t)E.natoms, hipMemcpyHostToDevice, E.stream));
+    int rc = engine_stage(E);
+    if (rc) return rc;
+    const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
+    /* the previous copy out of the staging buffer has finished */
+    HIPCHK(hipEventSynchronize(E.stage_ev));
+    memcpy(E.h_stage, x, nb);
+    HIPCHK(hipMemcpyAsync(E.x_dev, E.h_stage, nb, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipEventRecord(E.stage_ev, E.stream));
     return 0;
 }
 

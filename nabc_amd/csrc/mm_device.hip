@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "engine.h"
@@ -499,15 +500,19 @@ extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
     /* per-pass timing events cannot sit inside a graph: a timing session
      * evaluates with plain launches */
     const bool graph = ctx->G.on && E.tev_cap == 0;
-    int rc = graph ? mme_graph_launch(ctx) : mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
+    int rc = engine_stage(E);
     if (rc) return rc;
-    double e[14];
-    if (grad)
-        HIPCHK(hipMemcpyAsync(grad, E.g_dev, 3 * sizeof(double) * (size_t)E.natoms, hipMemcpyDeviceToHost, E.stream));
-    HIPCHK(hipMemcpyAsync(e, E.ene14_dev, sizeof(e), hipMemcpyDeviceToHost, E.stream));
+    rc = graph ? mme_graph_launch(ctx) : mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
+    if (rc) return rc;
+    /* out through the pinned staging buffer: gradient after the coordinate
+     * part, energies last */
+    const size_t n3 = 3 * (size_t)E.natoms;
+    double *hg = E.h_stage + n3, *he = E.h_stage + 2 * n3;
+    if (grad) HIPCHK(hipMemcpyAsync(hg, E.g_dev, sizeof(double) * n3, hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipMemcpyAsync(he, E.ene14_dev, 14 * sizeof(double), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
-    if (ene14)
-        for (int k = 0; k < 14; k++) ene14[k] = e[k];
+    if (grad) memcpy(grad, hg, sizeof(double) * n3);
+    if (ene14) memcpy(ene14, he, 14 * sizeof(double));
     return 0;
 }
 

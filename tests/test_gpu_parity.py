@@ -278,3 +278,45 @@ def test_derivative_store_outgrown_by_async_rebuild(gpu, gen_dir):
     assert used2 and cap2 >= need2 == need1
     e = ctx.egb()
     assert rel(e[0], r[0]) <= FP64_E and rel_rms(e[3], r[3]) <= FP64_G
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_step_split_matches_oracle(gpu, gen_dir, precision):
+    """Every step split (each tile pair's steps in 1, 2, 4 or 8 work items),
+    per pass and mixed across passes, with the derivative store and without
+    it, gives the oracle's energies and gradient: the chunks' partial sums,
+    and the store offsets each chunk starts from, add up to the whole
+    sweep."""
+    na = gpu
+    parm, arr, x = generated_system(600, 12, gen_dir)
+    ctx, orc, _ = _compare(na, parm, arr, x, gb=8, cut=12.0, rgbmax=10.0, kappa=0.127315,
+                           precision=precision)
+    assert ctx.split() == (8, 8, 8)  # 10 tiles, 55 tile pairs: the default splits
+    r = orc.egb(x)
+    tol_e = FP64_E if precision == 0 else MIX_E
+    tol_g = FP64_G if precision == 0 else MIX_G
+    res = {}
+    combos = [(1, 1, 1), (2, 2, 2), (4, 4, 4), (8, 8, 8), (8, 1, 2), (1, 4, 8), (2, 8, 1)]
+    for store in (1, 0):
+        ctx.set_derivative_store(store)
+        ctx.nblist()
+        assert ctx.store_stats()[2] is bool(store)
+        for sp in combos:
+            ctx.set_split(*sp)
+            assert ctx.split() == sp
+            e = ctx.egb()
+            for c in range(3):
+                assert rel(e[c], r[c]) <= tol_e, (store, sp, c)
+            assert rel_rms(e[3], r[3]) <= tol_g, (store, sp)
+            res[(store, sp)] = e
+    # same arithmetic per pair, only the order of the sums differs
+    base = res[(1, (1, 1, 1))]
+    for key, e in res.items():
+        assert rel(e[0], base[0]) <= 1e-12, key
+        assert rel_rms(e[3], base[3]) <= (1e-12 if precision == 0 else 1e-6), key
+    with pytest.raises(na.GBHipError):
+        ctx.set_split(3, 1, 1)
+    ctx.set_split(0, 0, 0)
+    assert ctx.split() == (8, 8, 8)
+    ctx.close()
+    orc.close()
