@@ -140,11 +140,20 @@ struct Engine {
     void *store_mem = nullptr;
     int store_mode = 1;        /* 1: used when it fits, 0: always recompute */
     bool store_frozen = false; /* a captured graph holds the current pointer */
+    /* the host knows that the store holds the current list (sized for every
+     * tile pair full, or checked at the last synchronous list build), so
+     * pass 3's recomputing kernel, which would exit at once, is not launched */
+    bool store_holds_list = false;
+    /* set while a graph that rebuilds the list itself is captured: the
+     * recomputing pass-3 kernel is then left out only when the store holds
+     * every list the graph can build */
+    bool capture_spans_rebuilds = false;
     /* pinned host staging for coordinates in and gradient/energies out
      * (3*natoms + 3*natoms + 14 doubles); stage_ev marks the end of the
      * last copy out of the coordinate part */
     double *h_stage = nullptr;
     hipEvent_t stage_ev = nullptr;
+    bool stage_failed = false; /* no pinned memory: copy directly */
 
     ~Engine()
     {
@@ -257,7 +266,45 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
 int engine_gather(Engine &E, const double *x_dev);
 int engine_nblist(Engine &E);
 int engine_store_reserve(Engine &E, bool upper = false);
-int engine_eval(Engine &E, const double *x_dev, double *g_dev);
-int engine_stage(Engine &E);
+bool engine_store_holds_list(const Engine &E);
+/* engine_eval flags */
+#define EV_GATHERED 1 /* the slots already hold x_dev's coordinates, accumulators cleared */
+#define EV_NO_POST 2  /* the caller sums the energy slots and scatters the gradient itself */
+int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags = 0);
+
+/* ------------------------------------------- per-slot steps shared by the
+ * evaluation kernels and the fused kernels of the mme and dynamics paths */
+
+/* slot s: coordinates from AoS x (zero for padding slots) and the
+ * per-evaluation accumulators cleared */
+__device__ __forceinline__ void slot_gather(const DevPtrs &d, int s, const double *__restrict__ x)
+{
+    const int a = d.perm[s];
+    if (a >= 0) {
+        d.xs[s] = x[3 * (size_t)a];
+        d.ys[s] = x[3 * (size_t)a + 1];
+        d.zs[s] = x[3 * (size_t)a + 2];
+    } else {
+        d.xs[s] = 0.0; d.ys[s] = 0.0; d.zs[s] = 0.0;
+    }
+    d.sumi[s] = 0.0;
+    d.gx[s] = 0.0; d.gy[s] = 0.0; d.gz[s] = 0.0;
+}
+
+/* energy entry e (3 per molecule) = the sum of its ENE_SLOTS partials in
+ * slot order; the partials are cleared for the next evaluation (they start
+ * at zero from allocation) */
+__device__ __forceinline__ void ene_slot_sum(const DevPtrs &d, int e)
+{
+    double *p = d.eslot + (size_t)e * ENE_SLOTS;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < ENE_SLOTS; k++) {
+        acc += p[k];
+        p[k] = 0.0;
+    }
+    d.ene[e] = acc;
+}
+bool engine_stage(Engine &E);
 
 #endif

@@ -22,7 +22,8 @@
  *   k_chain    pass 3: sumdeijda * dR/dr chain-rule forces; the per-atom
  *              OBC factor is applied as the atoms are loaded, so the
  *              separate sumdeijda reduction of eff.c:2077-2129 disappears
- *   k_scatter  add the slot gradients into the caller's AoS array
+ *   k_post     sum the energy slots; add the slot gradients into the
+ *              caller's AoS array
  * Atoms are ordered per molecule by a k-d split into 64-atom leaf boxes.
  * Pair kernels: one 64-lane wavefront per tile pair; lane l holds atom l of
  * tile A, the non-empty columns (atoms of tile B with at least one partner)
@@ -449,17 +450,7 @@ __device__ __forceinline__ int entries_before(unsigned long long mr, int sb)
 __global__ __launch_bounds__(256) void k_gather(DevPtrs d, GBPar p, const double *__restrict__ x)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= p.nslots) return;
-    const int a = d.perm[s];
-    if (a >= 0) {
-        d.xs[s] = x[3 * (size_t)a];
-        d.ys[s] = x[3 * (size_t)a + 1];
-        d.zs[s] = x[3 * (size_t)a + 2];
-    } else {
-        d.xs[s] = 0.0; d.ys[s] = 0.0; d.zs[s] = 0.0;
-    }
-    d.sumi[s] = 0.0;
-    d.gx[s] = 0.0; d.gy[s] = 0.0; d.gz[s] = 0.0;
+    if (s < p.nslots) slot_gather(d, s, x);
 }
 
 template <typename R> __device__ __forceinline__ R ldri0(const DevPtrs &d, int s);
@@ -727,25 +718,6 @@ __device__ __forceinline__ void ene_flush(const DevPtrs &d, int mol, double epol
     unsafeAtomicAdd(e, epol);
     unsafeAtomicAdd(e + ENE_SLOTS, evdw);
     unsafeAtomicAdd(e + 2 * ENE_SLOTS, eel);
-}
-
-/*
- * ene[s] = sum of its ENE_SLOTS partials in slot order, which are cleared for
- * the next evaluation (they start at zero from allocation).  One thread per
- * entry, after pass 2.
- */
-__global__ __launch_bounds__(64) void k_enesum(DevPtrs d, int n)
-{
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= n) return;
-    double *e = d.eslot + (size_t)s * ENE_SLOTS;
-    double acc = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < ENE_SLOTS; k++) {
-        acc += e[k];
-        e[k] = 0.0;
-    }
-    d.ene[s] = acc;
 }
 
 template <typename R, bool SPL>
@@ -1163,10 +1135,16 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(DevPtrs d, GBPar p, double *__restrict__ g)
+/*
+ * After pass 3: thread e < nene sums energy entry e's slots (ene_slot_sum),
+ * and with a gradient array, thread s < nslots adds slot s's GB gradient
+ * into the gradient of its atom (original order, AoS).
+ */
+__global__ __launch_bounds__(256) void k_post(DevPtrs d, GBPar p, double *__restrict__ g, int nene)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= p.nslots) return;
+    if (s < nene) ene_slot_sum(d, s);
+    if (!g || s >= p.nslots) return;
     const int a = d.perm[s];
     if (a < 0) return;
     g[3 * (size_t)a] += d.gx[s];
@@ -1507,6 +1485,13 @@ int engine_gather(Engine &E, const double *x_dev)
     return 0;
 }
 
+/* the store is large enough for every tile pair full (k_qscan's test,
+ * need < cap, then holds for any list) */
+static bool store_covers_all(const Engine &E)
+{
+    return E.d.dstore && E.d.store_cap > E.tp_cap * TILE * TILE;
+}
+
 int engine_nblist(Engine &E)
 {
     const int NT = E.ntiles;
@@ -1527,7 +1512,19 @@ int engine_nblist(Engine &E)
     hipLaunchKernelGGL(k_steporder, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d);
     LAUNCHCHK();
     E.have_list = true;
+    /* a store sized for every tile pair full holds any list; otherwise the
+     * host does not know until a synchronous build checks (k_qscan has
+     * set the device flag either way) */
+    E.store_holds_list = store_covers_all(E);
     return 0;
+}
+
+/* the derivative store holds the current list, as the host knows it (and,
+ * while a graph that rebuilds the list is captured, every list) */
+bool engine_store_holds_list(const Engine &E)
+{
+    if (E.capture_spans_rebuilds) return E.store_mode == 1 && store_covers_all(E);
+    return E.store_mode == 1 && E.d.dstore && E.store_holds_list;
 }
 
 /*
@@ -1541,11 +1538,8 @@ int engine_nblist(Engine &E)
  * most a quarter of the free device memory (25k atoms: 2.5 GB mixed, 5 GB
  * FP64), else as above.
  */
-int engine_store_reserve(Engine &E, bool upper)
+static int store_reserve(Engine &E, bool upper, long long need)
 {
-    long long need = 0;
-    HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
-    HIPCHK(hipStreamSynchronize(E.stream));
     if (E.store_mode == 0 || E.store_frozen) return 0;
     const long long full = E.tp_cap * TILE * TILE + (long long)TILE * TILE;
     if (E.d.dstore && need < E.d.store_cap && (!upper || E.d.store_cap >= full)) return 0;
@@ -1588,6 +1582,18 @@ int engine_store_reserve(Engine &E, bool upper)
     return 0;
 }
 
+int engine_store_reserve(Engine &E, bool upper)
+{
+    long long need = 0;
+    HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    const int rc = store_reserve(E, upper, need);
+    /* the same test k_qscan (or k_setflag after a regrowth) applied on the
+     * device */
+    E.store_holds_list = E.d.dstore && need < E.d.store_cap;
+    return rc;
+}
+
 static int set_store_mode(Engine &E, int mode)
 {
     if (mode != 0 && mode != 1) return seterr(GBHIP_EINVAL, "derivative store mode must be 0 or 1");
@@ -1610,12 +1616,18 @@ static int set_store_mode(Engine &E, int mode)
     return 0;
 }
 
-/* one evaluation; coordinates already in E.x_dev or x_dev */
-int engine_eval(Engine &E, const double *x_dev, double *g_dev)
+/*
+ * One evaluation; coordinates in x_dev (AoS).  flags: EV_GATHERED when the
+ * caller's own kernel has done k_gather's work, EV_NO_POST when it does
+ * k_post's (the fused kernels of the mme and dynamics paths).
+ */
+int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
 {
     if (!E.have_list) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
-    int rc = engine_gather(E, x_dev);
-    if (rc) return rc;
+    if (!(flags & EV_GATHERED)) {
+        int rc = engine_gather(E, x_dev);
+        if (rc) return rc;
+    }
     const int nb64 = (E.nslots + 63) / 64;
     hipEvent_t *ev = nullptr;
     if (E.tev_used + 5 <= E.tev_cap) {
@@ -1639,22 +1651,29 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev)
     else
         hipLaunchKernelGGL((sp2 ? k_pairs<float, true> : k_pairs<float, false>), dim3(E.pass_blocks[1]), dim3(64), 0,
                            E.stream, E.d, E.par);
-    hipLaunchKernelGGL(k_enesum, dim3((3 * E.nmol + 63) / 64), dim3(64), 0, E.stream, E.d, 3 * E.nmol);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
+    /* k_chain_st and k_chain each read the device flag and one of them
+     * works; when the host knows the store holds the list, k_chain (which
+     * would exit at once) is not launched */
+    const bool recompute = !engine_store_holds_list(E);
     if (E.precision == GBHIP_FP64) {
         hipLaunchKernelGGL((sp3 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL((sp3 ? k_chain<double, true> : k_chain<double, false>), dim3(E.pass_blocks[2]), dim3(64),
-                           0, E.stream, E.d, E.par);
+        if (recompute)
+            hipLaunchKernelGGL((sp3 ? k_chain<double, true> : k_chain<double, false>), dim3(E.pass_blocks[2]),
+                               dim3(64), 0, E.stream, E.d, E.par);
     } else {
         hipLaunchKernelGGL((sp3 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
-        hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]), dim3(64), 0,
-                           E.stream, E.d, E.par);
+        if (recompute)
+            hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]),
+                               dim3(64), 0, E.stream, E.d, E.par);
     }
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
-    if (g_dev)
-        hipLaunchKernelGGL(k_scatter, dim3((E.nslots + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev);
+    if (!(flags & EV_NO_POST)) {
+        const int nene = 3 * E.nmol, nt = std::max(nene, g_dev ? E.nslots : 0);
+        hipLaunchKernelGGL(k_post, dim3((nt + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev, nene);
+    }
     LAUNCHCHK();
     E.have_eval = true;
     return 0;
@@ -2054,33 +2073,41 @@ extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
  * Pinned host staging (allocated at first use): copies between pinned
  * memory and the device are plain DMA transfers, where copies from pageable
  * memory go through the runtime's bounce buffers (measured at 600 atoms:
- * about 30 us per copy, most of a minimizer step's host time).
+ * about 30 us per copy, most of a minimizer step's host time).  Returns
+ * false when pinned memory cannot be had; callers then copy directly.
  */
-int engine_stage(Engine &E)
+bool engine_stage(Engine &E)
 {
-    if (E.h_stage) return 0;
+    if (E.h_stage) return true;
+    if (E.stage_failed) return false;
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     void *p = nullptr;
+    hipEvent_t ev = nullptr;
     if (hipHostMalloc(&p, sizeof(double) * (6 * (size_t)E.natoms + 14), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
-        return seterr(GBHIP_ENOMEM, "hipHostMalloc of the staging buffer failed");
+        E.stage_failed = true;
+        return false;
     }
-    if (hipEventCreateWithFlags(&E.stage_ev, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
         (void)hipHostFree(p);
-        E.stage_ev = nullptr;
-        return seterr(GBHIP_EHIP, "hipEventCreate failed");
+        E.stage_failed = true;
+        return false;
     }
     E.h_stage = (double *)p;
-    return 0;
+    E.stage_ev = ev;
+    return true;
 }
 
 extern "C" int gbhip_set_coords(gbhip_ctx *ctx, const double *x)
 {
     if (!ctx || !x) return seterr(GBHIP_EINVAL, "NULL argument");
     Engine &E = ctx->E;
-    int rc = engine_stage(E);
-    if (rc) return rc;
     const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
+    if (!engine_stage(E)) {
+        HIPCHK(hipMemcpyAsync(E.x_dev, x, nb, hipMemcpyHostToDevice, E.stream));
+        return 0;
+    }
     /* the previous copy out of the staging buffer has finished */
     HIPCHK(hipEventSynchronize(E.stage_ev));
     memcpy(E.h_stage, x, nb);

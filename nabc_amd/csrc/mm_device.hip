@@ -258,57 +258,6 @@ __global__ __launch_bounds__(MM_THREADS) void k_bonded(BondedDev B, const double
     d_nb14(B, x, g, b);
 }
 
-/* gradient and the 14 energy slots cleared in one launch */
-__global__ __launch_bounds__(MM_THREADS) void k_zero_mme(double *g, long long n3, double *ene)
-{
-    const long long k = (long long)blockIdx.x * MM_THREADS + threadIdx.x;
-    if (k < n3) g[k] = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x < 14) ene[threadIdx.x] = 0.0;
-}
-
-/* zero frozen gradients; per-block sums of squared gradient components,
- * written per block so the total is summed in a fixed order */
-__global__ __launch_bounds__(MM_THREADS) void k_gnorm(BondedDev B, double *g, int natom, double *part)
-{
-    __shared__ double s_red[MM_THREADS / 64];
-    double s = 0.0;
-    for (int a = blockIdx.x * MM_THREADS + threadIdx.x; a < natom; a += gridDim.x * MM_THREADS) {
-        if (B.frozen[a]) {
-            g[3 * a] = 0.0; g[3 * a + 1] = 0.0; g[3 * a + 2] = 0.0;
-        } else {
-            s += g[3 * a] * g[3 * a] + g[3 * a + 1] * g[3 * a + 1] + g[3 * a + 2] * g[3 * a + 2];
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int k = 0; k < MM_THREADS / 64; k++) t += s_red[k];
-        part[blockIdx.x] = t;
-    }
-}
-
-/* copy the GB terms in, total energy and rms gradient (one wavefront) */
-__global__ __launch_bounds__(64) void k_finish(BondedDev B, const double *gbene, const double *part, int nparts,
-                                              int natom, double *ene_out)
-{
-    double s = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += 64) s += part[k];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (threadIdx.x == 0) {
-        double e[13];
-        for (int k = 0; k < 13; k++) e[k] = B.ene[k];
-        e[10] = gbene[0];
-        e[1] = gbene[1];
-        e[2] = gbene[2];
-        e[0] = 0.0;
-        for (int k = 1; k <= 12; k++) e[0] += e[k];
-        for (int k = 0; k < 13; k++) ene_out[k] = e[k];
-        ene_out[13] = sqrt(s / (3.0 * natom));
-    }
-}
-
 /* ---------------------------------------------------------------- C-ABI */
 
 extern "C" int gbhip_set_bonded(gbhip_ctx *ctx, const gbhip_bonded *b)
@@ -370,27 +319,14 @@ extern "C" int gbhip_set_bonded(gbhip_ctx *ctx, const gbhip_bonded *b)
 
 static int grid_of(long long n) { return (int)((n + MM_THREADS - 1) / MM_THREADS); }
 
+struct MDPar;
+static int mme_launch(gbhip_ctx *ctx, double *x_dev, double *d_grad, double *d_ene14, const MDPar *md, bool rebuild);
+
 /* full mme34 energy + gradient on the device; d_grad overwritten */
 int mme_device(gbhip_ctx *ctx, const double *x_dev, double *d_grad, double *d_ene14)
 {
-    Engine &E = ctx->E;
-    BondedDev &B = ctx->B;
-    if (!B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
-    const long long n3 = 3LL * E.natoms;
-    hipLaunchKernelGGL(k_zero_mme, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, d_grad, n3, B.ene);
-    const int4 nblk = make_int4(grid_of(B.nb), grid_of(B.na), grid_of(B.nd), grid_of(B.n14));
-    const int nbonded = nblk.x + nblk.y + nblk.z + nblk.w;
-    if (nbonded)
-        hipLaunchKernelGGL(k_bonded, dim3(nbonded), dim3(MM_THREADS), 0, E.stream, B, x_dev, d_grad, nblk);
-    LAUNCHCHK();
-    int rc = engine_eval(E, x_dev, d_grad);
-    if (rc) return rc;
-    const int nparts = std::min(GNORM_PARTS, std::max(1, grid_of(E.natoms)));
-    hipLaunchKernelGGL(k_gnorm, dim3(nparts), dim3(MM_THREADS), 0, E.stream, B, d_grad, E.natoms, B.gpart);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, E.stream, B, (const double *)E.d.ene,
-                       (const double *)B.gpart, nparts, E.natoms, d_ene14);
-    LAUNCHCHK();
-    return 0;
+    /* x_dev is only read without a dynamics step */
+    return mme_launch(ctx, const_cast<double *>(x_dev), d_grad, d_ene14, nullptr, false);
 }
 
 extern "C" int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14)
@@ -422,6 +358,8 @@ static std::vector<unsigned char> mme_key(const gbhip_ctx *c)
     put(&E.g_dev, sizeof(E.g_dev));
     put(&E.ene14_dev, sizeof(E.ene14_dev));
     put(&c->B, sizeof(c->B));
+    const bool held = engine_store_holds_list(E); /* decides whether pass 3's recompute kernel is launched */
+    put(&held, sizeof(held));
     return k;
 }
 
@@ -500,18 +438,18 @@ extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
     /* per-pass timing events cannot sit inside a graph: a timing session
      * evaluates with plain launches */
     const bool graph = ctx->G.on && E.tev_cap == 0;
-    int rc = engine_stage(E);
+    int rc = graph ? mme_graph_launch(ctx) : mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
     if (rc) return rc;
-    rc = graph ? mme_graph_launch(ctx) : mme_device(ctx, E.x_dev, E.g_dev, E.ene14_dev);
-    if (rc) return rc;
-    /* out through the pinned staging buffer: gradient after the coordinate
-     * part, energies last */
+    /* out through the pinned staging buffer (gradient after the coordinate
+     * part, energies last), or directly when there is none */
     const size_t n3 = 3 * (size_t)E.natoms;
-    double *hg = E.h_stage + n3, *he = E.h_stage + 2 * n3;
+    double e14[14];
+    const bool staged = engine_stage(E);
+    double *hg = staged ? E.h_stage + n3 : grad, *he = staged ? E.h_stage + 2 * n3 : e14;
     if (grad) HIPCHK(hipMemcpyAsync(hg, E.g_dev, sizeof(double) * n3, hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipMemcpyAsync(he, E.ene14_dev, 14 * sizeof(double), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
-    if (grad) memcpy(grad, hg, sizeof(double) * n3);
+    if (grad && staged) memcpy(grad, hg, sizeof(double) * n3);
     if (ene14) memcpy(ene14, he, 14 * sizeof(double));
     return 0;
 }
@@ -522,9 +460,10 @@ extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
  * Device-resident md() (sff.c:2071-2366): velocity Verlet with the
  * Langevin (gamma_ln > 0) or Berendsen (gamma_ln = 0) scheme of the
  * reference, vlimit clamp, frozen atoms held.  One step is
- *   k_md_drift -> [pair-list rebuild] -> mme (bonded + 1-4 + GB) ->
- *   k_md_kick -> k_md_finish
- * with no host synchronisation; the two variants (with / without the
+ *   drift -> [pair-list rebuild] -> mme (bonded + 1-4 + GB) -> kick ->
+ *   kinetic energy and log row
+ * (the drift, kick and bookkeeping fused into the evaluation's first and
+ * last kernels, see mme_launch) with no host synchronisation; the two variants (with / without the
  * rebuild) are captured once as hipGraphs and replayed.  Noise: one N(0,1)
  * per (step, coordinate) from a counter-based hash (splitmix64 mixing of
  * seed, step and index, then Box-Muller), the same function as the host
@@ -609,54 +548,180 @@ __global__ __launch_bounds__(MM_THREADS) void k_md_accel0(MDState M, MDPar P, co
     }
 }
 
-/* half kick + drift (sff.c:2225-2249) */
-__global__ __launch_bounds__(MM_THREADS) void k_md_drift(MDState M, MDPar P, double *x)
-{
-    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
-    if (i == 0) M.scal[1] = 0.0;
-    if (i >= P.n3) return;
-    double v;
-    if (P.gammai == 0.0) v = (M.v[i] + M.accel[i]) * md_tscal(P, M.scal);
-    else v = P.c_explic * M.v[i] + M.accel[i];
-    v = fmin(P.vlimit, fmax(-P.vlimit, v));
-    M.v[i] = v;
-    x[i] += v * P.dtx;
-}
+/* ------------------------------------------------- fused evaluation kernels
+ * One mme evaluation is k_mme_pre -> k_bonded -> the GB passes ->
+ * k_mme_post -> k_mme_finish; in a dynamics step the drift is done by
+ * k_mme_pre, the second half kick by k_mme_post and the step's bookkeeping
+ * by k_mme_finish (MD = true), so a step is as many launches as an
+ * evaluation.  Each launch costs a few microseconds however small its
+ * work, which is most of a step below a few thousand atoms.
+ */
 
-/* second half kick with the new forces (sff.c:2262-2277) and 2*ekin */
-__global__ __launch_bounds__(MM_THREADS) void k_md_kick(MDState M, MDPar P, const int *frz, const double *g)
+/* per slot: [dynamics: drift of its atom (sff.c:2225-2249)], its gradient
+ * cleared, its coordinates gathered (slot_gather); the bonded energies and
+ * [dynamics] the kinetic-energy accumulator cleared */
+template <bool MD>
+__global__ __launch_bounds__(MM_THREADS) void k_mme_pre(DevPtrs d, GBPar p, double *x, double *g, double *bene,
+                                                        MDState M, MDPar P)
 {
-    const int i = blockIdx.x * MM_THREADS + threadIdx.x;
-    double e = 0.0;
-    if (i < P.n3) {
-        double v = M.v[i];
-        if (P.gammai == 0.0) {
-            const double a = -g[i] * M.minv[i] * P.dt5;
-            M.accel[i] = a;
-            v = (v + a) * md_tscal(P, M.scal);
-        } else if (!frz[i / 3]) {
-            const double a = (-g[i] + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, *M.step, i)) * M.minv[i] * P.dt5;
-            M.accel[i] = a;
-            v = (v + a) * P.c_implic;
+    const int s = blockIdx.x * MM_THREADS + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 14) bene[threadIdx.x] = 0.0;
+    if (MD && s == 0) M.scal[1] = 0.0;
+    if (s >= p.nslots) return;
+    const int a = d.perm[s];
+    if (a >= 0) {
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * a + c;
+            if (MD) {
+                double v;
+                if (P.gammai == 0.0) v = (M.v[i] + M.accel[i]) * md_tscal(P, M.scal);
+                else v = P.c_explic * M.v[i] + M.accel[i];
+                v = fmin(P.vlimit, fmax(-P.vlimit, v));
+                M.v[i] = v;
+                x[i] += v * P.dtx;
+            }
+            g[i] = 0.0;
         }
-        M.v[i] = v;
-        e = v * v / M.minv[i];
     }
-    block_add(e, &M.scal[1]);
+    slot_gather(d, s, x);
 }
 
-/* kinetic energy, temperature, per-step log row, step counter (sff.c:2294-2339) */
-__global__ void k_md_finish(MDState M, MDPar P, const double *ene14)
+/*
+ * Grid-stride over slots: the energy-slot sums (ene_slot_sum), the slot's
+ * GB gradient added into its atom's, frozen gradients zeroed (eff.c:
+ * 2826-2834), per-block sums of squared gradient components (written per
+ * block, so the rms is summed in a fixed order), and [dynamics] the second
+ * half kick with the new forces and 2*ekin (sff.c:2262-2277).
+ */
+template <bool MD>
+__global__ __launch_bounds__(MM_THREADS) void k_mme_post(DevPtrs d, GBPar p, BondedDev B, double *g, int nene,
+                                                         MDState M, MDPar P)
 {
-    double ekin = 0.5 * M.scal[1];
-    if (P.gammai > 0.0) ekin *= P.c_ave;
-    const int s = *M.step;
-    double *row = M.log + 16 * (size_t)((s - 1) % P.log_cap);
-    for (int k = 0; k < 14; k++) row[k] = ene14[k];
-    row[14] = ekin;
-    row[15] = ekin / (P.boltz2 * P.rndf);
-    M.scal[0] = ekin;
-    *M.step = s + 1;
+    __shared__ double s_red[MM_THREADS / 64];
+    const int nt = max(p.nslots, nene);
+    double sq = 0.0, ek = 0.0;
+    for (int s = blockIdx.x * MM_THREADS + threadIdx.x; s < nt; s += gridDim.x * MM_THREADS) {
+        if (s < nene) ene_slot_sum(d, s);
+        if (s >= p.nslots) continue;
+        const int a = d.perm[s];
+        if (a < 0) continue;
+        double *ga = g + 3 * (size_t)a;
+        double gv[3] = {ga[0] + d.gx[s], ga[1] + d.gy[s], ga[2] + d.gz[s]};
+        const bool frz = B.frozen[a] != 0;
+        if (frz) gv[0] = gv[1] = gv[2] = 0.0;
+        else sq += gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+        ga[0] = gv[0]; ga[1] = gv[1]; ga[2] = gv[2];
+        if (MD) {
+            for (int c = 0; c < 3; c++) {
+                const int i = 3 * a + c;
+                double v = M.v[i];
+                if (P.gammai == 0.0) {
+                    const double acc = -gv[c] * M.minv[i] * P.dt5;
+                    M.accel[i] = acc;
+                    v = (v + acc) * md_tscal(P, M.scal);
+                } else if (!frz) {
+                    const double acc = (-gv[c] + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, *M.step, i)) * M.minv[i] * P.dt5;
+                    M.accel[i] = acc;
+                    v = (v + acc) * P.c_implic;
+                }
+                M.v[i] = v;
+                ek += v * v / M.minv[i];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < MM_THREADS / 64; k++) t += s_red[k];
+        B.gpart[blockIdx.x] = t;
+    }
+    if (MD) block_add(ek, &M.scal[1]);
+}
+
+/* the GB terms copied in, the total energy and the rms gradient (one
+ * wavefront); [dynamics] kinetic energy, temperature, the per-step log row
+ * and the step counter (sff.c:2294-2339) */
+template <bool MD>
+__global__ __launch_bounds__(64) void k_mme_finish(BondedDev B, const double *gbene, int nparts, int natom,
+                                                   double *ene_out, MDState M, MDPar P)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += 64) s += B.gpart[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (threadIdx.x != 0) return;
+    double e[14];
+    for (int k = 0; k < 13; k++) e[k] = B.ene[k];
+    e[10] = gbene[0];
+    e[1] = gbene[1];
+    e[2] = gbene[2];
+    e[0] = 0.0;
+    for (int k = 1; k <= 12; k++) e[0] += e[k];
+    e[13] = sqrt(s / (3.0 * natom));
+    for (int k = 0; k < 14; k++) ene_out[k] = e[k];
+    if (MD) {
+        double ekin = 0.5 * M.scal[1];
+        if (P.gammai > 0.0) ekin *= P.c_ave;
+        const int st = *M.step;
+        double *row = M.log + 16 * (size_t)((st - 1) % P.log_cap);
+        for (int k = 0; k < 14; k++) row[k] = e[k];
+        row[14] = ekin;
+        row[15] = ekin / (P.boltz2 * P.rndf);
+        M.scal[0] = ekin;
+        *M.step = st + 1;
+    }
+}
+
+/* grid of k_mme_post (its partial sums fit the GNORM_PARTS entries) */
+static int post_blocks(const Engine &E)
+{
+    return std::min(GNORM_PARTS, std::max(1, grid_of(std::max(E.nslots, 3 * E.nmol))));
+}
+
+/*
+ * The launches of one evaluation on the engine stream (also under
+ * capture).  md: a dynamics step around it (drift before, kick and
+ * bookkeeping after); rebuild: the pair list is rebuilt from the drifted
+ * coordinates first.
+ */
+static int mme_launch(gbhip_ctx *ctx, double *x_dev, double *d_grad, double *d_ene14, const MDPar *md, bool rebuild)
+{
+    Engine &E = ctx->E;
+    BondedDev &B = ctx->B;
+    if (!B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
+    if (!E.have_list && !rebuild) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
+    const MDPar P = md ? *md : MDPar{};
+    const int npre = std::max(1, grid_of(E.nslots)), npost = post_blocks(E);
+    if (md)
+        hipLaunchKernelGGL(k_mme_pre<true>, dim3(npre), dim3(MM_THREADS), 0, E.stream, E.d, E.par, x_dev, d_grad,
+                           B.ene, ctx->M, P);
+    else
+        hipLaunchKernelGGL(k_mme_pre<false>, dim3(npre), dim3(MM_THREADS), 0, E.stream, E.d, E.par, x_dev, d_grad,
+                           B.ene, ctx->M, P);
+    LAUNCHCHK();
+    int rc;
+    if (rebuild && (rc = engine_nblist(E))) return rc;
+    const int4 nblk = make_int4(grid_of(B.nb), grid_of(B.na), grid_of(B.nd), grid_of(B.n14));
+    const int nbonded = nblk.x + nblk.y + nblk.z + nblk.w;
+    if (nbonded)
+        hipLaunchKernelGGL(k_bonded, dim3(nbonded), dim3(MM_THREADS), 0, E.stream, B, (const double *)x_dev, d_grad,
+                           nblk);
+    LAUNCHCHK();
+    if ((rc = engine_eval(E, x_dev, d_grad, EV_GATHERED | EV_NO_POST))) return rc;
+    if (md) {
+        hipLaunchKernelGGL(k_mme_post<true>, dim3(npost), dim3(MM_THREADS), 0, E.stream, E.d, E.par, B, d_grad,
+                           3 * E.nmol, ctx->M, P);
+        hipLaunchKernelGGL(k_mme_finish<true>, dim3(1), dim3(64), 0, E.stream, B, (const double *)E.d.ene, npost,
+                           E.natoms, d_ene14, ctx->M, P);
+    } else {
+        hipLaunchKernelGGL(k_mme_post<false>, dim3(npost), dim3(MM_THREADS), 0, E.stream, E.d, E.par, B, d_grad,
+                           3 * E.nmol, ctx->M, P);
+        hipLaunchKernelGGL(k_mme_finish<false>, dim3(1), dim3(64), 0, E.stream, B, (const double *)E.d.ene, npost,
+                           E.natoms, d_ene14, ctx->M, P);
+    }
+    LAUNCHCHK();
+    return 0;
 }
 
 static MDPar md_par(const MDState &M, int n3)
@@ -675,16 +740,7 @@ static int md_step_launch(gbhip_ctx *c, bool rebuild)
     MDState &M = c->M;
     const int n3 = 3 * E.natoms;
     const MDPar P = md_par(M, n3);
-    hipLaunchKernelGGL(k_md_drift, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, E.x_dev);
-    LAUNCHCHK();
-    int rc;
-    if (rebuild && (rc = engine_nblist(E))) return rc;
-    if ((rc = mme_device(c, E.x_dev, E.g_dev, E.ene14_dev))) return rc;
-    hipLaunchKernelGGL(k_md_kick, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M, P, c->B.frozen,
-                       (const double *)E.g_dev);
-    hipLaunchKernelGGL(k_md_finish, dim3(1), dim3(1), 0, E.stream, M, P, (const double *)E.ene14_dev);
-    LAUNCHCHK();
-    return 0;
+    return mme_launch(c, E.x_dev, E.g_dev, E.ene14_dev, &P, rebuild);
 }
 
 static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t *ge)
@@ -693,10 +749,13 @@ static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t 
     const int saved_cap = E.tev_cap;
     E.tev_cap = 0; /* no timing events inside a graph */
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
-    /* the graph keeps the derivative-store pointer it was captured with */
+    /* the graph keeps the derivative-store pointer it was captured with;
+     * the steps replay across list rebuilds */
     E.store_frozen = true;
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    E.capture_spans_rebuilds = true;
     int rc = md_step_launch(c, rebuild);
+    E.capture_spans_rebuilds = false;
     hipError_t e2 = hipStreamEndCapture(E.stream, g);
     E.tev_cap = saved_cap;
     if (rc) return rc;
