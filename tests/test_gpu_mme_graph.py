@@ -104,3 +104,40 @@ def test_graph_recaptures_on_launch_and_store_change(gpu, gen_dir):
             assert rel_rms(f1, f2) < 1e-11
     it = na.C.c_int(-3)
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_graph_after_async_rebuild_outgrows_store(gpu, gen_dir):
+    """An asynchronous rebuild whose list no longer fits the derivative
+    store leaves pass 3 to the recomputing kernel: the captured evaluation
+    is re-captured with that kernel in it, and its gradient equals the one
+    after a synchronous rebuild has grown the store again (FP64, same
+    list)."""
+    na = gpu
+    parm, arr, x0 = generated_system(2500, 5, gen_dir)
+    _init(na, parm, "gb=5, cut=10.0, rgbmax=10.0, kappa=0.1, nsnb=100000, ntpr=100000")
+    ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+    assert ctx
+    f = np.zeros_like(x0)
+    it = na.C.c_int(1)
+    assert np.isfinite(na.lib.mme(x0.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it)))
+    assert _stats(na, ctx)[0] == 1
+    need, cap, used = na.C.c_longlong(), na.C.c_longlong(), na.C.c_int()
+    xc = np.ascontiguousarray(x0 * 0.8)  # denser: about twice the pairs
+    assert na.lib.gbhip_set_coords(ctx, xc.ctypes.data_as(na.dp)) == 0
+    assert na.lib.gbhip_nblist_async(ctx) == 0
+    assert na.lib.gbhip_store_stats(ctx, na.C.byref(need), na.C.byref(cap), na.C.byref(used)) == 0
+    assert need.value > cap.value and not used.value
+    g1, e1 = np.zeros_like(x0), np.zeros(14)
+    assert na.lib.gbhip_mme(ctx, g1.ctypes.data_as(na.dp), e1.ctypes.data_as(na.dp)) == 0
+    assert _stats(na, ctx) == (2, 2)
+    np_ = na.C.c_longlong()
+    assert na.lib.gbhip_nblist(ctx, na.C.byref(np_)) == 0
+    assert na.lib.gbhip_store_stats(ctx, na.C.byref(need), na.C.byref(cap), na.C.byref(used)) == 0
+    assert used.value
+    g2, e2 = np.zeros_like(x0), np.zeros(14)
+    assert na.lib.gbhip_mme(ctx, g2.ctypes.data_as(na.dp), e2.ctypes.data_as(na.dp)) == 0
+    assert _stats(na, ctx) == (3, 3)
+    assert np.max(np.abs(e1 - e2)) <= 1e-9 * np.abs(e2).max()
+    assert rel_rms(g1, g2) < 1e-9
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
