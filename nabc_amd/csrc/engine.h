@@ -66,7 +66,7 @@ struct DevPtrs {
     double *xs, *ys, *zs;
     double *sumi, *reff, *psi, *dfac, *sumdeijda, *gx, *gy, *gz;
     double *ene;          /* 3 per molecule: epol, evdw, eelec */
-    double *eslot;        /* ENE_SLOTS partial sums per ene entry (k_enesum adds them up) */
+    double *eslot;        /* ENE_SLOTS partial sums per ene entry (ene_slot_sum adds them up) */
     /* tiles and tile pairs */
     int ntiles;
     const int *tile_mol;

@@ -13,8 +13,8 @@
  * gradient in original atom order; energies reduced per block.
  * The assembled energy vector follows mme34's ene[] (eff.c:2617-2874);
  * frozen atoms get a zero gradient (eff.c:2826-2834); the rms gradient is
- * reduced over a grid of blocks, then summed in a fixed order (k_gnorm,
- * k_finish).
+ * reduced over a grid of blocks, then summed in a fixed order (k_mme_post,
+ * k_mme_finish).
  */
 #include <hip/hip_runtime.h>
 
