@@ -151,3 +151,37 @@ def test_set_stream_after_dynamics_keeps_callers_stream(gpu, gen_dir):
         t = torch.ones(1024, device="cuda:0") * 2.0
     s.synchronize()
     assert float(t.sum()) == 2048.0
+
+
+def test_device_md_frozen_langevin_follows_host_loop(gpu, gen_dir):
+    """Frozen atoms under Langevin dynamics: the device step (drift in the
+    evaluation's first kernel, kick in its gradient kernel) skips their kick
+    and zeroes their gradient as the host loop of md() does, with the same
+    counter-based noise, so the two trajectories agree to summation-order
+    noise and the frozen atoms stay where they were."""
+    na = gpu
+    parm, arr, x0 = generated_system(600, 27, gen_dir)
+    natom = arr["natom"]
+    n = 3 * natom
+    frozen = np.zeros(natom, dtype=np.int32)
+    frozen[::7] = 1
+    fz3 = np.repeat(frozen, 3).astype(bool)
+    rng = np.random.default_rng(4)
+    v0 = rng.normal(0, 0.05, n)
+    v0[fz3] = 0.0
+    opts = ("gb=5, cut=12.0, rgbmax=12.0, nsnb=5, gamma_ln=5.0, temp0=300.0, tempi=0.0, dt=0.001, "
+            "md_seed=9, ntpr_md=1000, ntpr=100000")
+    out = []
+    for host in (False, True):
+        na.lib.mm_reset_options()
+        assert na.lib.mm_options(opts.encode()) == 0, na.lib.mm_get_error()
+        assert na.lib.mme_init_sff(parm, frozen.ctypes.data_as(na.ip), None, None, None) == 0, na.lib.mm_get_error()
+        x, v = x0.copy(), v0.copy()
+        wrap = na.MMEFUNC(lambda xp, fp, itp: na.lib.mme(xp, fp, itp)) if host else None
+        _md(na, n, 25, x, v, func=wrap)
+        out.append((x, v))
+    (xa, va), (xb, vb) = out
+    assert np.max(np.abs(xa - xb)) < 1e-8
+    assert rel_rms(va, vb) < 1e-8
+    assert np.array_equal(xa[fz3], x0[fz3]) and np.all(va[fz3] == 0.0)
+    assert np.max(np.abs(xa[~fz3] - x0[~fz3])) > 1e-3  # the others moved
