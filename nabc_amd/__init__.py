@@ -513,7 +513,7 @@ class GBContext:
         _check(lib.gbhip_set_derivative_store(self.h, mode))
 
     def set_split(self, born=0, pairs=0, chain=0):
-        """Step chunks per tile pair for the three pair passes (1, 2, 4, 8;
+        """Step chunks per tile pair for the three pair passes (1, 2, 4, 8, 16;
         0 = the default for the system's size); resets the launch shapes to
         their defaults."""
         b = np.array([born, pairs, chain], np.int32)

@@ -55,7 +55,8 @@
 #define MAX_LAUNCH (64 * 1024) /* workgroups per pair-kernel launch */
 #define LJ_LDS 256 /* LJ table entries pass 2 keeps in LDS */
 #define CHAIN_DEPTH 8  /* derivative-store loads in flight per lane in pass 3 */
-#define SPLIT_MAX 8    /* step chunks per tile pair, at most */
+#define SPLIT_MAX 16   /* step chunks per tile pair, at most (a tile pair has at most 64 steps) */
+#define SPLIT_DEFAULT_MAX 8 /* the most the default split uses */
 
 /* ------------------------------------------------------------------ errors */
 
@@ -1269,19 +1270,21 @@ static int check_topology(const gbhip_topology *t, int gb)
  * Step split for a system with few tile pairs: one work item is one
  * wavefront's sweep, so a system with fewer tile pairs than the chip has
  * SIMDs leaves most of them idle, and a pass cannot end before its longest
- * sweep.  Each tile pair's steps are dealt out in up to SPLIT_MAX chunks
- * until a pass has split_items_per_cu work items per CU (at most as many as
- * tile pairs allow): 96 for pass 1, whose sweeps are the longest, 4 for the
- * others (measured, gb 8, 14 A cutoff: pass 1 gains from splitting up to
- * 12k atoms, passes 2 and 3 lose beyond 2,500 atoms).
+ * sweep.  Each tile pair's steps are dealt out in up to SPLIT_DEFAULT_MAX
+ * chunks until a pass has split_items_per_cu work items per CU, counted
+ * from the tile-pair capacity: 64 for pass 1, whose sweeps are the longest,
+ * 4 for the others.  Measured (HIP events, mixed): 600 atoms split 8 in
+ * every pass; 7,000 atoms at 20 A pass 1 44 us at split 4 against 62 at 1
+ * and 57 at 8, passes 2 and 3 best unsplit from a few thousand atoms on
+ * (more per-visit set-up and flush atomics).
  */
-static const int split_items_per_cu[3] = {96, 4, 4};
+static const int split_items_per_cu[3] = {64, 4, 4};
 
 static int default_split(const Engine &E, int pass)
 {
     int sh = 0;
     const long long cap = std::max<long long>(1, E.tp_cap);
-    while ((1 << sh) < SPLIT_MAX && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
+    while ((1 << sh) < SPLIT_DEFAULT_MAX && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
     return sh;
 }
 
@@ -1723,7 +1726,7 @@ static int timing_stop(Engine &E, double *ms4, int *nevals)
 
 /* ------------------------------------------------------------ launch shapes */
 
-/* per pass 1, 2, 4 or 8 step chunks per tile pair (0: the default for the
+/* per pass 1, 2, 4, 8 or 16 step chunks per tile pair (0: the default for the
  * system's size); the grids return to their defaults for the new split */
 static int set_split(Engine &E, const int *split3)
 {
@@ -1733,9 +1736,9 @@ static int set_split(Engine &E, const int *split3)
             sh[k] = default_split(E, k);
         } else {
             sh[k] = 0;
-            while (sh[k] < 4 && (1 << sh[k]) != split3[k]) sh[k]++;
-            if (sh[k] == 4 || split3[k] > SPLIT_MAX)
-                return seterr(GBHIP_EINVAL, "step split must be 0, 1, 2, 4 or 8");
+            while ((1 << sh[k]) < split3[k] && (1 << sh[k]) < SPLIT_MAX) sh[k]++;
+            if (split3[k] < 0 || (1 << sh[k]) != split3[k])
+                return seterr(GBHIP_EINVAL, "step split must be 0 or a power of two up to %d", SPLIT_MAX);
         }
         if ((std::max<long long>(1, E.tp_cap) << sh[k]) > (1LL << 30))
             return seterr(GBHIP_EINVAL, "too many tile pairs for a step split of %d", 1 << sh[k]);

@@ -282,7 +282,7 @@ def test_derivative_store_outgrown_by_async_rebuild(gpu, gen_dir):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_step_split_matches_oracle(gpu, gen_dir, precision):
-    """Every step split (each tile pair's steps in 1, 2, 4 or 8 work items),
+    """Every step split (each tile pair's steps in 1, 2, 4, 8 or 16 work items),
     per pass and mixed across passes, with the derivative store and without
     it, gives the oracle's energies and gradient: the chunks' partial sums,
     and the store offsets each chunk starts from, add up to the whole
@@ -296,7 +296,7 @@ def test_step_split_matches_oracle(gpu, gen_dir, precision):
     tol_e = FP64_E if precision == 0 else MIX_E
     tol_g = FP64_G if precision == 0 else MIX_G
     res = {}
-    combos = [(1, 1, 1), (2, 2, 2), (4, 4, 4), (8, 8, 8), (8, 1, 2), (1, 4, 8), (2, 8, 1)]
+    combos = [(1, 1, 1), (2, 2, 2), (4, 4, 4), (8, 8, 8), (16, 16, 16), (8, 1, 2), (1, 4, 8), (16, 2, 1)]
     for store in (1, 0):
         ctx.set_derivative_store(store)
         ctx.nblist()
@@ -314,8 +314,9 @@ def test_step_split_matches_oracle(gpu, gen_dir, precision):
     for key, e in res.items():
         assert rel(e[0], base[0]) <= 1e-12, key
         assert rel_rms(e[3], base[3]) <= (1e-12 if precision == 0 else 1e-6), key
-    with pytest.raises(na.GBHipError):
-        ctx.set_split(3, 1, 1)
+    for bad in ((3, 1, 1), (1, 32, 1), (1, 1, -2)):
+        with pytest.raises(na.GBHipError):
+            ctx.set_split(*bad)
     ctx.set_split(0, 0, 0)
     assert ctx.split() == (8, 8, 8)
     ctx.close()
