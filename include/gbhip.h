@@ -193,9 +193,10 @@ int gbhip_set_launch(gbhip_ctx *ctx, const int *blocks3);
  * steps are dealt out in that many chunks, one work item each, so that a
  * system with few tile pairs still occupies the chip (600 atoms: 55 tile
  * pairs for 1,024 SIMDs).  split3[3] per pass (pass 1, pass 2, pass 3):
- * 1, 2, 4, 8, 16, or 0 for the default, which doubles the split, up to 8,
- * while the pass has fewer than 64 (pass 1) or 4 (passes 2, 3) work items
- * per CU; large systems run with 1.  gbhip_set_split resets the launch shapes to
+ * 1, 2, 4, 8, 16, or 0 for the default, which doubles the split while the
+ * pass has fewer than 64 (pass 1) or 6 (passes 2, 3) work items per CU, up
+ * to 4 in pass 1 (8 below 128 tile pairs) and 8 in the others; large
+ * systems run with 1.  gbhip_set_split resets the launch shapes to
  * their defaults; gbhip_autotune also times each split for small systems.
  * No counterpart in the reference (a launch-shape choice).
  */

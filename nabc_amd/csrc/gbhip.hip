@@ -1270,21 +1270,26 @@ static int check_topology(const gbhip_topology *t, int gb)
  * Step split for a system with few tile pairs: one work item is one
  * wavefront's sweep, so a system with fewer tile pairs than the chip has
  * SIMDs leaves most of them idle, and a pass cannot end before its longest
- * sweep.  Each tile pair's steps are dealt out in up to SPLIT_DEFAULT_MAX
- * chunks until a pass has split_items_per_cu work items per CU, counted
- * from the tile-pair capacity: 64 for pass 1, whose sweeps are the longest,
- * 4 for the others.  Measured (HIP events, mixed): 600 atoms split 8 in
- * every pass; 7,000 atoms at 20 A pass 1 44 us at split 4 against 62 at 1
- * and 57 at 8, passes 2 and 3 best unsplit from a few thousand atoms on
- * (more per-visit set-up and flush atomics).
+ * sweep.  Each tile pair's steps are dealt out in chunks, doubling the
+ * split while a pass has fewer than split_items_per_cu work items per CU
+ * (counted from the tile-pair capacity): 64 for pass 1, whose sweeps are
+ * the longest, 6 for the others; pass 1 stops at 4 chunks unless the system
+ * has fewer than 128 tile pairs, the others at SPLIT_DEFAULT_MAX.  Fitted to
+ * what the autotuner picks (mixed, HIP events): 600 atoms 8 in every pass;
+ * 2,500 atoms without cutoff (820 tile pairs) and 3,000 atoms at 20 A
+ * (1,108) pass 1 at 4, passes 2 and 3 at 2 (3,000 atoms: 86 -> 74 us per
+ * evaluation against the earlier default); 5,000-7,000 atoms at 20 A pass
+ * 1 at 4, passes 2 and 3 unsplit (more chunks add per-visit set-up and
+ * flush atomics).
  */
-static const int split_items_per_cu[3] = {64, 4, 4};
+static const int split_items_per_cu[3] = {64, 6, 6};
 
 static int default_split(const Engine &E, int pass)
 {
     int sh = 0;
     const long long cap = std::max<long long>(1, E.tp_cap);
-    while ((1 << sh) < SPLIT_DEFAULT_MAX && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
+    const int most = pass == 0 && cap >= 128 ? 4 : SPLIT_DEFAULT_MAX;
+    while ((1 << sh) < most && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
     return sh;
 }
 
