@@ -77,7 +77,9 @@ const char *mm_get_error(void);
 /*
  * Initialisation (sff.c:858).  frozen[] and constrained[] are per-atom
  * flags (may be NULL = none).  x0 is kept for restraints (unused by GB).
- * ncname must be NULL (netCDF trajectories are not built).
+ * ncname (may be NULL): an Amber netCDF trajectory file is created there
+ * and md() appends a frame every ntwx steps (sff.c:1164-1168, :2341-2342;
+ * the layout of AmberNetcdf.c's netcdfCreate, plus the frame times).
  */
 int mme_init_sff(PARMSTRUCT_T *prm, int *frozen, int *constrained,
                  double *x0, char *ncname);

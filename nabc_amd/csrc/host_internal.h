@@ -57,6 +57,16 @@ void nabc_seterr(const char *fmt, ...);
 int nabc_atomic_number_from_mass(double mass);
 void nabc_build_14_list(PARMSTRUCT_T *prm);
 
+/* Amber netCDF trajectory writer (nctraj.c) */
+typedef struct {
+    FILE *fp;
+    int natom, nframes;
+    long long rec_begin, recsize; /* offset of the first frame, bytes per frame */
+} nabc_nctraj;
+int nabc_nctraj_create(nabc_nctraj *t, const char *path, int natom);
+int nabc_nctraj_write(nabc_nctraj *t, const double *x, double time_ps);
+void nabc_nctraj_close(nabc_nctraj *t);
+
 #ifdef __cplusplus
 }
 #endif

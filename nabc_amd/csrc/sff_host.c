@@ -29,6 +29,7 @@ static struct {
     int have_list;
     long long nb_pairs;
     double ene[14];     /* last mme(): mme34 energies + rms gradient */
+    nabc_nctraj traj;   /* md() trajectory, when mme_init_sff was given a file name */
 } S;
 
 static FILE *out(void) { return nabc_out ? nabc_out : stdout; }
@@ -42,15 +43,16 @@ static void release_ctx(void)
 
 int mme_init_sff(PARMSTRUCT_T *prm, int *frozen, int *constrained, double *x0, char *ncname)
 {
+    int rc;
     (void)x0;
     (void)constrained;
     release_ctx();
     nabc_model_free(&S.m);
-    if (ncname) {
-        nabc_seterr("mme_init_sff: netCDF trajectories are not provided by this library");
-        return -1;
-    }
-    return nabc_model_build(&S.m, prm, frozen, &nabc_opts);
+    nabc_nctraj_close(&S.traj);
+    rc = nabc_model_build(&S.m, prm, frozen, &nabc_opts);
+    /* the trajectory file md() writes every ntwx steps (sff.c:1164-1168) */
+    if (rc == 0 && ncname) rc = nabc_nctraj_create(&S.traj, ncname, prm->Natom);
+    return rc;
 }
 
 static int ensure_ctx(const double *x)
@@ -189,8 +191,28 @@ static int md_device(int n, int maxstep, double *x, double *f, double *v)
     mo.zerov = o->zerov;
     mo.seed = o->md_seed;
     if (cap > (1 << 20)) cap = 1 << 20;
-    if (gbhip_md_init(S.ctx, S.m.prm->Masses, x, v, &mo, cap) != 0 || gbhip_md_run(S.ctx, maxstep, 1) != 0 ||
-        gbhip_md_state(S.ctx, x, v, f) != 0) {
+    if (gbhip_md_init(S.ctx, S.m.prm->Masses, x, v, &mo, cap) != 0) {
+        nabc_seterr("%s", gbhip_last_error());
+        return -1;
+    }
+    if (S.traj.fp && o->ntwx > 0) {
+        /* frames every ntwx steps (sff.c:2341-2342): the device runs in
+         * chunks that end on a frame step */
+        int done = 0;
+        while (done < maxstep) {
+            const int chunk = maxstep - done < o->ntwx ? maxstep - done : o->ntwx;
+            if (gbhip_md_run(S.ctx, chunk, 1) != 0 || gbhip_md_state(S.ctx, x, NULL, NULL) != 0) {
+                nabc_seterr("%s", gbhip_last_error());
+                return -1;
+            }
+            done += chunk;
+            if (done % o->ntwx == 0 && nabc_nctraj_write(&S.traj, x, o->t + done * o->dt) != 0) return -1;
+        }
+    } else if (gbhip_md_run(S.ctx, maxstep, 1) != 0) {
+        nabc_seterr("%s", gbhip_last_error());
+        return -1;
+    }
+    if (gbhip_md_state(S.ctx, x, v, f) != 0) {
         nabc_seterr("%s", gbhip_last_error());
         return -1;
     }
@@ -214,7 +236,8 @@ static int md_device(int n, int maxstep, double *x, double *f, double *v)
     return 0;
 }
 
-/* md(), sff.c:2071-2366 (no rattle, no COM removal, no trajectory output) */
+/* md(), sff.c:2071-2366 (no rattle, no COM removal); trajectory frames
+ * every ntwx steps when mme_init_sff was given a file name */
 int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(double *, double *, int *))
 {
     const nabc_opts_t *o = &nabc_opts;
@@ -303,6 +326,12 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
         if (gammai > 0.) ekin *= c_ave;
         temp = ekin / (o->boltz2 * rndf);
         t += o->dt;
+        if (o->ntwx > 0 && nstep % o->ntwx == 0 && S.traj.fp && nabc_nctraj_write(&S.traj, x, t) != 0) {
+            free(minv);
+            free(accel);
+            free(sqrmass);
+            return -1;
+        }
         if (nstep % o->ntpr_md == 0 || nstep == 1) {
             fprintf(out(), "md:       %5d %10.3f %10.2f %10.2f %10.2f %10.2f\n", nstep, t, ekin, epot, ekin + epot,
                     temp);

@@ -185,3 +185,44 @@ def test_device_md_frozen_langevin_follows_host_loop(gpu, gen_dir):
     assert rel_rms(va, vb) < 1e-8
     assert np.array_equal(xa[fz3], x0[fz3]) and np.all(va[fz3] == 0.0)
     assert np.max(np.abs(xa[~fz3] - x0[~fz3])) > 1e-3  # the others moved
+
+
+def test_md_writes_netcdf_trajectory(gpu, gen_dir, tmp_path):
+    """mme_init_sff with a file name and ntwx > 0: md() appends a frame of
+    single-precision coordinates every ntwx steps (sff.c:1164-1168,
+    :2341-2342), from the device loop (which runs in chunks ending on frame
+    steps) and from the host loop alike; the last frame is the final
+    coordinates."""
+    from scipy.io import netcdf_file
+    na = gpu
+    parm, arr, x0 = generated_system(600, 28, gen_dir)
+    n = 3 * arr["natom"]
+    rng = np.random.default_rng(2)
+    v0 = rng.normal(0, 0.05, n)
+    opts = "gb=5, cut=12.0, rgbmax=12.0, nsnb=5, gamma_ln=0.0, tempi=0.0, dt=0.001, ntwx=5, ntpr_md=1000, ntpr=100000"
+    frames, finals = [], []
+    for host in (False, True):
+        path = str(tmp_path / ("host.nc" if host else "device.nc"))
+        na.lib.mm_reset_options()
+        assert na.lib.mm_options(opts.encode()) == 0, na.lib.mm_get_error()
+        assert na.lib.mme_init_sff(parm, None, None, None, path.encode()) == 0, na.lib.mm_get_error()
+        x, v = x0.copy(), v0.copy()
+        wrap = na.MMEFUNC(lambda xp, fp, itp: na.lib.mme(xp, fp, itp)) if host else None
+        _md(na, n, 20, x, v, func=wrap)
+        f = netcdf_file(path, "r", mmap=False)
+        c = f.variables["coordinates"][:].copy()
+        tm = f.variables["time"][:].copy()
+        f.close()
+        assert c.shape == (4, arr["natom"], 3)  # steps 5, 10, 15, 20
+        assert np.allclose(np.diff(tm), 0.005, atol=1e-6)
+        assert np.array_equal(c[-1], x.astype(np.float32).reshape(-1, 3))
+        frames.append(c)
+        finals.append(x)
+    # the two loops agree to summation-order noise, so their float frames
+    # agree to a rounding step of single precision
+    assert np.max(np.abs(frames[0] - frames[1])) < 1e-4
+    assert np.max(np.abs(frames[0][0] - x0.reshape(-1, 3))) > 1e-4  # it moved
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+    # a fresh initialisation closes the trajectory file
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0, na.lib.mm_get_error()
