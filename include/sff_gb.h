@@ -118,6 +118,53 @@ int conjgrad(double x[], int *n, double *f,
 /* Output stream used for the ff:/md: lines (libsff's nabout). */
 void mm_set_output(FILE *fp);
 
+/*
+ * Amber netCDF trajectories and restarts (AmberNetcdf.h:22-62, declared in
+ * sff.h:240-274): the same structure (field names, types and order) and
+ * functions, written and read without the netCDF library.  ncid indexes a
+ * table of at most 64 open files; the *VID fields are -1 for a variable
+ * the file lacks and 1 otherwise.  netcdfCreate writes trajectories
+ * without box information only (isBox must be 0).  Return 0 on success, 1
+ * on failure (message via mm_get_error).
+ */
+struct AmberNetcdf {
+    double temp0;       /* temperature of a restart (TempVID != -1) */
+    double restartTime; /* simulation time of a restart */
+    int isNCrestart;    /* 0 trajectory, 1 restart */
+    int ncid;
+    int frameDID;
+    int ncframe;        /* frames in the file */
+    int currentFrame;   /* next frame of netcdfGet/WriteNextFrame */
+    int atomDID;
+    int ncatom;
+    int ncatom3;
+    int coordVID;
+    int velocityVID;
+    int cellAngleVID;
+    int cellLengthVID;
+    int spatialDID;
+    int labelDID;
+    int cell_spatialDID;
+    int cell_angularDID;
+    int spatialVID;
+    int timeVID;
+    int cell_spatialVID;
+    int cell_angularVID;
+    int TempVID;
+};
+int netcdfDebug(struct AmberNetcdf *A);
+int netcdfLoad(struct AmberNetcdf *A, char *filename);
+int netcdfClose(struct AmberNetcdf *A);
+int netcdfWriteRestart(char *filename, int natom, double *X, double *V, double *box, double time,
+                       double temp);
+int netcdfCreate(struct AmberNetcdf *A, char *filename, int natom, int isBox);
+int netcdfGetVelocity(struct AmberNetcdf *A, int set, double *V);
+int netcdfGetFrame(struct AmberNetcdf *A, int set, double *X, double *box);
+int netcdfGetNextFrame(struct AmberNetcdf *A, double *X, double *box);
+int netcdfWriteFrame(struct AmberNetcdf *A, int set, double *X, double *box);
+int netcdfWriteNextFrame(struct AmberNetcdf *A, double *X, double *box);
+int netcdfInfo(struct AmberNetcdf *A);
+
 #ifdef __cplusplus
 }
 #endif

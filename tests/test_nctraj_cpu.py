@@ -109,3 +109,118 @@ def test_bad_arguments_are_errors(tmp_path):
     assert lib.nabc_nctraj_create(C.byref(t), str(tmp_path / "no_such_dir" / "w.nc").encode(), 3) != 0
     x = np.zeros(9)
     assert lib.nabc_nctraj_write(C.byref(t), x.ctypes.data_as(C.POINTER(C.c_double)), 0.0) != 0
+
+
+class AmberNetcdf(C.Structure):
+    _fields_ = [("temp0", C.c_double), ("restartTime", C.c_double)] + [
+        (n, C.c_int) for n in ("isNCrestart", "ncid", "frameDID", "ncframe", "currentFrame", "atomDID", "ncatom",
+                               "ncatom3", "coordVID", "velocityVID", "cellAngleVID", "cellLengthVID", "spatialDID",
+                               "labelDID", "cell_spatialDID", "cell_angularDID", "spatialVID", "timeVID",
+                               "cell_spatialVID", "cell_angularVID", "TempVID")]
+
+
+def _api():
+    import nabc_amd as na
+    lib = na.lib
+    A, dp = C.POINTER(AmberNetcdf), C.POINTER(C.c_double)
+    for name, args in (("netcdfLoad", [A, C.c_char_p]), ("netcdfClose", [A]), ("netcdfCreate", [A, C.c_char_p, C.c_int, C.c_int]),
+                       ("netcdfGetFrame", [A, C.c_int, dp, dp]), ("netcdfGetNextFrame", [A, dp, dp]),
+                       ("netcdfGetVelocity", [A, C.c_int, dp]), ("netcdfWriteFrame", [A, C.c_int, dp, dp]),
+                       ("netcdfWriteNextFrame", [A, dp, dp]),
+                       ("netcdfWriteRestart", [C.c_char_p, C.c_int, dp, dp, dp, C.c_double, C.c_double])):
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = C.c_int
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def test_reads_reference_trajectory():
+    """netcdfLoad / netcdfGetFrame on the reference's test/trpcage.nc (a
+    trajectory sander wrote through the netCDF library) give what scipy
+    reads from it."""
+    lib = _api()
+    path = os.path.join(GOLDEN, "trpcage.nc")
+    f = netcdf_file(path, "r", mmap=False)
+    ref = f.variables["coordinates"][:].astype(np.float64)
+    f.close()
+    A = AmberNetcdf()
+    assert lib.netcdfLoad(C.byref(A), path.encode()) == 0
+    assert (A.ncframe, A.ncatom, A.ncatom3, A.isNCrestart) == (101, 223, 669, 0)
+    assert A.velocityVID == -1 and A.cellLengthVID == -1
+    x = np.zeros(3 * 223)
+    for k in (0, 50, 100):
+        assert lib.netcdfGetFrame(C.byref(A), k, _p(x), None) == 0
+        assert np.array_equal(x, ref[k].ravel())
+    A.currentFrame = 99
+    assert lib.netcdfGetNextFrame(C.byref(A), _p(x), None) == 0 and A.currentFrame == 100
+    assert np.array_equal(x, ref[99].ravel())
+    assert lib.netcdfGetFrame(C.byref(A), 101, _p(x), None) != 0
+    assert lib.netcdfClose(C.byref(A)) == 0 and A.ncid == -1
+
+
+def test_create_write_load_round_trip(tmp_path):
+    lib = _api()
+    rng = np.random.default_rng(8)
+    natom = 11
+    path = str(tmp_path / "c.nc").encode()
+    A = AmberNetcdf()
+    assert lib.netcdfCreate(C.byref(A), path, natom, 0) == 0
+    frames = [rng.normal(0, 9, 3 * natom) for _ in range(3)]
+    for fr in frames:
+        assert lib.netcdfWriteNextFrame(C.byref(A), _p(fr), None) == 0
+    assert A.currentFrame == 3 and A.ncframe == 3
+    # a frame rewritten in place
+    frames[1] = rng.normal(0, 9, 3 * natom)
+    assert lib.netcdfWriteFrame(C.byref(A), 1, _p(frames[1]), None) == 0
+    assert lib.netcdfClose(C.byref(A)) == 0
+    assert lib.netcdfCreate(C.byref(A), path, natom, 1) != 0  # box output is not written
+    B = AmberNetcdf()
+    assert lib.netcdfLoad(C.byref(B), path) == 0
+    assert (B.ncframe, B.ncatom, B.isNCrestart) == (3, natom, 0)
+    x = np.zeros(3 * natom)
+    for k in range(3):
+        assert lib.netcdfGetFrame(C.byref(B), k, _p(x), None) == 0
+        assert np.array_equal(x, frames[k].astype(np.float32).astype(np.float64))
+    assert lib.netcdfClose(C.byref(B)) == 0
+
+
+def test_restart_write_and_load(tmp_path):
+    """netcdfWriteRestart: the AMBERRESTART layout of AmberNetcdf.c:366-593
+    (FP64 coordinates, velocities with scale_factor 20.455, time, temp0),
+    checked with scipy and read back with netcdfLoad."""
+    lib = _api()
+    rng = np.random.default_rng(9)
+    natom = 13
+    X = rng.normal(0, 10, 3 * natom)
+    V = rng.normal(0, 0.1, 3 * natom)
+    path = str(tmp_path / "r.ncrst")
+    assert lib.netcdfWriteRestart(path.encode(), natom, _p(X), _p(V), None, 12.5, 300.0) == 0
+    f = netcdf_file(path, "r", mmap=False)
+    assert f.version_byte == 2
+    assert f._attributes["Conventions"] == b"AMBERRESTART"
+    assert "frame" not in f.dimensions and f.dimensions["atom"] == natom
+    assert np.array_equal(f.variables["coordinates"][:].ravel(), X)
+    assert np.array_equal(f.variables["velocities"][:].ravel(), V)
+    assert f.variables["velocities"]._attributes["scale_factor"] == 20.455
+    assert f.variables["velocities"]._attributes["units"] == b"angstrom/picosecond"
+    assert float(f.variables["time"].getValue()) == 12.5
+    assert float(f.variables["temp0"].getValue()) == 300.0
+    f.close()
+    A = AmberNetcdf()
+    assert lib.netcdfLoad(C.byref(A), path.encode()) == 0
+    assert (A.isNCrestart, A.ncframe, A.ncatom, A.restartTime, A.temp0) == (1, 1, natom, 12.5, 300.0)
+    x, v = np.zeros(3 * natom), np.zeros(3 * natom)
+    assert lib.netcdfGetFrame(C.byref(A), 0, _p(x), None) == 0 and np.array_equal(x, X)
+    assert lib.netcdfGetVelocity(C.byref(A), 0, _p(v)) == 0 and np.array_equal(v, V)
+    assert lib.netcdfClose(C.byref(A)) == 0
+    # without velocities and temperature
+    path2 = str(tmp_path / "r2.ncrst")
+    assert lib.netcdfWriteRestart(path2.encode(), natom, _p(X), None, None, -1.0, -1.0) == 0
+    assert lib.netcdfLoad(C.byref(A), path2.encode()) == 0
+    assert A.velocityVID == -1 and A.TempVID == -1 and A.restartTime == 0.0
+    assert lib.netcdfGetVelocity(C.byref(A), 0, _p(v)) != 0
+    assert lib.netcdfClose(C.byref(A)) == 0
+    assert lib.netcdfLoad(C.byref(A), str(tmp_path / "missing.nc").encode()) != 0
