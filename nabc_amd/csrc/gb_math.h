@@ -137,7 +137,17 @@ __device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R di
     const R dm = dij - sj;
     u1 = M::rcp(dij + sj);
     u2 = M::rcp(k.a ? dm : r2 - sj * sj);
-    const R arg = k.a ? dm * rgbmax1i : (k.d ? ri * u1 : fabs(dm) * u1);
+    R arg;
+    if constexpr (sizeof(R) == sizeof(float)) {
+        /* FP32: every candidate computed, then selected per lane, instead
+         * of the branches the nested conditional compiles to (pass 1 at
+         * 25k atoms, 25 A: 0.246 -> 0.233 ms); the FP64 kernel is faster
+         * with the conditional */
+        const R arg_a = dm * rgbmax1i, arg_d = ri * u1, arg_ce = fabs(dm) * u1;
+        arg = k.a ? arg_a : (k.d ? arg_d : arg_ce);
+    } else {
+        arg = k.a ? dm * rgbmax1i : (k.d ? ri * u1 : fabs(dm) * u1);
+    }
     L = M::log_(arg);
 }
 
