@@ -377,6 +377,9 @@ static int mme_graph_launch(gbhip_ctx *c)
     if (!c->B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
     if (!E.have_list) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
     if (!E.stream) {
+        /* work queued on the default stream (a coordinate copy, a list
+         * build) is not ordered before a non-blocking stream's: wait for it */
+        HIPCHK(hipStreamSynchronize(nullptr));
         hipStream_t s;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         E.stream = s;
@@ -775,7 +778,9 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
     const int n = E.natoms, n3 = 3 * n;
     int rc;
     if (!E.stream) {
-        /* graph capture needs a stream of our own (default priority) */
+        /* graph capture needs a stream of our own (default priority); work
+         * already queued on the default stream is not ordered before it */
+        HIPCHK(hipStreamSynchronize(nullptr));
         hipStream_t s;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         E.stream = s;

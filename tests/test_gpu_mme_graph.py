@@ -141,3 +141,30 @@ def test_graph_after_async_rebuild_outgrows_store(gpu, gen_dir):
     assert rel_rms(g1, g2) < 1e-9
     it = na.C.c_int(-3)
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_first_graph_call_sees_coordinates_copied_on_default_stream(gpu, gen_dir):
+    """A context made without a stream copies coordinates on the default
+    stream; its first captured evaluation runs on a stream of its own, which
+    is ordered after that copy (the evaluation equals a plain one at the new
+    coordinates, not one at the old)."""
+    na = gpu
+    parm, arr, x0 = generated_system(2500, 29, gen_dir)
+    _init(na, parm, "gb=5, cut=12.0, rgbmax=12.0, nsnb=100000, ntpr=100000")
+    ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+    assert ctx
+    np_ = na.C.c_longlong()
+    assert na.lib.gbhip_nblist(ctx, na.C.byref(np_)) == 0
+    x1 = np.ascontiguousarray(x0 + np.random.default_rng(6).normal(0, 0.05, x0.shape))
+    g1, e1 = np.zeros_like(x0), np.zeros(14)
+    assert na.lib.gbhip_set_coords(ctx, x1.ctypes.data_as(na.dp)) == 0
+    assert na.lib.gbhip_mme(ctx, g1.ctypes.data_as(na.dp), e1.ctypes.data_as(na.dp)) == 0
+    assert _stats(na, ctx) == (1, 1)
+    assert na.lib.gbhip_set_mme_graph(ctx, 0) == 0
+    g2, e2 = np.zeros_like(x0), np.zeros(14)
+    assert na.lib.gbhip_set_coords(ctx, x1.ctypes.data_as(na.dp)) == 0
+    assert na.lib.gbhip_mme(ctx, g2.ctypes.data_as(na.dp), e2.ctypes.data_as(na.dp)) == 0
+    assert np.max(np.abs(e1 - e2)) <= 1e-11 * np.abs(e2).max()
+    assert rel_rms(g1, g2) < 1e-11
+    it = na.C.c_int(-3)
+    na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
