@@ -386,6 +386,46 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+def bench_sweep(args):
+    """Size sweep of the headline workload (igb=8, cut = rgbmax = 25 A,
+    kappa 0.127315, list rebuilt every 20 evaluations): evaluations/s and
+    pair interactions/s per precision at 5k-100k atoms, default launch
+    shapes and step split (no autotuning), HIP-event pass times."""
+    import numpy as np
+    import nabc_amd as na
+    rows = []
+    for natom in (5000, 10000, 25000, 50000, 100000):
+        parm, arr, x = na.build_system(natom, SEED)
+        row = {"natom": natom}
+        for name, prec in (("fp64", na.GBHIP_FP64), ("mixed", na.GBHIP_MIXED)):
+            ctx = na.GBContext(arr, x, gb=GB, cut=CUT, rgbmax=RGBMAX, kappa=KAPPA, precision=prec)
+            ctx.nblist()
+            npairs = ctx.npairs()
+            steps = max(20, min(200, int(2e10 / max(1, npairs))))
+            for k in range(5):
+                ctx.egb_device()
+            ctx.sync()
+            ctx.timing_start(steps)
+            t0 = time.perf_counter()
+            for k in range(steps):
+                if k % NSNB == 0:
+                    ctx.nblist_async()
+                ctx.egb_device()
+            ctx.sync()
+            t1 = time.perf_counter()
+            ms, n = ctx.timing_stop()
+            row[name] = {"evals_per_s": round(steps / (t1 - t0), 1),
+                         "pairs_per_s": float("%.4g" % (npairs * steps / (t1 - t0))),
+                         "pass_ms": [round(v / n, 4) for v in ms], "split": list(ctx.split())}
+            row["npairs"] = npairs
+            ctx.close()
+        row["mixed_vs_fp64"] = round(row["mixed"]["evals_per_s"] / row["fp64"]["evals_per_s"], 3)
+        rows.append(row)
+        print("# %d atoms: %s" % (natom, json.dumps(row)), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "GB energy+grad evals/s vs system size (igb=8, 25 A)", "unit": "evals/s",
+                      "data": "synthetic (nabc_amd/csrc/gen.c, seed %d)" % SEED, "rows": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,10 +438,10 @@ def main():
                          "reproduces a tuned run's shapes under a profiler")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy"],
+    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy", "sweep"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
-                         "errors against the oracle at C1/C2/C3")
+                         "errors against the oracle at C1/C2/C3; sweep: evaluation rate at 5k-100k atoms")
     ap.add_argument("--jobs", type=int, default=200)
     ap.add_argument("--rate", type=float, default=10.0)
     ap.add_argument("--slots", type=int, default=8)
@@ -426,6 +466,8 @@ def main():
         return bench_serve(args)
     if args.config == "accuracy":
         return bench_accuracy(args)
+    if args.config == "sweep":
+        return bench_sweep(args)
 
     import numpy as np
     import nabc_amd as na
