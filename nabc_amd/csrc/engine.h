@@ -66,7 +66,7 @@ struct DevPtrs {
     double *xs, *ys, *zs;
     double *sumi, *reff, *psi, *dfac, *sumdeijda, *gx, *gy, *gz;
     double *ene;          /* 3 per molecule: epol, evdw, eelec */
-    double *eslot;        /* ENE_SLOTS partial sums per ene entry (ene_slot_sum adds them up) */
+    double *eslot;        /* ENE_SLOTS partial sums per ene entry (ene_slot_sums adds them up) */
     /* tiles and tile pairs */
     int ntiles;
     const int *tile_mol;
@@ -291,20 +291,26 @@ __device__ __forceinline__ void slot_gather(const DevPtrs &d, int s, const doubl
     d.gx[s] = 0.0; d.gy[s] = 0.0; d.gz[s] = 0.0;
 }
 
-/* energy entry e (3 per molecule) = the sum of its ENE_SLOTS partials in
- * slot order; the partials are cleared for the next evaluation (they start
- * at zero from allocation) */
-__device__ __forceinline__ void ene_slot_sum(const DevPtrs &d, int e)
+/* energy entries (3 per molecule): entry e = the sum of its ENE_SLOTS
+ * partials, each read by one lane of a wavefront and summed by a fixed
+ * shuffle tree (deterministic); the partials are cleared for the next
+ * evaluation (they start at zero from allocation).  Called by every lane
+ * of the grid; its wavefronts take the entries in turn. */
+static_assert(ENE_SLOTS == 64, "one lane per energy slot");
+__device__ __forceinline__ void ene_slot_sums(const DevPtrs &d, int nene)
 {
-    double *p = d.eslot + (size_t)e * ENE_SLOTS;
-    double acc = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < ENE_SLOTS; k++) {
-        acc += p[k];
-        p[k] = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    for (int e = wave; e < nene; e += nwaves) {
+        double *p = d.eslot + (size_t)e * ENE_SLOTS;
+        double v = p[lane];
+        p[lane] = 0.0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) d.ene[e] = v;
     }
-    d.ene[e] = acc;
 }
+
 bool engine_stage(Engine &E);
 
 #endif

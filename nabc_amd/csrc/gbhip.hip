@@ -1137,14 +1137,14 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
 }
 
 /*
- * After pass 3: thread e < nene sums energy entry e's slots (ene_slot_sum),
- * and with a gradient array, thread s < nslots adds slot s's GB gradient
- * into the gradient of its atom (original order, AoS).
+ * After pass 3: the energy entries' slot sums (ene_slot_sums), and with a
+ * gradient array, thread s < nslots adds slot s's GB gradient into the
+ * gradient of its atom (original order, AoS).
  */
 __global__ __launch_bounds__(256) void k_post(DevPtrs d, GBPar p, double *__restrict__ g, int nene)
 {
+    ene_slot_sums(d, nene);
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s < nene) ene_slot_sum(d, s);
     if (!g || s >= p.nslots) return;
     const int a = d.perm[s];
     if (a < 0) return;
@@ -1679,7 +1679,8 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
     }
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (!(flags & EV_NO_POST)) {
-        const int nene = 3 * E.nmol, nt = std::max(nene, g_dev ? E.nslots : 0);
+        /* a wavefront per energy entry, a thread per slot */
+        const int nene = 3 * E.nmol, nt = std::max(64 * nene, g_dev ? E.nslots : 0);
         hipLaunchKernelGGL(k_post, dim3((nt + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev, nene);
     }
     LAUNCHCHK();

@@ -590,7 +590,7 @@ __global__ __launch_bounds__(MM_THREADS) void k_mme_pre(DevPtrs d, GBPar p, doub
 }
 
 /*
- * Grid-stride over slots: the energy-slot sums (ene_slot_sum), the slot's
+ * The energy-slot sums (ene_slot_sums), then grid-stride over slots: the slot's
  * GB gradient added into its atom's, frozen gradients zeroed (eff.c:
  * 2826-2834), per-block sums of squared gradient components (written per
  * block, so the rms is summed in a fixed order), and [dynamics] the second
@@ -601,11 +601,9 @@ __global__ __launch_bounds__(MM_THREADS) void k_mme_post(DevPtrs d, GBPar p, Bon
                                                          MDState M, MDPar P)
 {
     __shared__ double s_red[MM_THREADS / 64];
-    const int nt = max(p.nslots, nene);
+    ene_slot_sums(d, nene);
     double sq = 0.0, ek = 0.0;
-    for (int s = blockIdx.x * MM_THREADS + threadIdx.x; s < nt; s += gridDim.x * MM_THREADS) {
-        if (s < nene) ene_slot_sum(d, s);
-        if (s >= p.nslots) continue;
+    for (int s = blockIdx.x * MM_THREADS + threadIdx.x; s < p.nslots; s += gridDim.x * MM_THREADS) {
         const int a = d.perm[s];
         if (a < 0) continue;
         double *ga = g + 3 * (size_t)a;
@@ -679,7 +677,7 @@ __global__ __launch_bounds__(64) void k_mme_finish(BondedDev B, const double *gb
 /* grid of k_mme_post (its partial sums fit the GNORM_PARTS entries) */
 static int post_blocks(const Engine &E)
 {
-    return std::min(GNORM_PARTS, std::max(1, grid_of(std::max(E.nslots, 3 * E.nmol))));
+    return std::min(GNORM_PARTS, std::max(1, grid_of(E.nslots)));
 }
 
 /*
