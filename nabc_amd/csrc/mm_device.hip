@@ -590,11 +590,14 @@ __global__ __launch_bounds__(MM_THREADS) void k_mme_pre(DevPtrs d, GBPar p, doub
 }
 
 /*
- * The energy-slot sums (ene_slot_sums), then grid-stride over slots: the slot's
- * GB gradient added into its atom's, frozen gradients zeroed (eff.c:
- * 2826-2834), per-block sums of squared gradient components (written per
- * block, so the rms is summed in a fixed order), and [dynamics] the second
- * half kick with the new forces and 2*ekin (sff.c:2262-2277).
+ * The energy-slot sums (ene_slot_sums), then grid-stride over (slot,
+ * component) pairs, one gradient component per thread (a dynamics step's
+ * noise is three FP64 log/sqrt/cos evaluations per atom, so a thread per
+ * component triples the parallelism of the kick): the slot's GB gradient
+ * added into its atom's, frozen gradients zeroed (eff.c:2826-2834),
+ * per-block sums of squared gradient components (written per block, so the
+ * rms is summed in a fixed order), and [dynamics] the second half kick with
+ * the new forces and 2*ekin (sff.c:2262-2277).
  */
 template <bool MD>
 __global__ __launch_bounds__(MM_THREADS) void k_mme_post(DevPtrs d, GBPar p, BondedDev B, double *g, int nene,
@@ -603,31 +606,30 @@ __global__ __launch_bounds__(MM_THREADS) void k_mme_post(DevPtrs d, GBPar p, Bon
     __shared__ double s_red[MM_THREADS / 64];
     ene_slot_sums(d, nene);
     double sq = 0.0, ek = 0.0;
-    for (int s = blockIdx.x * MM_THREADS + threadIdx.x; s < p.nslots; s += gridDim.x * MM_THREADS) {
+    const int n3 = 3 * p.nslots;
+    for (int t = blockIdx.x * MM_THREADS + threadIdx.x; t < n3; t += gridDim.x * MM_THREADS) {
+        const int s = t / 3, c = t - 3 * s;
         const int a = d.perm[s];
         if (a < 0) continue;
-        double *ga = g + 3 * (size_t)a;
-        double gv[3] = {ga[0] + d.gx[s], ga[1] + d.gy[s], ga[2] + d.gz[s]};
+        const int i = 3 * a + c;
         const bool frz = B.frozen[a] != 0;
-        if (frz) gv[0] = gv[1] = gv[2] = 0.0;
-        else sq += gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
-        ga[0] = gv[0]; ga[1] = gv[1]; ga[2] = gv[2];
+        const double gs = c == 0 ? d.gx[s] : (c == 1 ? d.gy[s] : d.gz[s]);
+        const double gv = frz ? 0.0 : g[i] + gs;
+        sq += gv * gv;
+        g[i] = gv;
         if (MD) {
-            for (int c = 0; c < 3; c++) {
-                const int i = 3 * a + c;
-                double v = M.v[i];
-                if (P.gammai == 0.0) {
-                    const double acc = -gv[c] * M.minv[i] * P.dt5;
-                    M.accel[i] = acc;
-                    v = (v + acc) * md_tscal(P, M.scal);
-                } else if (!frz) {
-                    const double acc = (-gv[c] + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, *M.step, i)) * M.minv[i] * P.dt5;
-                    M.accel[i] = acc;
-                    v = (v + acc) * P.c_implic;
-                }
-                M.v[i] = v;
-                ek += v * v / M.minv[i];
+            double v = M.v[i];
+            if (P.gammai == 0.0) {
+                const double acc = -gv * M.minv[i] * P.dt5;
+                M.accel[i] = acc;
+                v = (v + acc) * md_tscal(P, M.scal);
+            } else if (!frz) {
+                const double acc = (-gv + P.sdfac * M.sqrmass[i] * md_gauss(P.seed, *M.step, i)) * M.minv[i] * P.dt5;
+                M.accel[i] = acc;
+                v = (v + acc) * P.c_implic;
             }
+            M.v[i] = v;
+            ek += v * v / M.minv[i];
         }
     }
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
@@ -677,7 +679,7 @@ __global__ __launch_bounds__(64) void k_mme_finish(BondedDev B, const double *gb
 /* grid of k_mme_post (its partial sums fit the GNORM_PARTS entries) */
 static int post_blocks(const Engine &E)
 {
-    return std::min(GNORM_PARTS, std::max(1, grid_of(E.nslots)));
+    return std::min(GNORM_PARTS, std::max(1, grid_of(3LL * E.nslots)));
 }
 
 /*
