@@ -749,13 +749,13 @@ static int md_step_launch(gbhip_ctx *c, bool rebuild)
 static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t *ge)
 {
     Engine &E = c->E;
-    const int saved_cap = E.tev_cap;
-    E.tev_cap = 0; /* no timing events inside a graph */
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     /* the graph keeps the derivative-store pointer it was captured with;
      * the steps replay across list rebuilds */
     E.store_frozen = true;
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    const int saved_cap = E.tev_cap;
+    E.tev_cap = 0; /* no timing events inside a graph */
     E.capture_spans_rebuilds = true;
     int rc = md_step_launch(c, rebuild);
     E.capture_spans_rebuilds = false;
