@@ -12,8 +12,9 @@
  *       src/sff/nblist.c:813-815 (pair list: r^2 < cut^2 at build time,
  *       not both frozen; returns the number of list entries, 2 per pair)
  *
- * mme34() (eff.c:2442) is their only caller on the energy path; INTEGRATION.md
- * shows the lines that replace the two calls with the functions below.
+ * mme34() (eff.c:2442) is their only caller on the energy path: it calls
+ * gbhip_nblist where it called nblist (eff.c:2572) and gbhip_egb where it
+ * called egb (eff.c:2506, :2704), after gbhip_set_coords.
  * Everything here is plain C: pointers, sizes and int status codes
  * (0 = success, negative = error, message via gbhip_last_error()).
  * No function calls exit().
