@@ -321,3 +321,19 @@ def test_step_split_matches_oracle(gpu, gen_dir, precision):
     assert ctx.split() == (8, 8, 8)
     ctx.close()
     orc.close()
+
+
+def test_default_step_split_by_size(gpu, gen_dir):
+    """The default step split per pass, from the tile-pair capacity: 8 in
+    every pass below 128 tile pairs, pass 1 at most 4 above, passes 2 and 3
+    split while they have fewer than 6 work items per CU, large systems
+    unsplit (the headline 25,000-atom workload runs the unsplit kernels)."""
+    na = gpu
+    want = {600: (8, 8, 8), 2500: (4, 2, 2), 7000: (4, 1, 1), 25000: (1, 1, 1)}
+    for natom, split in want.items():
+        parm, arr, x = generated_system(natom, 30, gen_dir)
+        ctx = na.GBContext(arr, x, gb=5, cut=20.0, rgbmax=20.0)
+        ctx.nblist()
+        nt, ntp = ctx.tile_stats()
+        assert ctx.split() == split, (natom, nt, ntp, ctx.split())
+        ctx.close()
