@@ -226,6 +226,7 @@ struct MDState {
     double dtx = 0, dt5 = 0, c_implic = 1, c_explic = 1, c_ave = 1, sdfac = 0, gammai = 0;
     double ekin0 = 0, dttp = 0, vlimit = 10, rndf = 1, boltz2 = 9.93595e-4;
     int nsnb = 10;
+    bool list_static = false; /* every pair stays within the cutoff: no rebuilds */
     unsigned long long seed = 0;
     hipGraphExec_t g_step = nullptr, g_rebuild = nullptr;
     hipGraph_t gr_step = nullptr, gr_rebuild = nullptr;

@@ -824,6 +824,22 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
     M.sdfac = sqrt(4.0 * M.gammai * o->boltz2 * o->temp0 / dtx);
     M.vlimit = o->vlimit;
     M.nsnb = o->nsnb;
+    /* A rebuild can change the list only if some pair crosses the cutoff.
+     * When the starting coordinates' bounding box, widened by 100 A of
+     * motion, fits inside the cutoff sphere (a molecule without a cutoff,
+     * cut = 999), every pair stays listed for the run and the rebuild steps
+     * would rebuild the same list: they are skipped (same results). */
+    {
+        double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
+        for (int a = 1; a < n; a++)
+            for (int c = 0; c < 3; c++) {
+                lo[c] = fmin(lo[c], x[3 * a + c]);
+                hi[c] = fmax(hi[c], x[3 * a + c]);
+            }
+        const double diag = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                 (hi[2] - lo[2]) * (hi[2] - lo[2])) + 100.0;
+        M.list_static = std::isfinite(diag) && diag * diag < E.par.cut2;
+    }
     M.seed = o->seed;
     M.steps_done = 0;
     const MDPar P = md_par(M, n3);
@@ -865,7 +881,8 @@ extern "C" int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph)
     }
     for (int k = 0; k < nsteps; k++) {
         const int step = M.steps_done + 1;
-        const bool rebuild = step % M.nsnb == 0; /* mme34: iter % nsnb == 0 (eff.c:2564) */
+        /* mme34: iter % nsnb == 0 (eff.c:2564), unless no pair can leave the list */
+        const bool rebuild = step % M.nsnb == 0 && !M.list_static;
         if (use_graph) HIPCHK(hipGraphLaunch(rebuild ? M.g_rebuild : M.g_step, E.stream));
         else if ((rc = md_step_launch(ctx, rebuild))) return rc;
         M.steps_done = step;

@@ -226,3 +226,26 @@ def test_md_writes_netcdf_trajectory(gpu, gen_dir, tmp_path):
     na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
     # a fresh initialisation closes the trajectory file
     assert na.lib.mme_init_sff(parm, None, None, None, None) == 0, na.lib.mm_get_error()
+
+
+def test_device_md_without_cutoff_follows_host_loop(gpu, gen_dir):
+    """Without a cutoff no pair can leave the list, so the device run skips
+    the nsnb rebuilds; the host loop of md() rebuilds through mme() every
+    nsnb steps, and the trajectories still agree (the rebuilt list is the
+    same list)."""
+    na = gpu
+    parm, arr, x0 = generated_system(600, 31, gen_dir)
+    n = 3 * arr["natom"]
+    rng = np.random.default_rng(7)
+    v0 = rng.normal(0, 0.05, n)
+    opts = "gb=5, cut=999.0, rgbmax=999.0, nsnb=3, gamma_ln=0.0, tempi=0.0, dt=0.001, ntpr_md=1000, ntpr=100000"
+    _init(na, parm, opts)
+    xa, va = x0.copy(), v0.copy()
+    _md(na, n, 25, xa, va)
+    _init(na, parm, opts)
+    xb, vb = x0.copy(), v0.copy()
+    wrap = na.MMEFUNC(lambda xp, fp, itp: na.lib.mme(xp, fp, itp))
+    _md(na, n, 25, xb, vb, func=wrap)
+    assert np.max(np.abs(xa - xb)) < 1e-8
+    assert rel_rms(va, vb) < 1e-8
+    assert np.max(np.abs(xa - x0)) > 1e-3
