@@ -321,6 +321,16 @@ int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double *x,
  * step with the pair-list rebuild every nsnb steps).
  */
 int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph);
+/*
+ * List-rebuild bookkeeping of the runs since gbhip_md_init: rebuild steps
+ * run (every nsnb steps, eff.c:2564), rebuild steps skipped because every
+ * pair provably stays within the cutoff (bounding-box diagonal plus the
+ * vlimit bound on motion, 2 sqrt(3) vlimit dt per step, below the cutoff:
+ * the rebuild would build the list it replaces), and bounding-box checks
+ * (one at gbhip_md_init, another whenever the bound runs out).  Any may be
+ * NULL.  No counterpart in the reference (a scheduling statistic).
+ */
+int gbhip_md_stats(gbhip_ctx *ctx, long long *rebuilds, long long *skipped, long long *bbox_checks);
 /* Current coordinates, velocities and gradient (any may be NULL); syncs. */
 int gbhip_md_state(gbhip_ctx *ctx, double *x, double *v, double *grad);
 /*

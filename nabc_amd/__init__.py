@@ -145,6 +145,7 @@ _sig("gbhip_timing_start", C.c_int, [vp, C.c_int])
 _sig("gbhip_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_md_init", C.c_int, [vp, dp, dp, dp, C.POINTER(MDOptions), C.c_int])
 _sig("gbhip_md_run", C.c_int, [vp, C.c_int, C.c_int])
+_sig("gbhip_md_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)])
 _sig("gbhip_md_state", C.c_int, [vp, dp, dp, dp])
 _sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
@@ -554,8 +555,9 @@ class GBContext:
         n = C.c_longlong()
         _check(lib.gbhip_get_pairs(self.h, _p(pi, ip), _p(pj, ip), cap, C.byref(n)))
         assert n.value == cap
-        p = np.stack([np.minimum(pi[:cap], pj[:cap]), np.maximum(pi[:cap], pj[:cap])], axis=1)
-        return p[np.lexsort((p[:, 1], p[:, 0]))]
+        key = np.minimum(pi[:cap], pj[:cap]).astype(np.int64) * self.natom + np.maximum(pi[:cap], pj[:cap])
+        key.sort()
+        return np.stack([key // self.natom, key % self.natom], axis=1)
 
 
 class Batch:

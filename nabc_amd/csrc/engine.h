@@ -226,7 +226,14 @@ struct MDState {
     double dtx = 0, dt5 = 0, c_implic = 1, c_explic = 1, c_ave = 1, sdfac = 0, gammai = 0;
     double ekin0 = 0, dttp = 0, vlimit = 10, rndf = 1, boltz2 = 9.93595e-4;
     int nsnb = 10;
-    bool list_static = false; /* every pair stays within the cutoff: no rebuilds */
+    /* Rebuild steps up to static_until can be skipped: every pair is within
+     * the cutoff at every step up to it (bounding-box diagonal at the last
+     * check + the largest separation growth per step, 2 sqrt(3) vlimit dtx,
+     * times the steps since), so a rebuild would build the list it replaces.
+     * static_growth > 0 while the bound can apply (vlimit > 0, finite). */
+    long long static_until = 0;
+    double static_growth = 0.0;
+    long long rebuilds_run = 0, rebuilds_skipped = 0, bbox_checks = 0;
     unsigned long long seed = 0;
     hipGraphExec_t g_step = nullptr, g_rebuild = nullptr;
     hipGraph_t gr_step = nullptr, gr_rebuild = nullptr;

@@ -746,6 +746,34 @@ static int md_step_launch(gbhip_ctx *c, bool rebuild)
     return mme_launch(c, E.x_dev, E.g_dev, E.ene14_dev, &P, rebuild);
 }
 
+/* bounding-box diagonal of x (3n, AoS) */
+static double bbox_diag(const double *x, int n)
+{
+    double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
+    for (int a = 1; a < n; a++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fmin(lo[c], x[3 * a + c]);
+            hi[c] = fmax(hi[c], x[3 * a + c]);
+        }
+    return sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                (hi[2] - lo[2]) * (hi[2] - lo[2]));
+}
+
+/* From coordinates x at step `step`: the last step up to which every pair
+ * stays strictly within the cutoff, given the per-step growth bound; steps
+ * from `step` on are covered while diag + (s - step) g < cut. */
+static void md_static_bound(const Engine &E, MDState &M, const double *x, long long step)
+{
+    M.bbox_checks++;
+    M.static_until = step;
+    if (M.static_growth <= 0.0) return;
+    const double diag = bbox_diag(x, E.natoms), cut = sqrt(E.par.cut2);
+    if (!std::isfinite(diag) || !(diag < cut)) return;
+    long long k = (long long)floor((cut - diag) / M.static_growth);
+    while (k > 0 && !(diag + (double)k * M.static_growth < cut)) k--;
+    M.static_until = step + k;
+}
+
 static int md_capture(gbhip_ctx *c, bool rebuild, hipGraph_t *g, hipGraphExec_t *ge)
 {
     Engine &E = c->E;
@@ -825,21 +853,17 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
     M.vlimit = o->vlimit;
     M.nsnb = o->nsnb;
     /* A rebuild can change the list only if some pair crosses the cutoff.
-     * When the starting coordinates' bounding box, widened by 100 A of
-     * motion, fits inside the cutoff sphere (a molecule without a cutoff,
-     * cut = 999), every pair stays listed for the run and the rebuild steps
-     * would rebuild the same list: they are skipped (same results). */
-    {
-        double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
-        for (int a = 1; a < n; a++)
-            for (int c = 0; c < 3; c++) {
-                lo[c] = fmin(lo[c], x[3 * a + c]);
-                hi[c] = fmax(hi[c], x[3 * a + c]);
-            }
-        const double diag = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
-                                 (hi[2] - lo[2]) * (hi[2] - lo[2])) + 100.0;
-        M.list_static = std::isfinite(diag) && diag * diag < E.par.cut2;
-    }
+     * Each step moves a coordinate by at most vlimit * dtx (the clamp of
+     * the drift), so a pair's separation grows by at most
+     * 2 sqrt(3) vlimit dtx per step; while the bounding-box diagonal plus
+     * that growth stays below the cutoff, every pair is listed and a
+     * rebuild builds the list it replaces: such rebuild steps are skipped
+     * (same results).  gbhip_md_run checks the box again when the bound
+     * runs out. */
+    M.static_growth = (o->vlimit > 0.0 && std::isfinite(o->vlimit)) ? 2.0 * sqrt(3.0) * o->vlimit * dtx : 0.0;
+    M.rebuilds_run = M.rebuilds_skipped = M.bbox_checks = 0;
+    M.static_until = 0;
+    md_static_bound(E, M, x, 0);
     M.seed = o->seed;
     M.steps_done = 0;
     const MDPar P = md_par(M, n3);
@@ -879,14 +903,40 @@ extern "C" int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph)
         if ((rc = md_capture(ctx, false, &M.gr_step, &M.g_step))) return rc;
         if ((rc = md_capture(ctx, true, &M.gr_rebuild, &M.g_rebuild))) return rc;
     }
+    bool recheck = M.static_growth > 0.0;
     for (int k = 0; k < nsteps; k++) {
         const int step = M.steps_done + 1;
         /* mme34: iter % nsnb == 0 (eff.c:2564), unless no pair can leave the list */
-        const bool rebuild = step % M.nsnb == 0 && !M.list_static;
+        bool rebuild = step % M.nsnb == 0;
+        if (rebuild && step > M.static_until && recheck) {
+            /* the bound ran out: measure the box of the current coordinates
+             * (the steps queued so far finish first); once it no longer
+             * holds, this run rebuilds as the reference does */
+            std::vector<double> xh(3 * (size_t)E.natoms);
+            HIPCHK(hipMemcpyAsync(xh.data(), E.x_dev, xh.size() * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+            HIPCHK(hipStreamSynchronize(E.stream));
+            md_static_bound(E, M, xh.data(), step - 1);
+            recheck = M.static_until > step - 1;
+        }
+        if (rebuild && step <= M.static_until) {
+            rebuild = false;
+            M.rebuilds_skipped++;
+        } else if (rebuild) {
+            M.rebuilds_run++;
+        }
         if (use_graph) HIPCHK(hipGraphLaunch(rebuild ? M.g_rebuild : M.g_step, E.stream));
         else if ((rc = md_step_launch(ctx, rebuild))) return rc;
         M.steps_done = step;
     }
+    return 0;
+}
+
+extern "C" int gbhip_md_stats(gbhip_ctx *ctx, long long *rebuilds, long long *skipped, long long *bbox_checks)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (rebuilds) *rebuilds = ctx->M.rebuilds_run;
+    if (skipped) *skipped = ctx->M.rebuilds_skipped;
+    if (bbox_checks) *bbox_checks = ctx->M.bbox_checks;
     return 0;
 }
 

@@ -43,6 +43,17 @@ _sig("orc_log_get", None, [C.c_int, ip, dp, dp])
 _sig("orc_log_clear", None, [])
 _sig("orc_set_threads", None, [C.c_int])
 
+
+class MDOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("temp0", C.c_double), ("tempi", C.c_double),
+                ("gamma_ln", C.c_double), ("tautp", C.c_double), ("vlimit", C.c_double),
+                ("boltz2", C.c_double), ("nsnb", C.c_int), ("zerov", C.c_int),
+                ("seed", C.c_ulonglong)]
+
+
+_sig("orc_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])
+_sig("orc_md", C.c_int, [vp, C.c_int, dp, dp, dp, C.POINTER(MDOpts), dp])
+
 MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
 MME_CB = MMEFUNC(("orc_mme_cb", lib))
 
@@ -98,6 +109,20 @@ class Oracle:
         frms = C.c_double()
         lib.orc_mme(self.h, _p(x), _p(g), 1 if rebuild else 0, _p(ene), C.byref(frms))
         return ene, frms.value, g
+
+    def md(self, x, v, masses, nsteps, dt=0.001, temp0=300.0, tempi=0.0, gamma_ln=0.0,
+           tautp=999999.0, vlimit=10.0, nsnb=10, zerov=0, seed=71277):
+        """md() (sff.c:2071-2366) for nsteps from (x, v): returns (x, v, rows),
+        rows[k] = ene[0..12], frms, ekin, temperature of step k+1."""
+        x = np.array(x, dtype=np.float64)
+        v = np.zeros(3 * self.n) if v is None else np.array(v, dtype=np.float64)
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        rows = np.zeros((max(1, nsteps), 16))
+        o = MDOpts(dt, temp0, tempi, gamma_ln, tautp, vlimit, 9.93595e-4, nsnb, zerov, seed)
+        rc = lib.orc_md(self.h, nsteps, _p(x), _p(v), _p(m), C.byref(o), _p(rows))
+        if rc != 0:
+            raise MemoryError("orc_md")
+        return x, v, rows[:nsteps]
 
     def gbparams(self):
         n = self.n

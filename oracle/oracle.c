@@ -770,3 +770,129 @@ void orc_log_get(int k, int *iter, double *ene13, double *frms)
 }
 
 void orc_log_clear(void) { log_n = 0; }
+
+/* ------------------------------------------------------------------ md() */
+
+/*
+ * Langevin noise: N(0,1) of (seed, step, coordinate index) from a counter
+ * hash (splitmix64 finaliser of seed, step and index; Box-Muller of two
+ * 53-bit uniforms).  This is the noise stream the product documents in
+ * place of rand2.c's serial generator (the one deviation of its md());
+ * restated here from that description so the oracle's dynamics draw the
+ * same deviates: initial velocities at step -1, the step-0 accelerations
+ * at step 0, step k's kick at step k.
+ */
+static unsigned long long o_mix(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+double orc_gauss(unsigned long long seed, long long step, long long idx)
+{
+    unsigned long long h = o_mix(seed ^ o_mix((unsigned long long)step * 0x100000001B3ULL ^ o_mix((unsigned long long)idx)));
+    unsigned long long h2 = o_mix(h);
+    double u1 = ((double)(h >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+    double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/*
+ * md(), sff.c:2071-2366, over orc_mme with mme34's rebuild cadence (the
+ * list is built at the first call and when the step number is a positive
+ * multiple of nsnb, eff.c:2564).  Constants sff.c:2140-2155; initial
+ * velocities sff.c:2165-2188 (zerov, tempi > 0, or the given v with the
+ * reference's v*v*minv kinetic sum); step-0 accelerations sff.c:2199-2215;
+ * the loop sff.c:2223-2352: Berendsen scaling (gamma_ln = 0) or the
+ * Langevin half kicks, the vlimit clamp, frozen atoms held.  No RATTLE, no
+ * centre-of-mass removal (nscm = 0, the reference's default, sff.c:76).
+ * rows (16 per step, may be NULL): ene[0..12], frms, ekin, temperature.
+ */
+int orc_md(orc_t *o, int maxstep, double *x, double *v, const double *masses, const orc_md_opts *mo,
+           double *rows)
+{
+    const int n = 3 * o->n, dim = 3;
+    double *minv = (double *)calloc(n, sizeof(double));
+    double *accel = (double *)calloc(n, sizeof(double));
+    double *sqrmass = (double *)calloc(n, sizeof(double));
+    double *f = (double *)calloc(n, sizeof(double));
+    double dtx = mo->dt * 20.455, dt5 = 0.5 * dtx, rndf, ekin0, dttp, gammai, c_implic, c_explic, c_ave, sdfac;
+    double ekin = 0.0, tscal, ene[13], frms;
+    int i, nstep, nfrozen = 0, nsnb = mo->nsnb > 0 ? mo->nsnb : 1;
+    if (!minv || !accel || !sqrmass || !f) {
+        free(minv); free(accel); free(sqrmass); free(f);
+        return -1;
+    }
+    for (i = 0; i < o->n; i++) nfrozen += o->frozen[i] != 0;
+    rndf = n - 3 * nfrozen;
+    ekin0 = mo->boltz2 * rndf * mo->temp0;
+    dttp = mo->dt / mo->tautp;
+    gammai = mo->gamma_ln / 20.455;
+    c_implic = 1.0 / (1.0 + gammai * dt5);
+    c_explic = 1.0 - gammai * dt5;
+    c_ave = 1.0 + gammai * dt5;
+    sdfac = sqrt(4.0 * gammai * mo->boltz2 * mo->temp0 / dtx);
+    for (i = 0; i < n; i++) minv[i] = 1.0 / masses[i / dim];
+    if (mo->zerov) {
+        for (i = 0; i < n; i++) v[i] = 0.0;
+    } else if (mo->tempi > 0.0) {
+        for (i = 0; i < n; i++) {
+            if (o->frozen[i / dim]) {
+                v[i] = 0.0;
+            } else {
+                v[i] = sqrt(2.0 * mo->boltz2 * mo->tempi * minv[i]) * orc_gauss(mo->seed, -1, i);
+                ekin += v[i] * v[i] / minv[i];
+            }
+        }
+        ekin *= 0.5;
+    } else {
+        for (i = 0; i < n; i++) ekin += v[i] * v[i] * minv[i];
+        ekin *= 0.5;
+    }
+    orc_mme(o, x, f, 0, ene, &frms); /* nstep = 0: builds the list when there is none */
+    for (i = 0; i < n; i++) {
+        if (gammai == 0.0) {
+            accel[i] = -f[i] * minv[i] * dt5;
+        } else if (o->frozen[i / dim]) {
+            accel[i] = 0.0;
+        } else {
+            sqrmass[i] = 1.0 / sqrt(minv[i]);
+            accel[i] = (-f[i] + sdfac * sqrmass[i] * orc_gauss(mo->seed, 0, i)) * minv[i] * dt5;
+        }
+    }
+    for (nstep = 1; nstep <= maxstep; nstep++) {
+        tscal = ekin > 0.01 ? sqrt(1.0 + dttp * (ekin0 / ekin - 1.0)) : 1.0;
+        ekin = 0.0;
+        for (i = 0; i < n; i++) {
+            if (gammai == 0.0) v[i] = (v[i] + accel[i]) * tscal;
+            else v[i] = c_explic * v[i] + accel[i];
+            v[i] = v[i] > mo->vlimit ? mo->vlimit : v[i];
+            v[i] = v[i] < -mo->vlimit ? -mo->vlimit : v[i];
+            x[i] += v[i] * dtx;
+        }
+        orc_mme(o, x, f, nstep % nsnb == 0, ene, &frms);
+        for (i = 0; i < n; i++) {
+            if (gammai == 0.0) {
+                accel[i] = -f[i] * minv[i] * dt5;
+                v[i] = (v[i] + accel[i]) * tscal;
+            } else if (!o->frozen[i / dim]) {
+                accel[i] = (-f[i] + sdfac * sqrmass[i] * orc_gauss(mo->seed, nstep, i)) * minv[i] * dt5;
+                v[i] = (v[i] + accel[i]) * c_implic;
+            }
+        }
+        for (i = 0; i < n; i++) ekin += v[i] * v[i] / minv[i];
+        ekin *= 0.5;
+        if (gammai > 0.0) ekin *= c_ave;
+        if (rows) {
+            double *r = rows + 16 * (size_t)(nstep - 1);
+            memcpy(r, ene, sizeof(ene));
+            r[13] = frms;
+            r[14] = ekin;
+            r[15] = ekin / (mo->boltz2 * rndf);
+        }
+    }
+    free(minv); free(accel); free(sqrmass); free(f);
+    return 0;
+}

@@ -13,6 +13,7 @@
  *   orc_egb      egb() passes 1-3 (eff.c:1363-2417)
  *   orc_mme      mme34() assembly (eff.c:2442-2947) with ebond/eangl/ephi
  *                (eff.c:82-627) and the 1-4 nbond() (eff.c:896-1331)
+ *   orc_md       md() (sff.c:2071-2366) over orc_mme
  */
 #ifndef NABC_AMD_ORACLE_H
 #define NABC_AMD_ORACLE_H
@@ -60,6 +61,22 @@ double orc_mme_cb(double *x, double *f, int *iter);
 int orc_log_count(void);
 void orc_log_get(int k, int *iter, double *ene13, double *frms);
 void orc_log_clear(void);
+
+/*
+ * md(), sff.c:2071-2366, over orc_mme (rebuild cadence of mme34: first
+ * call, then step % nsnb == 0).  Langevin noise from orc_gauss, the
+ * counter-based stream the product uses in place of rand2.c.  x, v (3N)
+ * are updated; rows (16 doubles per step, may be NULL) receive ene[0..12],
+ * frms, ekin and the temperature of each step.
+ */
+typedef struct {
+    double dt, temp0, tempi, gamma_ln, tautp, vlimit, boltz2;
+    int nsnb, zerov;
+    unsigned long long seed;
+} orc_md_opts;
+double orc_gauss(unsigned long long seed, long long step, long long idx);
+int orc_md(orc_t *o, int maxstep, double *x, double *v, const double *masses,
+           const orc_md_opts *mo, double *rows);
 
 /* threads used by the OpenMP loops (0 = library default) */
 void orc_set_threads(int n);

@@ -50,3 +50,20 @@ def rel_rms(a, b):
 
 def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
+
+
+def oracle_pairs(orc, natom):
+    """The oracle's current list as a sorted (P, 2) array of (i, j), i < j:
+    the upper part (j > i) of each atom's list (nblist.c:1146-1147)."""
+    lo, up, flat = orc.lists()
+    lo = lo.astype(np.int64)
+    up = up.astype(np.int64)
+    start = np.concatenate(([0], np.cumsum(lo + up)[:-1]))
+    cum_up = np.concatenate(([0], np.cumsum(up)[:-1]))
+    i = np.repeat(np.arange(natom, dtype=np.int64), up)
+    pos = np.repeat(start + lo - cum_up, up) + np.arange(int(up.sum()), dtype=np.int64)
+    j = flat[pos].astype(np.int64)
+    assert np.all(j > i)
+    key = i * natom + j
+    key.sort()
+    return np.stack([key // natom, key % natom], axis=1)

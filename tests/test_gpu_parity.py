@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import GOLDEN, generated_system, golden, read_pdb_xyz, rel, rel_rms
+from common import GOLDEN, generated_system, golden, oracle_pairs, read_pdb_xyz, rel, rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -29,14 +29,7 @@ def _compare(na, parm, arr, x, gb, cut, rgbmax, kappa=0.0, precision=0, frozen=N
     r_nent = orc.nblist(x)
     assert nent == r_nent
     if check_list:
-        lo, up, flat = orc.lists()
-        ref = []
-        off = 0
-        for i in range(arr["natom"]):
-            js = flat[off:off + lo[i] + up[i]]
-            off += lo[i] + up[i]
-            ref.extend((i, j) for j in js if j > i)
-        ref = np.array(ref, dtype=np.int64).reshape(-1, 2)
+        ref = oracle_pairs(orc, arr["natom"])
         got = ctx.pairs()
         assert got.shape == ref.shape and np.array_equal(got, ref)
     epol, evdw, eel, g = ctx.egb()

@@ -231,3 +231,11 @@ def test_pair_list_brute_force():
         off += lo[i] + up[i]
         assert list(js[:lo[i]]) == sorted(np.nonzero(keep[i, :i])[0].tolist())
         assert list(js[lo[i]:]) == sorted((np.nonzero(keep[i, i + 1:])[0] + i + 1).tolist())
+    # the vectorised extraction the GPU parity tests compare lists with
+    from common import oracle_pairs
+    o = oracle.Oracle(p, 300, gb=5, cut=8.0, rgbmax=8.0, frozen=frozen)
+    o.nblist(x)
+    got = oracle_pairs(o, 300)
+    o.close()
+    iu, ju = np.nonzero(np.triu(keep, 1))
+    assert np.array_equal(got, np.stack([iu, ju], 1))
