@@ -3,9 +3,11 @@
  * GB engine, so one energy + gradient never leaves the GPU.
  *
  *   d_bond    ebond, eff.c:82-168    E = Rk (r - Req)^2
- *   d_angle   eangl, eff.c:168-316   E = Tk (theta - Teq)^2, sin clamp 1e-3
+ *   d_angle   eangl, eff.c:168-316   E = Tk (theta - Teq)^2
  *   d_dihed   ephi,  eff.c:316-627   Fourier terms; a series (Pn < 0) is
  *             expanded on the host into one entry per term
+ *   (angle and torsion gradients from the geometry of the plane normals,
+ *   see each function)
  *   d_nb14    nbond(N14=1), eff.c:896-1331 (diel C or R), scaled by the
  *             per-pair 1/SCEE and 1/SCNB from the prmtop
  * all four in one launch (k_bonded, a block range per term type).  One
@@ -72,129 +74,113 @@ __device__ __forceinline__ void d_bond(const BondedDev &B, const double *__restr
     block_add(e, &B.ene[3]);
 }
 
+__device__ __forceinline__ double3 cross3(double3 a, double3 b)
+{
+    return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double3 sub3(const double *__restrict__ x, int i, int j)
+{
+    return make_double3(x[i] - x[j], x[i + 1] - x[j + 1], x[i + 2] - x[j + 2]);
+}
+__device__ __forceinline__ void add_grad(double *g, int i, double s, double3 v)
+{
+    unsafeAtomicAdd(&g[i], s * v.x);
+    unsafeAtomicAdd(&g[i + 1], s * v.y);
+    unsafeAtomicAdd(&g[i + 2], s * v.z);
+}
+
+/*
+ * Bond angle i-j-k, E = Tk (theta - Teq)^2.  With u = ri - rj, v = rk - rj
+ * and w = u x v (|w| = |u||v| sin theta), theta = atan2(|w|, u.v), and
+ * moving ri (rk) changes theta only through its component perpendicular to
+ * u (v) in the plane of the angle:
+ *   d theta / d ri = (u x w) / (|u|^2 |w|),  d theta / d rk = (w x v) / (|v|^2 |w|),
+ *   d theta / d rj = -(d theta / d ri + d theta / d rk)  (translation).
+ * The reference bounds sin theta below by 1e-3 in the gradient
+ * (eff.c:270-276), which is |w| >= 1e-3 |u||v| here.
+ */
 __device__ __forceinline__ void d_angle(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
     const int t = blk * MM_THREADS + threadIdx.x;
     double e = 0.0;
     if (t < B.na) {
-        const int i1 = 3 * B.a_i[t], i2 = 3 * B.a_j[t], i3 = 3 * B.a_k[t];
-        const double dxi = x[i1] - x[i2], dyi = x[i1 + 1] - x[i2 + 1], dzi = x[i1 + 2] - x[i2 + 2];
-        const double dxj = x[i3] - x[i2], dyj = x[i3 + 1] - x[i2 + 1], dzj = x[i3 + 2] - x[i2 + 2];
-        const double rir = 1.0 / sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
-        const double rjr = 1.0 / sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
-        const double dxir = dxi * rir, dyir = dyi * rir, dzir = dzi * rir;
-        const double dxjr = dxj * rjr, dyjr = dyj * rjr, dzjr = dzj * rjr;
-        double cst = dxir * dxjr + dyir * dyjr + dzir * dzjr;
-        cst = fmin(1.0, fmax(-1.0, cst));
-        const double th = acos(cst);
-        const double da = th - B.a_teq[t];
-        double df = da * B.a_tk[t];
-        e = df * da;
-        df = df + df;
-        double sn = sin(th);
-        if (sn > 0 && sn < 1.e-3) sn = 1.e-3;
-        else if (sn < 0 && sn > -1.e-3) sn = -1.e-3;
-        df = -df / sn;
-        const double xt = df * rir * (dxjr - cst * dxir), dxt = df * rjr * (dxir - cst * dxjr);
-        const double yt = df * rir * (dyjr - cst * dyir), dyt = df * rjr * (dyir - cst * dyjr);
-        const double zt = df * rir * (dzjr - cst * dzir), dzt = df * rjr * (dzir - cst * dzjr);
-        unsafeAtomicAdd(&g[i1], xt);
-        unsafeAtomicAdd(&g[i1 + 1], yt);
-        unsafeAtomicAdd(&g[i1 + 2], zt);
-        unsafeAtomicAdd(&g[i3], dxt);
-        unsafeAtomicAdd(&g[i3 + 1], dyt);
-        unsafeAtomicAdd(&g[i3 + 2], dzt);
-        unsafeAtomicAdd(&g[i2], -(xt + dxt));
-        unsafeAtomicAdd(&g[i2 + 1], -(yt + dyt));
-        unsafeAtomicAdd(&g[i2 + 2], -(zt + dzt));
+        const int i = 3 * B.a_i[t], j = 3 * B.a_j[t], k = 3 * B.a_k[t];
+        const double3 u = sub3(x, i, j), v = sub3(x, k, j);
+        const double3 w = cross3(u, v);
+        const double uu = dot3(u, u), vv = dot3(v, v), wn = sqrt(dot3(w, w));
+        const double theta = atan2(wn, dot3(u, v));
+        const double dth = theta - B.a_teq[t];
+        e = B.a_tk[t] * dth * dth;
+        const double de = 2.0 * B.a_tk[t] * dth / fmax(wn, 1.e-3 * sqrt(uu * vv));
+        const double3 gi = cross3(u, w), gk = cross3(w, v);
+        const double si = de / uu, sk = de / vv;
+        add_grad(g, i, si, gi);
+        add_grad(g, k, sk, gk);
+        add_grad(g, j, -1.0, make_double3(si * gi.x + sk * gk.x, si * gi.y + sk * gk.y, si * gi.z + sk * gk.z));
     }
     block_add(e, &B.ene[4]);
 }
 
 /*
- * One Fourier term of a torsion.  The derivative of cos(phi) with respect
- * to the four atoms is formed from the dot products of the bond vectors
- * a = r2-r1, b = r3-r2, c = r4-r3 (the same decomposition as eff.c:384-461):
- * cos(phi) = uv / sqrt(uu vv) with uu = |a|^2|b|^2 - (a.b)^2,
- * vv = |b|^2|c|^2 - (b.c)^2, uv = (a.b)(b.c) - (a.c)|b|^2.
+ * One Fourier term of the torsion i-j-k-l, E = Pk (1 + cos(n phi - gamma)).
+ * Following Blondel & Karplus (J. Comput. Chem. 17:1132, 1996): with
+ * F = ri - rj, G = rj - rk, H = rl - rk and the plane normals A = F x G,
+ * B = H x G, phi = atan2((B x A).G / |G|, A.B) (the sign convention of
+ * eff.c:582-590: positive when (a x b) x (b x c) points along b), and
+ *   d phi / d ri = -|G| A / |A|^2,      d phi / d rl = |G| B / |B|^2,
+ *   d phi / d rj = (|G| + F.G/|G|) A / |A|^2 - (H.G/|G|) B / |B|^2,
+ *   d phi / d rk = -|G| B / |B|^2 - (F.G/|G|) A / |A|^2 + (H.G/|G|) B / |B|^2,
+ * no division by sin phi anywhere.  Energies as the reference evaluates
+ * them: a phase of 0 or pi (within 0.01) is the exact sign +-1 and a
+ * multiplicity 1 to 6 (eff.c:600-627, others give 0); other phases use the
+ * signed Pn as stored (eff.c:591-599).  For those the reference divides by
+ * sin phi and replaces 1/sin phi by +-1000 where |sin phi| <= 0.001
+ * (eff.c:594-598); the factor `scale` below reproduces that gradient.
+ * Collinear i-j-k or j-k-l: cos phi taken as 0 with no gradient, as the
+ * reference does (eff.c:412).
  */
 __device__ __forceinline__ void d_dihed(const BondedDev &B, const double *__restrict__ x, double *g, int blk)
 {
     const int t = blk * MM_THREADS + threadIdx.x;
     double e = 0.0;
     if (t < B.nd) {
-        const int i1 = 3 * B.d_i[t], i2 = 3 * B.d_j[t], i3 = 3 * B.d_k[t], i4 = 3 * B.d_l[t];
-        const double ax = x[i2] - x[i1], ay = x[i2 + 1] - x[i1 + 1], az = x[i2 + 2] - x[i1 + 2];
-        const double bx = x[i3] - x[i2], by = x[i3 + 1] - x[i2 + 1], bz = x[i3 + 2] - x[i2 + 2];
-        const double cx = x[i4] - x[i3], cy = x[i4 + 1] - x[i3 + 1], cz = x[i4 + 2] - x[i3 + 2];
-        const double ab = ax * bx + ay * by + az * bz, bc = bx * cx + by * cy + bz * cz;
-        const double ac = ax * cx + ay * cy + az * cz, aa = ax * ax + ay * ay + az * az;
-        const double bb = bx * bx + by * by + bz * bz, cc = cx * cx + cy * cy + cz * cz;
-        const double uu = aa * bb - ab * ab, vv = bb * cc - bc * bc, uv = ab * bc - ac * bb;
-        const double den = uu == 0.0 ? 0.0 : 1.0 / sqrt(uu * vv);
-        double co = uv * den;
-        const double co1 = 0.5 * co * den;
-        /* d cos(phi) / d r_k = dd_k */
-        const double a0x = -bc * bx + bb * cx, a0y = -bc * by + bb * cy, a0z = -bc * bz + bb * cz;
-        const double b0x = ab * cx + bc * ax - 2. * ac * bx, b0y = ab * cy + bc * ay - 2. * ac * by;
-        const double b0z = ab * cz + bc * az - 2. * ac * bz;
-        const double c0x = ab * bx - bb * ax, c0y = ab * by - bb * ay, c0z = ab * bz - bb * az;
-        const double a1x = 2. * uu * (-cc * bx + bc * cx), a1y = 2. * uu * (-cc * by + bc * cy);
-        const double a1z = 2. * uu * (-cc * bz + bc * cz);
-        const double b1x = 2. * uu * (bb * cx - bc * bx), b1y = 2. * uu * (bb * cy - bc * by);
-        const double b1z = 2. * uu * (bb * cz - bc * bz);
-        const double a2x = -2. * vv * (bb * ax - ab * bx), a2y = -2. * vv * (bb * ay - ab * by);
-        const double a2z = -2. * vv * (bb * az - ab * bz);
-        const double b2x = 2. * vv * (aa * bx - ab * ax), b2y = 2. * vv * (aa * by - ab * ay);
-        const double b2z = 2. * vv * (aa * bz - ab * az);
-        const double d1x = (a0x - a2x * co1) * den, d1y = (a0y - a2y * co1) * den, d1z = (a0z - a2z * co1) * den;
-        const double d2x = (-a0x - b0x - (a1x - a2x - b2x) * co1) * den;
-        const double d2y = (-a0y - b0y - (a1y - a2y - b2y) * co1) * den;
-        const double d2z = (-a0z - b0z - (a1z - a2z - b2z) * co1) * den;
-        const double d3x = (b0x - c0x - (-a1x - b1x + b2x) * co1) * den;
-        const double d3y = (b0y - c0y - (-a1y - b1y + b2y) * co1) * den;
-        const double d3z = (b0z - c0z - (-a1z - b1z + b2z) * co1) * den;
-        const double d4x = (c0x - b1x * co1) * den, d4y = (c0y - b1y * co1) * den, d4z = (c0z - b1z * co1) * den;
+        const int i = 3 * B.d_i[t], j = 3 * B.d_j[t], k = 3 * B.d_k[t], l = 3 * B.d_l[t];
+        const double3 F = sub3(x, i, j), G = sub3(x, j, k), H = sub3(x, l, k);
+        const double3 A = cross3(F, G), Bn = cross3(H, G);
+        const double aa = dot3(A, A), bb = dot3(Bn, Bn), gn = sqrt(dot3(G, G));
+        const bool degenerate = aa * bb == 0.0;
+        const double phi = degenerate ? 1.5707963267948966 : atan2(dot3(cross3(Bn, A), G) / gn, dot3(A, Bn));
         const double pk = B.d_pk[t], pn = B.d_pn[t], ph = B.d_phase[t];
-        double df;
+        double dedphi;
         if (fabs(ph - 3.142) > 0.01 && fabs(ph) > 0.01) {
-            /* general phase: signed phi from the sign of (u x v).b */
-            co = fmin(1.0, fmax(-1.0, co));
-            double phi = acos(co);
-            const double ux = ay * bz - az * by, uy = az * bx - ax * bz, uz = ax * by - ay * bx;
-            const double vx = by * cz - bz * cy, vy = bz * cx - bx * cz, vz = bx * cy - by * cx;
-            const double sx = uy * vz - uz * vy, sy = uz * vx - ux * vz, sz = ux * vy - uy * vx;
-            if (sx * bx + sy * by + sz * bz < 0.0) phi = -phi;
             const double delta = pn * phi - ph;
             e = pk * (1.0 + cos(delta));
-            const double yy = sin(phi);
-            if (fabs(yy) > 0.001) df = pk * pn * sin(delta) / yy;
-            else if (phi > -1.0 && phi < 1.0) df = pk * pn * sin(delta) * 1000.;
-            else df = -pk * pn * sin(delta) * 1000.;
+            const double sp = sin(phi);
+            const double scale = fabs(sp) > 0.001 ? 1.0 : ((phi > -1.0 && phi < 1.0) ? 1000.0 : -1000.0) * sp;
+            dedphi = -pk * pn * sin(delta) * scale;
         } else {
-            /* phase 0 or pi: polynomials in cos(phi) */
-            const double phase = fabs(ph - 3.142) < 0.01 ? -1.0 : 1.0;
-            const double c2 = co * co;
-            switch ((int)fabs(pn)) {
-            case 1: e = pk * (1.0 + phase * co); df = phase * pk; break;
-            case 2: e = pk * (1.0 + phase * (2. * c2 - 1.)); df = phase * pk * 4. * co; break;
-            case 3: e = pk * (1.0 + phase * co * (4. * c2 - 3.)); df = phase * pk * (12. * c2 - 3.); break;
-            case 4: e = pk * (1.0 + phase * (8. * c2 * (c2 - 1.) + 1.)); df = phase * pk * co * (32. * c2 - 16.); break;
-            case 5:
-                e = pk * (1.0 + phase * (16. * c2 * c2 * co - 20. * c2 * co + 5. * co));
-                df = phase * pk * (80. * c2 * c2 - 60. * c2 + 5.);
-                break;
-            case 6:
-                e = pk * (1.0 + phase * (32. * c2 * c2 * c2 - 48. * c2 * c2 + 18. * c2 - 1.));
-                df = phase * pk * co * (192. * c2 * c2 - 192. * c2 + 36.);
-                break;
-            default: e = 0.0; df = 0.0;
+            const double sgn = fabs(ph - 3.142) < 0.01 ? -1.0 : 1.0;
+            const int n = (int)fabs(pn);
+            if (n >= 1 && n <= 6) {
+                e = pk * (1.0 + sgn * cos(n * phi));
+                dedphi = -pk * sgn * n * sin(n * phi);
+            } else {
+                e = 0.0;
+                dedphi = 0.0;
             }
         }
-        unsafeAtomicAdd(&g[i1], df * d1x); unsafeAtomicAdd(&g[i1 + 1], df * d1y); unsafeAtomicAdd(&g[i1 + 2], df * d1z);
-        unsafeAtomicAdd(&g[i2], df * d2x); unsafeAtomicAdd(&g[i2 + 1], df * d2y); unsafeAtomicAdd(&g[i2 + 2], df * d2z);
-        unsafeAtomicAdd(&g[i3], df * d3x); unsafeAtomicAdd(&g[i3 + 1], df * d3y); unsafeAtomicAdd(&g[i3 + 2], df * d3z);
-        unsafeAtomicAdd(&g[i4], df * d4x); unsafeAtomicAdd(&g[i4 + 1], df * d4y); unsafeAtomicAdd(&g[i4 + 2], df * d4z);
+        if (!degenerate) {
+            const double ga = dedphi * gn / aa, gb = dedphi * gn / bb;
+            const double fg = dot3(F, G) / gn, hg = dot3(H, G) / gn;
+            const double ca = dedphi * fg / aa, cb = dedphi * hg / bb;
+            add_grad(g, i, -ga, A);
+            add_grad(g, l, gb, Bn);
+            add_grad(g, j, 1.0, make_double3((ga + ca) * A.x - cb * Bn.x, (ga + ca) * A.y - cb * Bn.y,
+                                             (ga + ca) * A.z - cb * Bn.z));
+            add_grad(g, k, 1.0, make_double3((cb - gb) * Bn.x - ca * A.x, (cb - gb) * Bn.y - ca * A.y,
+                                             (cb - gb) * Bn.z - ca * A.z));
+        }
     }
     block_add(e, &B.ene[5]);
 }

@@ -330,3 +330,70 @@ def test_default_step_split_by_size(gpu, gen_dir):
         nt, ntp = ctx.tile_stats()
         assert ctx.split() == split, (natom, nt, ntp, ctx.split())
         ctx.close()
+
+
+TERM_INDEX = {"bond": 3, "angle": 4, "dihedral": 5, "enb14": 7, "eel14": 8, "enb": 1,
+              "eel": 2, "egb": 10, "econs": 9, "esurf": 11, "Total": 0}
+
+
+def test_aspgb8_mme_terms_on_gpu(gpu):
+    """The product's whole mme() (bonded, 1-4 and GB terms on the device)
+    reproduces every energy term and frms of test/nab/aspgb8.out at its
+    printed 9 decimals."""
+    na = gpu
+    g = golden()["aspgb8"]
+    parm = na.read_parm(GOLDEN + "/asp.top")
+    arr = na.parm_arrays(parm)
+    x, _ = read_pdb_xyz(GOLDEN + "/asp.pdb")
+    na.lib.mm_reset_options()
+    assert na.lib.mm_options(b"gb=8, cut=100.0, rgbmax=100.0, kappa=0.0, diel=C, ntpr=100000") == 0
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+    f = np.zeros(3 * arr["natom"])
+    xc = np.ascontiguousarray(x)
+    it = na.C.c_int(-2)
+    na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+    ene = np.zeros(13)
+    frms = na.C.c_double()
+    na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(frms))
+    for name, t in g["terms"].items():
+        got = frms.value if name == "frms" else ene[TERM_INDEX[name]]
+        assert abs(got - t["value"]) <= 6e-10 + 1e-12 * abs(t["value"]), (name, got, t)
+    it = na.C.c_int(-3)
+    na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+
+
+def test_general_torsion_phases_match_oracle(gpu, gen_dir):
+    """Torsion terms whose phase is neither 0 nor pi take the general branch
+    (E = Pk (1 + cos(Pn phi - phase)) with the signed Pn of a series,
+    eff.c:591-599): the generated topology's four torsion types are given
+    phases 0.7, 2.0, -1.1 and pi, and the device's mme() matches the
+    oracle's mme34 per term and in the gradient to 1e-9."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 9, gen_dir)
+    s = parm.contents
+    saved = [s.Phase[k] for k in range(4)]
+    for k, ph in enumerate((0.7, 2.0, -1.1, 3.14159400)):
+        s.Phase[k] = ph
+    try:
+        na.lib.mm_reset_options()
+        assert na.lib.mm_options(b"gb=5, cut=16.0, rgbmax=16.0, kappa=0.1, diel=C, ntpr=100000") == 0
+        assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+        f = np.zeros(3 * arr["natom"])
+        xc = np.ascontiguousarray(x)
+        it = na.C.c_int(-2)
+        e0 = na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+        ene = np.zeros(13)
+        frms = na.C.c_double()
+        na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(frms))
+        orc = oracle.Oracle(parm, arr["natom"], gb=5, cut=16.0, rgbmax=16.0, kappa=0.1)
+        r_ene, r_frms, r_g = orc.mme(x)
+        orc.close()
+        assert rel(e0, r_ene[0]) <= 1e-9
+        for k in (3, 4, 5, 7, 8):
+            assert abs(ene[k] - r_ene[k]) <= 1e-9 * max(1.0, abs(r_ene[k])), (k, ene[k], r_ene[k])
+        assert rel_rms(f, r_g) <= 1e-9
+        it = na.C.c_int(-3)
+        na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+    finally:
+        for k in range(4):
+            s.Phase[k] = saved[k]
