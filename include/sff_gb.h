@@ -5,7 +5,7 @@
  * Every function keeps the name, argument order and argument meaning of the
  * libsff routine it replaces (src/sff/sff.h:124-180 in dacase/nabc):
  *
- *   rdparm        prm.c:584          read an Amber prmtop (new %FLAG format)
+ *   rdparm        prm.c:584          read an Amber prmtop (%FLAG and old layouts)
  *   mm_options    mm_options.l:276   keyword=value option string
  *   mme_init_sff  sff.c:858          GB parameter set-up, exclusion lists
  *   mme           sff.c:1854         energy + gradient, returns the total
@@ -30,39 +30,59 @@ extern "C" {
 #endif
 
 #define NABC_NAMESIZE 6
+/* atom, residue and type names: 4 characters of the prmtop plus padding
+ * and the terminating NUL (NAME_T, sff.h:36) */
+typedef char NABC_NAME_T[NABC_NAMESIZE];
 
 /*
- * Topology as read from a prmtop.  Field names follow PARMSTRUCT_T
- * (src/sff/sff.h:40-62).  Atom indices in the bond/angle/dihedral arrays are
- * stored as in the file (3*(atom-1), negative third/fourth index flags as in
- * Amber); ExclAt is 1-based with 0 meaning "none".  Charges are in the
- * prmtop's internal unit (electron charge * 18.2223).
+ * Topology as read from a prmtop: PARMSTRUCT_T of src/sff/sff.h:40-62,
+ * field for field in the same order and with the same types (int for
+ * INT_T, double for REAL_T), so code compiled against libsff's header
+ * reads every field at the offset it expects.  Atom indices in the
+ * bond/angle/dihedral arrays are stored as in the file (3*(atom-1),
+ * negative third/fourth index flags as in Amber); ExclAt is 1-based with
+ * 0 meaning "none".  Charges are in the prmtop's internal unit (electron
+ * charge * 18.2223).  Scee/Scnb hold the inverted SCEE/SCNB factors.
+ * AtomRes holds each atom's 0-based residue; Boundary/Nspm the molecule
+ * boundaries (one molecule of Natom atoms without a box); Box[0..2] the
+ * box lengths (999999999 without a box) and Box[3] its angle; Gvdw is
+ * allocated and zero (the reference leaves its computation disabled);
+ * Nstrand/Ncomplex and Ipstrand/Ipcomplex are 1 / {1} unless the file has
+ * the HCP strand and complex pointers.
  */
 typedef struct parm {
     char ititl[81];
-    int IfBox, Nmxrs, IfCap;
-    int Natom, Ntypes, Nbonh, Mbona, Ntheth, Mtheta, Nphih, Mphia,
-        Nhparm, Nparm, Nnb, Nres, Nbona, Ntheta, Nphia, Numbnd, Numang,
-        Nptra, Natyp, Nphb, Ntype2d, Nttyp;
-    char (*AtomNames)[NABC_NAMESIZE];
+    int IfBox, Nmxrs, IfCap,
+        Natom, Ntypes, Nbonh, Mbona, Ntheth, Mtheta,
+        Nphih, Mphia, Nhparm, Nparm, Nnb, Nres,
+        Nbona, Ntheta, Nphia, Numbnd, Numang, Nptra,
+        Natyp, Nphb, Nat3, Ntype2d, Nttyp, Nspm, Iptres, Nspsol,
+        Ipatm, Natcap, Numextra;
+    NABC_NAME_T *AtomNames, *ResNames, *AtomType, *AtomTree;
     double *Charges, *Masses, *Rk, *Req, *Tk, *Teq, *Pk, *Pn, *Phase,
-           *Cn1, *Cn2, *HB12, *HB10, *Rborn, *Fs, *Scee, *Scnb;
-    double Fsmax;
-    int *Iac, *Iblo, *Cno, *Ipres, *ExclAt;
-    int *BondHAt1, *BondHAt2, *BondHNum, *BondAt1, *BondAt2, *BondNum;
-    int *AngleHAt1, *AngleHAt2, *AngleHAt3, *AngleHNum;
-    int *AngleAt1, *AngleAt2, *AngleAt3, *AngleNum;
-    int *DihHAt1, *DihHAt2, *DihHAt3, *DihHAt4, *DihHNum;
-    int *DihAt1, *DihAt2, *DihAt3, *DihAt4, *DihNum;
-    int *AtomicNum;
+        *Solty, *Cn1, *Cn2, *HB12, *HB10, *Rborn, *Fs;
+    double Box[4], Cutcap, Xcap, Ycap, Zcap, Fsmax;
+    int *Iac, *Iblo, *Cno, *Ipres, *ExclAt, *TreeJoin,
+        *AtomRes, *BondHAt1, *BondHAt2, *BondHNum, *BondAt1, *BondAt2,
+        *BondNum, *AngleHAt1, *AngleHAt2, *AngleHAt3, *AngleHNum,
+        *AngleAt1, *AngleAt2, *AngleAt3, *AngleNum, *DihHAt1,
+        *DihHAt2, *DihHAt3, *DihHAt4, *DihHNum, *DihAt1, *DihAt2,
+        *DihAt3, *DihAt4, *DihNum, *Boundary, *AtomicNum;
     int *N14pairs, *N14pairlist;
-    double *N14sceelist, *N14scnblist;
+    double *Gvdw, *Scee, *Scnb, *N14sceelist, *N14scnblist;
+    int Nstrand, Ncomplex;
+    int *Ipstrand, *Ipcomplex;
 } PARMSTRUCT_T;
 
-/* prmtop I/O (prm.c:584 rdparm, prm.c:1997 wrparm). */
+/*
+ * prmtop I/O (prm.c:584 rdparm, prm.c:1997 wrparm, free_prm sff.h:178).
+ * rdparm reads both layouts the reference reads: %FLAG/%FORMAT files and
+ * the older fixed-order files without flags (whose GB radii and screening
+ * factors come from the first letter of each atom name, prm.c:1656-1741).
+ */
 PARMSTRUCT_T *rdparm(char *name);
 int wrparm(PARMSTRUCT_T *prm, char *name);
-void freeparm(PARMSTRUCT_T *prm);
+int free_prm(PARMSTRUCT_T *prm);
 
 /* rst7 I/O (sff.c:651 getxv, sff.c:733 putxv). */
 int getxv(char *fname, int natom, double start_time, double *x, double *v);
@@ -115,7 +135,9 @@ int conjgrad(double x[], int *n, double *f,
              double (*func)(double *, double *, int *),
              double *rmsgrad, double *dfpred, int *maxiter);
 
-/* Output stream used for the ff:/md: lines (libsff's nabout). */
+/* Output stream of the ff:/md: lines and mm_options echo (libsff's
+ * nabout, sff.h:119; NULL means stdout); mm_set_output sets it too. */
+extern FILE *nabout;
 void mm_set_output(FILE *fp);
 
 /*

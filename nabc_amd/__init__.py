@@ -47,29 +47,23 @@ class Options(C.Structure):
 
 
 class Parm(C.Structure):
-    """Prefix of PARMSTRUCT_T (include/sff_gb.h); only leading fields are read."""
-    _fields_ = [("ititl", C.c_char * 81), ("IfBox", C.c_int), ("Nmxrs", C.c_int),
-                ("IfCap", C.c_int), ("Natom", C.c_int), ("Ntypes", C.c_int),
-                ("Nbonh", C.c_int), ("Mbona", C.c_int), ("Ntheth", C.c_int),
-                ("Mtheta", C.c_int), ("Nphih", C.c_int), ("Mphia", C.c_int),
-                ("Nhparm", C.c_int), ("Nparm", C.c_int), ("Nnb", C.c_int),
-                ("Nres", C.c_int), ("Nbona", C.c_int), ("Ntheta", C.c_int),
-                ("Nphia", C.c_int), ("Numbnd", C.c_int), ("Numang", C.c_int),
-                ("Nptra", C.c_int), ("Natyp", C.c_int), ("Nphb", C.c_int),
-                ("Ntype2d", C.c_int), ("Nttyp", C.c_int), ("AtomNames", C.c_void_p),
-                ("Charges", dp), ("Masses", dp), ("Rk", dp), ("Req", dp), ("Tk", dp),
-                ("Teq", dp), ("Pk", dp), ("Pn", dp), ("Phase", dp), ("Cn1", dp),
-                ("Cn2", dp), ("HB12", dp), ("HB10", dp), ("Rborn", dp), ("Fs", dp),
-                ("Scee", dp), ("Scnb", dp), ("Fsmax", C.c_double), ("Iac", ip),
-                ("Iblo", ip), ("Cno", ip), ("Ipres", ip), ("ExclAt", ip),
-                ("BondHAt1", ip), ("BondHAt2", ip), ("BondHNum", ip), ("BondAt1", ip),
-                ("BondAt2", ip), ("BondNum", ip), ("AngleHAt1", ip), ("AngleHAt2", ip),
-                ("AngleHAt3", ip), ("AngleHNum", ip), ("AngleAt1", ip), ("AngleAt2", ip),
-                ("AngleAt3", ip), ("AngleNum", ip), ("DihHAt1", ip), ("DihHAt2", ip),
-                ("DihHAt3", ip), ("DihHAt4", ip), ("DihHNum", ip), ("DihAt1", ip),
-                ("DihAt2", ip), ("DihAt3", ip), ("DihAt4", ip), ("DihNum", ip),
-                ("AtomicNum", ip), ("N14pairs", ip), ("N14pairlist", ip),
-                ("N14sceelist", dp), ("N14scnblist", dp)]
+    """PARMSTRUCT_T (include/sff_gb.h; the reference's sff.h:40-62 layout)."""
+    _fields_ = ([("ititl", C.c_char * 81)] +
+                [(f, C.c_int) for f in (
+                    "IfBox Nmxrs IfCap Natom Ntypes Nbonh Mbona Ntheth Mtheta Nphih Mphia Nhparm "
+                    "Nparm Nnb Nres Nbona Ntheta Nphia Numbnd Numang Nptra Natyp Nphb Nat3 Ntype2d "
+                    "Nttyp Nspm Iptres Nspsol Ipatm Natcap Numextra").split()] +
+                [(f, C.c_void_p) for f in "AtomNames ResNames AtomType AtomTree".split()] +
+                [(f, dp) for f in "Charges Masses Rk Req Tk Teq Pk Pn Phase Solty Cn1 Cn2 HB12 HB10 Rborn Fs".split()] +
+                [("Box", C.c_double * 4)] +
+                [(f, C.c_double) for f in "Cutcap Xcap Ycap Zcap Fsmax".split()] +
+                [(f, ip) for f in (
+                    "Iac Iblo Cno Ipres ExclAt TreeJoin AtomRes BondHAt1 BondHAt2 BondHNum BondAt1 "
+                    "BondAt2 BondNum AngleHAt1 AngleHAt2 AngleHAt3 AngleHNum AngleAt1 AngleAt2 "
+                    "AngleAt3 AngleNum DihHAt1 DihHAt2 DihHAt3 DihHAt4 DihHNum DihAt1 DihAt2 DihAt3 "
+                    "DihAt4 DihNum Boundary AtomicNum N14pairs N14pairlist").split()] +
+                [(f, dp) for f in "Gvdw Scee Scnb N14sceelist N14scnblist".split()] +
+                [("Nstrand", C.c_int), ("Ncomplex", C.c_int), ("Ipstrand", ip), ("Ipcomplex", ip)])
 
 
 class MDOptions(C.Structure):
@@ -178,7 +172,7 @@ _sig("gbgen_build", C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(ParmP), dp])
 _sig("gbgen_write", C.c_int, [C.c_int, C.c_ulonglong, C.c_char_p, C.c_char_p])
 _sig("rdparm", ParmP, [C.c_char_p])
 _sig("wrparm", C.c_int, [ParmP, C.c_char_p])
-_sig("freeparm", None, [ParmP])
+_sig("free_prm", C.c_int, [ParmP])
 _sig("getxv", C.c_int, [C.c_char_p, C.c_int, C.c_double, dp, dp])
 _sig("putxv", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_double, dp, dp])
 _sig("mm_options", C.c_int, [C.c_char_p])

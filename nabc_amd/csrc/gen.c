@@ -300,7 +300,7 @@ int gbgen_build(int natom, unsigned long long seed, PARMSTRUCT_T **prm_out, doub
     p->Nphb = 0;
 
     cls = (int *)malloc(sizeof(int) * (size_t)n);
-    p->AtomNames = (char (*)[NABC_NAMESIZE])calloc((size_t)n, NABC_NAMESIZE);
+    p->AtomNames = (NABC_NAME_T *)calloc((size_t)n, NABC_NAMESIZE);
     p->Charges = (double *)calloc((size_t)n, sizeof(double));
     p->Masses = (double *)calloc((size_t)n, sizeof(double));
     p->Rborn = (double *)calloc((size_t)n, sizeof(double));
@@ -434,8 +434,10 @@ int gbgen_build(int natom, unsigned long long seed, PARMSTRUCT_T **prm_out, doub
         p->Iblo[i] = c;
         p->Nnb += c;
     }
-    p->Ipres = (int *)calloc((size_t)p->Nres, sizeof(int));
-    for (i = 0; i < p->Nres; i++) p->Ipres[i] = 20 * i + 1;
+    /* residues of 20 atoms, 0-based first atoms with Ipres[Nres] = Natom */
+    p->Ipres = (int *)calloc((size_t)p->Nres + 1, sizeof(int));
+    for (i = 0; i < p->Nres; i++) p->Ipres[i] = 20 * i;
+    p->Ipres[p->Nres] = n;
     p->HB12 = (double *)calloc(1, sizeof(double));
     p->HB10 = (double *)calloc(1, sizeof(double));
 
@@ -444,6 +446,7 @@ int gbgen_build(int natom, unsigned long long seed, PARMSTRUCT_T **prm_out, doub
         double si = p->Fs[i] * (p->Rborn[i] - 0.09);
         if (si > p->Fsmax) p->Fsmax = si;
     }
+    nabc_parm_complete(p);
     nabc_build_14_list(p);
     free(cls);
     *prm_out = p;
@@ -458,7 +461,7 @@ int gbgen_write(int natom, unsigned long long seed, const char *prmtop_path,
     int rc = gbgen_build(natom, seed, &p, x);
     if (rc == 0) rc = wrparm(p, (char *)prmtop_path);
     if (rc == 0) rc = putxv((char *)rst7_path, p->ititl, natom, 0.0, x, NULL);
-    freeparm(p);
+    free_prm(p);
     free(x);
     return rc;
 }

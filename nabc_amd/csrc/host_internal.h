@@ -31,7 +31,7 @@ typedef struct {
 } nabc_opts_t;
 
 extern nabc_opts_t nabc_opts;
-extern FILE *nabc_out;
+extern FILE *nabout;
 
 /* what mme_init_sff derives from a topology (model.c) */
 typedef struct {
@@ -56,6 +56,7 @@ int nabc_model_context(const nabc_model *m, const double *x, const nabc_opts_t *
 void nabc_seterr(const char *fmt, ...);
 int nabc_atomic_number_from_mass(double mass);
 void nabc_build_14_list(PARMSTRUCT_T *prm);
+void nabc_parm_complete(PARMSTRUCT_T *prm);
 
 /* Amber netCDF trajectory writer (nctraj.c) */
 typedef struct {

@@ -20,7 +20,7 @@
 #include "host_internal.h"
 
 nabc_opts_t nabc_opts;
-FILE *nabc_out = NULL;
+FILE *nabout = NULL;
 static char nabc_errbuf[512];
 
 void nabc_seterr(const char *fmt, ...)
@@ -34,7 +34,7 @@ void nabc_seterr(const char *fmt, ...)
 
 const char *mm_get_error(void) { return nabc_errbuf; }
 
-void mm_set_output(FILE *fp) { nabc_out = fp; }
+void mm_set_output(FILE *fp) { nabout = fp; }
 
 void mm_reset_options(void)
 {
@@ -163,7 +163,7 @@ static int set_key(const char *key, const char *val)
             else goto bad;
             break;
         }
-        if (nabc_out) fprintf(nabc_out, "\tmm_options:  %s=%s\n", key, val);
+        if (nabout) fprintf(nabout, "\tmm_options:  %s=%s\n", key, val);
         return 0;
     }
 bad:

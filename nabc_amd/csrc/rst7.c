@@ -78,7 +78,7 @@ int putxv(char *fname, char *title, int natom, double start_time,
 {
     FILE *fp;
     int i, pass;
-    if (!strcmp(fname, "-")) fp = nabc_out ? nabc_out : stdout;
+    if (!strcmp(fname, "-")) fp = nabout ? nabout : stdout;
     else if (!(fp = fopen(fname, "w"))) {
         nabc_seterr("Can't open file %s", fname);
         return -1;
@@ -94,6 +94,6 @@ int putxv(char *fname, char *title, int natom, double start_time,
             fputc('\n', fp);
         }
     }
-    if (fp != stdout && fp != nabc_out) fclose(fp);
+    if (fp != stdout && fp != nabout) fclose(fp);
     return 0;
 }

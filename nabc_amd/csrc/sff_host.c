@@ -32,7 +32,7 @@ static struct {
     nabc_nctraj traj;   /* md() trajectory, when mme_init_sff was given a file name */
 } S;
 
-static FILE *out(void) { return nabc_out ? nabc_out : stdout; }
+static FILE *out(void) { return nabout ? nabout : stdout; }
 
 static void release_ctx(void)
 {

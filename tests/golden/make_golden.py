@@ -5,13 +5,21 @@ copies the input files the reference's tests use (data files only) and
 transcribes the expected numbers from their golden outputs into
 golden.json, each with its source file:line.
 
-  aspgb8   test/nab/aspgb8.nab:8-18, aspgb8.out.check:12-47
-           24 igb=8 effective Born radii (7 decimals) and per-term
-           energies (9 decimals), cut = rgbmax = 100.
+  aspgb8   test/nab/aspgb8.nab:8-18, aspgb8.out: 24 igb=8 effective Born
+           radii (7 decimals) and per-term energies and frms (9 decimals)
+           of the reference's current code, cut = rgbmax = 100.
+  aspgb8_check
+           test/nab/aspgb8.out.check:12-47, the older golden file of the
+           same input (its non-GB terms agree with aspgb8.out).
   gb5trx, gb8trx, gb7trx, gbtrx
            test/nab/gb*trx.nab, gb*trx.out.check: the "ff:" line of every
            mme() call of a 10-iteration conjgrad run on thioredoxin
            (1654 atoms), 2 decimals, frms to 3 significant digits.
+  gbrna    test/nab/gbrna.nab, gbrna.out.check:12-22: the same for the
+           Dickerson decamer (640 atoms) with gb=1 and salt, read from an
+           old-format prmtop (GB radii from the atom names); gbrna.pdb's
+           coordinates are also written as gbrna.rst7 (file order, the
+           order of the prmtop's atoms).
 """
 import json
 import os
@@ -80,6 +88,20 @@ def main():
                                                              rmsgrad_per_dof=0.0001))
         g[name] = dict(source="test/nab/%s.out.check" % name, prmtop="prmtop.gb2",
                        pdb="trxox.2.4ns.pdb", options=opt, ff=ff_lines(src))
+    # gbrna: old-format prmtop, gb=1, kappa 0.10395 (gbrna.nab:8)
+    shutil.copyfile(os.path.join(REF, "gbrna.prmtop"), os.path.join(HERE, "gbrna.prmtop"))
+    xyz = []
+    for line in open(os.path.join(REF, "gbrna.pdb")):
+        if line.startswith(("ATOM", "HETATM")):
+            xyz.extend((float(line[30:38]), float(line[38:46]), float(line[46:54])))
+    with open(os.path.join(HERE, "gbrna.rst7"), "w") as fp:
+        fp.write("gbrna.pdb coordinates\n%6d\n" % (len(xyz) // 3))
+        for k in range(0, len(xyz), 6):
+            fp.write("".join("%12.7f" % v for v in xyz[k:k + 6]) + "\n")
+    g["gbrna"] = dict(source="test/nab/gbrna.out.check", prmtop="gbrna.prmtop", rst7="gbrna.rst7",
+                      options=dict(gb=1, kappa=0.10395, rgbmax=99.0, cut=99.0, epsext=78.5, diel="C",
+                                   nsnb=10, cg=dict(maxiter=10, dfpred=10.0, rmsgrad_per_dof=0.0001)),
+                      ff=ff_lines(os.path.join(REF, "gbrna.out.check")))
     with open(os.path.join(HERE, "golden.json"), "w") as fp:
         json.dump(g, fp, indent=1)
 

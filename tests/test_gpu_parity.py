@@ -397,3 +397,15 @@ def test_general_torsion_phases_match_oracle(gpu, gen_dir):
     finally:
         for k in range(4):
             s.Phase[k] = saved[k]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gbrna_old_format_gb1(gpu, precision):
+    """gb=1 (HCT) with salt on the decamer of test/nab/gbrna.nab, read from
+    its old-format prmtop: pair list, energies and gradient against the
+    oracle (whose gb=1 CG trajectory reproduces gbrna.out.check)."""
+    na = gpu
+    parm = na.read_parm(GOLDEN + "/gbrna.prmtop")
+    arr = na.parm_arrays(parm)
+    x, _ = na.read_rst7(GOLDEN + "/gbrna.rst7", arr["natom"])
+    _compare(na, parm, arr, x, gb=1, cut=99.0, rgbmax=99.0, kappa=0.10395, precision=precision)

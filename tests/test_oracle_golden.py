@@ -91,3 +91,61 @@ def test_trx_cg_trajectory(case):
     assert [e[0] for e in log] == [w["iter"] for w in g["ff"]]
     for (it, ene, frms), want in zip(log, g["ff"]):
         _match_ff(_ff_of(ene, frms), want)
+
+
+def test_gbrna_old_format_cg_trajectory():
+    """gbrna.nab: the Dickerson decamer (640 atoms) from an old-format
+    prmtop (no %FLAG sections; GB radii and screens from the first letter
+    of each atom name, prm.c:1656-1741), gb=1 (HCT) with kappa 0.10395:
+    the 11 ff: lines of its 10-iteration CG run (gbrna.out.check:12-22)
+    from the oracle's mme through the product's conjgrad.  This pins the
+    old-layout reader (every section, in the fixed order, or the energies
+    would not come out) and the gb=1 totals."""
+    g = golden()["gbrna"]
+    o = g["options"]
+    parm = na.read_parm(GOLDEN + "/gbrna.prmtop")
+    arr = na.parm_arrays(parm)
+    x, _ = na.read_rst7(GOLDEN + "/gbrna.rst7", arr["natom"])
+    assert arr["natom"] == 640
+    orc = oracle.Oracle(parm, arr["natom"], gb=o["gb"], cut=o["cut"], rgbmax=o["rgbmax"], kappa=o["kappa"])
+    oracle.lib.orc_log_clear()
+    oracle.lib.orc_set_current(orc.h, o["nsnb"])
+    n = na.C.c_int(3 * arr["natom"])
+    f = na.C.c_double()
+    rms = na.C.c_double(3 * arr["natom"] * 0.0001)
+    dfp = na.C.c_double(10.0)
+    mx = na.C.c_int(10)
+    xc = np.ascontiguousarray(x)
+    cb = na.MMEFUNC(oracle.lib.orc_mme_cb)
+    ret = na.lib.conjgrad(xc.ctypes.data_as(na.dp), na.C.byref(n), na.C.byref(f), cb,
+                          na.C.byref(rms), na.C.byref(dfp), na.C.byref(mx))
+    log = oracle.log_entries()
+    oracle.lib.orc_set_current(None, 1)
+    orc.close()
+    assert ret == -3
+    assert [e[0] for e in log] == [w["iter"] for w in g["ff"]]
+    for (it, ene, frms), want in zip(log, g["ff"]):
+        _match_ff(_ff_of(ene, frms), want)
+
+
+def test_old_format_reader_fields():
+    """The old-layout reader fills what rdparm fills: title, POINTERS, names
+    (4 characters + blank), 0-based residue pointers with Ipres[Nres] =
+    Natom, each atom's residue, the defaults a file without box or HCP
+    sections gets, SCEE/SCNB defaults, and the Bondi radii of the name
+    rule (e.g. H5T after nothing: oxygen's 0.8; H5'1 after C5': 1.3)."""
+    p = na.read_parm(GOLDEN + "/gbrna.prmtop")
+    s = p.contents
+    assert s.ititl.startswith(b"PDB 5DNB, Dickerson decamer")
+    assert (s.Natom, s.Ntypes, s.Nbonh, s.Mbona, s.Nres, s.Nat3, s.Nspm, s.Nstrand, s.Ncomplex) == \
+        (640, 15, 220, 468, 20, 1920, 1, 1, 1)
+    names = (na.C.c_char * 6 * s.Natom).from_address(s.AtomNames)
+    assert bytes(names[0]) == b"H5T  \0" and bytes(names[3]) == b"H5'1 \0"
+    assert [s.Ipres[i] for i in range(3)] == [0, 29, 60] and s.Ipres[s.Nres] == 640
+    assert s.AtomRes[28] == 0 and s.AtomRes[29] == 1 and s.AtomRes[639] == 19
+    assert s.Boundary[0] == 640 and s.Box[0] == 999999999.0
+    assert all(abs(s.Scee[k] - 1 / 1.2) < 1e-15 and abs(s.Scnb[k] - 0.5) < 1e-15 for k in range(s.Nptra))
+    assert (s.Rborn[0], s.Fs[0], s.Rborn[3], s.Rborn[1], s.Rborn[2]) == (0.8, 0.85, 1.3, 1.5, 1.7)
+    fsmax = max(s.Fs[i] * (s.Rborn[i] - 0.09) for i in range(s.Natom))
+    assert abs(s.Fsmax - fsmax) < 1e-15
+    assert na.lib.free_prm(p) == 0
