@@ -26,6 +26,8 @@
 #ifndef NABC_AMD_GBHIP_H
 #define NABC_AMD_GBHIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -105,6 +107,32 @@ int gbhip_ctx_create_on(const gbhip_topology *top, const gbhip_options *opt,
 
 /* Release everything (egb(..., freevectors=1), eff.c:1463-1491). */
 int gbhip_ctx_destroy(gbhip_ctx *ctx);
+
+/*
+ * Slots: memory a context can live in without allocating, for callers that
+ * keep many short-lived contexts on one device (a serving loop) and must
+ * not allocate, free or synchronise the device while other contexts run.
+ * A slot holds a device region (its own, device_bytes > 0, or a part of
+ * the caller's memory given by gbhip_slot_set_region), pinned host memory
+ * for the context's copies (io_bytes: coordinates, gradient, energies and
+ * dynamics logs; 8 * max(6 natom + 64, 16 log rows) bytes cover a
+ * molecule; upload_bytes: set-up copies, reused in turn when smaller than
+ * they need), an event and a stream (a hipStream_t, not owned).
+ * gbhip_ctx_create_in_slot builds a context whose device arrays, pass-3
+ * derivative store and dynamics state come from the region (error
+ * GBHIP_ENOMEM when it is too small; gbhip_footprint_bytes gives a size
+ * that suffices) and whose copies go through the slot's pinned memory; its
+ * gbhip_ctx_destroy hands the slot back without freeing anything.  One
+ * context per slot at a time.  No counterpart in the reference (libsff
+ * holds one system per process).
+ */
+typedef struct gbhip_slot gbhip_slot;
+int gbhip_slot_create(size_t device_bytes, size_t io_bytes, size_t upload_bytes, void *stream,
+                      gbhip_slot **out);
+int gbhip_slot_set_region(gbhip_slot *slot, void *device_base, size_t device_bytes);
+int gbhip_slot_destroy(gbhip_slot *slot);
+int gbhip_ctx_create_in_slot(const gbhip_topology *top, const gbhip_options *opt,
+                             const double *x0, gbhip_slot *slot, gbhip_ctx **out);
 
 /* Host coordinates, AoS x[3*natom] (copied to the device). */
 int gbhip_set_coords(gbhip_ctx *ctx, const double *x);
@@ -291,6 +319,28 @@ int gbhip_mme_device(gbhip_ctx *ctx, double *d_grad, double *d_ene14);
  */
 int gbhip_set_mme_graph(gbhip_ctx *ctx, int on);
 int gbhip_mme_graph_stats(gbhip_ctx *ctx, long long *captures, long long *replays);
+
+/* ---- minimisation on the device (conjgrad(), conjgrad.c:34-263) ---- */
+
+/*
+ * conjgrad(x, &n, &f, mme, &rmsgrad, &dfpred, &maxiter) over this
+ * context's mme34 (gbhip_set_bonded must have been called), with the
+ * minimiser's vectors (coordinates, gradient, search and restart
+ * directions, the best point) resident on the device: only the energies
+ * and the scalars the line search branches on come back per evaluation.
+ * The same steps, restarts and tests as conjgrad.c, with the pair list
+ * rebuilt as mme34 does (no list yet, or iteration % nsnb == 0,
+ * eff.c:2564).  x (n = 3 natom doubles, host) is the start and receives
+ * the result, *f its energy; *result gets conjgrad's return value (the
+ * iteration count, or -1 bad line search, -2 uphill direction, -3 maxiter
+ * reached, -4 no decrease).  report (may be NULL) is called after each
+ * evaluation with its iteration number and the 14 energies of gbhip_mme
+ * (the reference prints its ff: line there).  Dot products are reduced in
+ * a fixed order.
+ */
+typedef void (*gbhip_cg_report)(int iter, const double *ene14, void *user);
+int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, double rmsgrad, double dfpred, int maxiter,
+                   int nsnb, gbhip_cg_report report, void *user, int *result);
 
 /* ---- dynamics on the device (md(), sff.c:2071-2366) ---- */
 

@@ -6,7 +6,10 @@
  * driver does with one molecule: conjgrad() over mme() (conjgrad.c:34) or
  * md() (sff.c:2071); the loop runs many such jobs at once, each with its
  * state resident on the device, admitting new ones as slots and device
- * memory free up.
+ * memory free up.  The device memory is one pool of hbm_budget bytes
+ * allocated when the loop starts; each job gets a contiguous block of its
+ * footprint (first fit) and its context lives in it, so nothing is
+ * allocated, freed or synchronised device-wide while jobs run.
  */
 #ifndef NABC_AMD_GBSERVE_H
 #define NABC_AMD_GBSERVE_H
@@ -28,6 +31,7 @@ typedef struct {
     int nsteps;              /* CG: maxiter; MD: steps */
     unsigned long long seed; /* MD noise stream */
     double arrival_s;        /* arrival time, seconds after the run starts */
+    double *x_out;           /* may be NULL: receives the final coordinates, 3*natom */
 } gbserve_job;
 
 typedef struct {
@@ -38,6 +42,8 @@ typedef struct {
     double cg_rmsgrad;       /* conjgrad rmsgrad argument */
     double cg_dfpred;        /* conjgrad dfpred argument */
     int progress;            /* > 0: a line on stderr every `progress` completed jobs */
+    double watchdog_s;       /* a job in one phase longer than this is reported on stderr
+                                (0: 30 s; < 0: no watchdog) */
 } gbserve_config;
 
 typedef struct {
@@ -51,6 +57,8 @@ typedef struct {
     int status;              /* conjgrad return code, 0 for MD, < 0 error; -2 never fits */
     int slot;                /* worker that ran it */
     int max_active;          /* jobs resident when it was admitted */
+    double setup_s;          /* admission to context ready */
+    size_t pool_offset;      /* its block of the device pool */
 } gbserve_result;
 
 /* Device bytes a context for natom atoms (and an md log of md_steps rows) holds. */
@@ -70,8 +78,9 @@ int gbserve_run(const gbserve_job *jobs, int njobs, const gbserve_config *cfg,
  * caller's function instead of the GPU job: run(job, slot, user, &energy)
  * is called on the slot's worker thread (slots threads at once) and returns
  * the job status (0 or a positive code for success, negative for failure).
- * Admission by arrival time, free slot and gbhip_footprint_bytes against
- * cfg->hbm_budget is unchanged; precision and check_fp64 are ignored and
+ * Admission by arrival time, free slot and a free block of
+ * gbhip_footprint_bytes in a pool of cfg->hbm_budget bytes is unchanged
+ * (pool_offset reports the block); precision and check_fp64 are ignored and
  * no HIP call is made.  For schedulers that run other work per job and for
  * testing the loop itself without a device.  No counterpart in the
  * reference.

@@ -88,20 +88,21 @@ class Bonded(C.Structure):
 
 class ServeJob(C.Structure):
     _fields_ = [("prm", C.c_void_p), ("x", dp), ("kind", C.c_int), ("nsteps", C.c_int),
-                ("seed", C.c_ulonglong), ("arrival_s", C.c_double)]
+                ("seed", C.c_ulonglong), ("arrival_s", C.c_double), ("x_out", dp)]
 
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("hbm_budget", C.c_size_t), ("precision", C.c_int),
                 ("check_fp64", C.c_int), ("cg_rmsgrad", C.c_double), ("cg_dfpred", C.c_double),
-                ("progress", C.c_int)]
+                ("progress", C.c_int), ("watchdog_s", C.c_double)]
 
 
 class ServeResult(C.Structure):
     _fields_ = [("start_s", C.c_double), ("end_s", C.c_double), ("latency_s", C.c_double),
                 ("energy", C.c_double), ("energy_fp64", C.c_double),
                 ("energy_final_eval", C.c_double), ("footprint", C.c_size_t),
-                ("status", C.c_int), ("slot", C.c_int), ("max_active", C.c_int)]
+                ("status", C.c_int), ("slot", C.c_int), ("max_active", C.c_int),
+                ("setup_s", C.c_double), ("pool_offset", C.c_size_t)]
 
 
 GBSERVE_CG = 0
@@ -140,6 +141,13 @@ _sig("gbhip_timing_stop", C.c_int, [vp, dp, ip])
 _sig("gbhip_md_init", C.c_int, [vp, dp, dp, dp, C.POINTER(MDOptions), C.c_int])
 _sig("gbhip_md_run", C.c_int, [vp, C.c_int, C.c_int])
 _sig("gbhip_md_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)])
+CG_REPORT = C.CFUNCTYPE(None, C.c_int, dp, C.c_void_p)
+_sig("gbhip_conjgrad", C.c_int, [vp, dp, C.c_int, dp, C.c_double, C.c_double, C.c_int, C.c_int, CG_REPORT,
+                                 C.c_void_p, ip])
+_sig("gbhip_slot_create", C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, vp, C.POINTER(vp)])
+_sig("gbhip_slot_set_region", C.c_int, [vp, vp, C.c_size_t])
+_sig("gbhip_slot_destroy", C.c_int, [vp])
+_sig("gbhip_ctx_create_in_slot", C.c_int, [C.POINTER(Topology), C.POINTER(Options), dp, vp, C.POINTER(vp)])
 _sig("gbhip_md_state", C.c_int, [vp, dp, dp, dp])
 _sig("gbhip_md_log", C.c_int, [vp, dp, C.c_int, ip])
 _sig("gbhip_batch_timing_start", C.c_int, [vp, C.c_int])
@@ -191,26 +199,35 @@ _sig("gbserve_run_custom", C.c_int, [C.POINTER(ServeJob), C.c_int, C.POINTER(Ser
 
 
 def serve(jobs, slots=8, hbm_budget=64 << 30, precision=GBHIP_MIXED, check_fp64=False,
-          mm_opts="gb=5, cut=20.0, rgbmax=20.0", cg_rmsgrad=1e-4, cg_dfpred=10.0, progress=0):
+          mm_opts="gb=5, cut=20.0, rgbmax=20.0", cg_rmsgrad=1e-4, cg_dfpred=10.0, progress=0,
+          watchdog_s=0.0, want_x=False):
     """Run jobs through the serving loop.
 
     jobs: list of dicts with prm (PARMSTRUCT_T pointer), x, kind
     (GBSERVE_CG / GBSERVE_MD), nsteps, seed, arrival_s.
-    Returns a list of result dicts (times in seconds from the run start)."""
+    Returns a list of result dicts (times in seconds from the run start);
+    with want_x each also holds the job's final coordinates as "x"."""
     n = len(jobs)
     arr = (ServeJob * n)()
-    keep = []
+    keep, outs = [], []
     for k, j in enumerate(jobs):
         x = np.ascontiguousarray(j["x"], dtype=np.float64)
+        xo = np.zeros_like(x) if want_x else None
         keep.append(x)
+        outs.append(xo)
         arr[k] = ServeJob(C.cast(j["prm"], C.c_void_p), _p(x), j["kind"], j["nsteps"],
-                          j.get("seed", k + 1), j["arrival_s"])
-    cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred, progress)
+                          j.get("seed", k + 1), j["arrival_s"], _p(xo) if want_x else None)
+    cfg = ServeConfig(slots, hbm_budget, precision, 1 if check_fp64 else 0, cg_rmsgrad, cg_dfpred, progress,
+                      watchdog_s)
     res = (ServeResult * n)()
     rc = lib.gbserve_run(arr, n, C.byref(cfg), mm_opts.encode() if mm_opts else None, res)
     if rc != 0:
         raise GBHipError(lib.mm_get_error().decode(errors="replace"))
-    return [{f[0]: getattr(r, f[0]) for f in ServeResult._fields_} for r in res]
+    out = [{f[0]: getattr(r, f[0]) for f in ServeResult._fields_} for r in res]
+    if want_x:
+        for r, xo in zip(out, outs):
+            r["x"] = xo
+    return out
 
 
 def serve_custom(jobs, run, slots=8, hbm_budget=64 << 30):
@@ -224,7 +241,7 @@ def serve_custom(jobs, run, slots=8, hbm_budget=64 << 30):
         x = np.ascontiguousarray(j.get("x", np.zeros(3)), dtype=np.float64)
         keep.append(x)
         arr[k] = ServeJob(C.cast(j["prm"], C.c_void_p), _p(x), j["kind"], j["nsteps"],
-                          j.get("seed", k + 1), j["arrival_s"])
+                          j.get("seed", k + 1), j["arrival_s"], None)
     base = C.addressof(arr)
     size = C.sizeof(ServeJob)
 
@@ -238,7 +255,7 @@ def serve_custom(jobs, run, slots=8, hbm_budget=64 << 30):
         return st
 
     cb = SERVE_RUNNER(tramp)
-    cfg = ServeConfig(slots, hbm_budget, GBHIP_MIXED, 0, 0.0, 0.0, 0)
+    cfg = ServeConfig(slots, hbm_budget, GBHIP_MIXED, 0, 0.0, 0.0, 0, -1.0)
     res = (ServeResult * n)()
     rc = lib.gbserve_run_custom(arr, n, C.byref(cfg), cb, None, res)
     if rc != 0:

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -111,6 +112,26 @@ struct GBPar {
     double neckscale, kappa, epsext;
 };
 
+/*
+ * Memory a context can live in without allocating (a serving slot,
+ * gbhip_slot): a device region the engine suballocates from (256-byte
+ * aligned, released as a whole when the context goes), pinned host memory
+ * for copies (an io part for coordinates / gradient / energies / logs, an
+ * upload part for set-up copies, reused in turn), an event marking the end
+ * of the last copy out of the io part, and the stream.  None of it is
+ * allocated or freed by a context that uses it.
+ */
+struct SlotMem {
+    char *dbase = nullptr;
+    size_t dsize = 0, dused = 0;
+    char *hio = nullptr, *hup = nullptr;   /* pinned */
+    size_t hio_size = 0, hup_size = 0, hup_used = 0;
+    hipEvent_t ev = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_device = false;               /* dbase allocated by the slot itself */
+    bool in_use = false;                   /* a context lives in it */
+};
+
 struct Engine {
     int nmol = 0, natoms = 0, nslots = 0, ntiles = 0;
     long long tp_cap = 0;
@@ -149,14 +170,29 @@ struct Engine {
      * every list the graph can build */
     bool capture_spans_rebuilds = false;
     /* pinned host staging for coordinates in and gradient/energies out
-     * (3*natoms + 3*natoms + 14 doubles); stage_ev marks the end of the
-     * last copy out of the coordinate part */
+     * (3*natoms + 3*natoms + 64 doubles: energies and minimiser scalars at
+     * the end); stage_ev marks the end of the last copy out of the
+     * coordinate part */
     double *h_stage = nullptr;
+    size_t h_stage_doubles = 0;
     hipEvent_t stage_ev = nullptr;
     bool stage_failed = false; /* no pinned memory: copy directly */
+    /* set when the context lives in a slot: device memory, staging and event
+     * are the slot's (nothing is allocated or freed here) */
+    SlotMem *slot = nullptr;
 
     ~Engine()
     {
+        if (slot) {
+            /* the slot's memory outlives the context: only wait for this
+             * context's work, then hand the region back */
+            if (stream) (void)hipStreamSynchronize(stream);
+            for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+            slot->dused = 0;
+            slot->hup_used = 0;
+            slot->in_use = false;
+            return;
+        }
         if (h_stage) {
             if (stream) (void)hipStreamSynchronize(stream);
             (void)hipHostFree(h_stage);
@@ -171,14 +207,58 @@ struct Engine {
         for (void *p : allocs) (void)hipFree(p);
     }
 
+    /* device bytes: from the slot's region, or hipMalloc */
+    int dalloc_bytes(void **p, size_t bytes)
+    {
+        if (bytes == 0) bytes = 1;
+        if (slot) {
+            const size_t off = (slot->dused + 255) & ~(size_t)255;
+            if (off + bytes > slot->dsize)
+                return seterr(GBHIP_ENOMEM, "slot device region of %zu bytes is full (%zu more wanted)", slot->dsize,
+                              bytes);
+            *p = slot->dbase + off;
+            slot->dused = off + bytes;
+            return 0;
+        }
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) return seterr(GBHIP_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+        allocs.push_back(q);
+        *p = q;
+        return 0;
+    }
+
     template <typename T> int dalloc(T **p, size_t n)
     {
         void *q = nullptr;
-        if (n == 0) n = 1;
-        if (hipMalloc(&q, n * sizeof(T)) != hipSuccess)
-            return seterr(GBHIP_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
-        allocs.push_back(q);
+        int rc = dalloc_bytes(&q, (n ? n : 1) * sizeof(T));
+        if (rc) return rc;
         *p = (T *)q;
+        return 0;
+    }
+
+    /*
+     * Host-to-device copy on the engine stream.  In a slot it goes through
+     * the pinned upload area (asynchronous; when the area is full the
+     * stream is waited for and the area reused); otherwise from pageable
+     * memory and waited for.  A legacy-stream copy would have to order
+     * against other threads' streams, some of them capturing.
+     */
+    int h2d(void *dst, const void *src, size_t bytes)
+    {
+        if (bytes == 0) return 0;
+        if (slot && slot->hup && bytes <= slot->hup_size) {
+            size_t off = (slot->hup_used + 255) & ~(size_t)255;
+            if (off + bytes > slot->hup_size) {
+                HIPCHK(hipStreamSynchronize(stream));
+                off = 0;
+            }
+            memcpy(slot->hup + off, src, bytes);
+            slot->hup_used = off + bytes;
+            HIPCHK(hipMemcpyAsync(dst, slot->hup + off, bytes, hipMemcpyHostToDevice, stream));
+            return 0;
+        }
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         return 0;
     }
 
@@ -187,11 +267,16 @@ struct Engine {
         T *p;
         int rc = dalloc(&p, src.size());
         if (rc) return rc;
-        /* on the engine's own stream: a legacy-stream copy would have to
-         * order against other threads' streams, some of them capturing */
-        HIPCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        if ((rc = h2d(p, src.data(), src.size() * sizeof(T)))) return rc;
         *dst = p;
+        return 0;
+    }
+
+    /* wait for the set-up copies of a slot's upload area */
+    int upload_done()
+    {
+        HIPCHK(hipStreamSynchronize(stream));
+        if (slot) slot->hup_used = 0;
         return 0;
     }
 };
@@ -249,11 +334,22 @@ struct MmeGraph {
     long long captures = 0, replays = 0;
 };
 
+/* device-resident conjugate-gradient vectors (cg_device.hip), allocated at
+ * the first gbhip_conjgrad call: search direction, restart direction and
+ * gradient change, gradient at the line start, best point and its
+ * gradient (3 natoms each), per-block partial sums and the reduced
+ * scalars */
+struct CGState {
+    double *d = nullptr, *rsdx = nullptr, *rsdg = nullptr, *ginit = nullptr, *xopt = nullptr, *gopt = nullptr;
+    double *part = nullptr, *res = nullptr;
+};
+
 struct gbhip_ctx {
     Engine E;
     BondedDev B;
     MDState M;
     MmeGraph G;
+    CGState C;
     ~gbhip_ctx()
     {
         if (G.exec) (void)hipGraphExecDestroy(G.exec);
@@ -279,6 +375,8 @@ bool engine_store_holds_list(const Engine &E);
 #define EV_GATHERED 1 /* the slots already hold x_dev's coordinates, accumulators cleared */
 #define EV_NO_POST 2  /* the caller sums the energy slots and scatters the gradient itself */
 int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags = 0);
+/* mme34 of E.x_dev into E.g_dev and E.ene14_dev (mm_device.hip) */
+int engine_mme(struct gbhip_ctx *c);
 
 /* ------------------------------------------- per-slot steps shared by the
  * evaluation kernels and the fused kernels of the mme and dynamics paths */
@@ -320,5 +418,8 @@ __device__ __forceinline__ void ene_slot_sums(const DevPtrs &d, int nene)
 }
 
 bool engine_stage(Engine &E);
+/* device-to-host copy on the engine stream, waited for; through the pinned
+ * staging when it is large enough (no pageable copy), else directly */
+int engine_d2h(Engine &E, void *dst, const void *src, size_t bytes);
 
 #endif

@@ -1469,11 +1469,11 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     HIPCHK(hipMemsetAsync(E.x_dev, 0, 3 * sizeof(double) * (size_t)E.natoms, E.stream));
     if (x0) {
         for (int m = 0; m < nmol; m++)
-            if (x0[m])
-                HIPCHK(hipMemcpyAsync(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x0[m],
-                                      3 * sizeof(double) * (size_t)E.mol_natom[m], hipMemcpyHostToDevice, E.stream));
+            if (x0[m] && (rc = E.h2d(E.x_dev + 3 * (size_t)E.mol_atom_off[m], x0[m],
+                                     3 * sizeof(double) * (size_t)E.mol_natom[m])))
+                return rc;
     }
-    HIPCHK(hipStreamSynchronize(E.stream));
+    if ((rc = E.upload_done())) return rc;
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1553,6 +1553,23 @@ static int store_reserve(Engine &E, bool upper, long long need)
     if (E.d.dstore && need < E.d.store_cap && (!upper || E.d.store_cap >= full)) return 0;
     const size_t esz = E.precision == GBHIP_FP64 ? 2 * sizeof(double) : 2 * sizeof(float);
     long long cap = need + need / 4 + (long long)TILE * TILE;
+    if (E.slot) {
+        /* in a slot: taken once from the slot's region, for every tile pair
+         * full when that fits (then no list outgrows it), else for what the
+         * region has left; never freed or regrown (a list it cannot hold is
+         * evaluated with pass 3 recomputing, as the device flag says) */
+        if (E.d.dstore) return 0;
+        const size_t left = E.slot->dsize - std::min(E.slot->dsize, (E.slot->dused + 255) & ~(size_t)255);
+        const long long want = (size_t)full * esz <= left ? full : std::min(cap, (long long)(left / esz));
+        void *p = nullptr;
+        if (want <= need || E.dalloc_bytes(&p, (size_t)want * esz) != 0) return 0;
+        E.d.dstore = p;
+        E.d.store_cap = want;
+        hipLaunchKernelGGL(k_setflag, dim3(1), dim3(64), 0, E.stream, E.d.store_ok, 1);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(E.stream));
+        return 0;
+    }
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     if (E.store_mem) {
         HIPCHK(hipStreamSynchronize(E.stream));
@@ -1593,9 +1610,9 @@ static int store_reserve(Engine &E, bool upper, long long need)
 int engine_store_reserve(Engine &E, bool upper)
 {
     long long need = 0;
-    HIPCHK(hipMemcpyAsync(&need, E.d.store_need, sizeof(need), hipMemcpyDeviceToHost, E.stream));
-    HIPCHK(hipStreamSynchronize(E.stream));
-    const int rc = store_reserve(E, upper, need);
+    int rc = engine_d2h(E, &need, E.d.store_need, sizeof(need));
+    if (rc) return rc;
+    rc = store_reserve(E, upper, need);
     /* the same test k_qscan (or k_setflag after a regrowth) applied on the
      * device */
     E.store_holds_list = E.d.dstore && need < E.d.store_cap;
@@ -2062,6 +2079,103 @@ extern "C" int gbhip_ctx_destroy(gbhip_ctx *ctx)
     return 0;
 }
 
+/* ------------------------------------------------------------ slots */
+
+struct gbhip_slot {
+    SlotMem m;
+};
+
+extern "C" int gbhip_slot_create(size_t device_bytes, size_t io_bytes, size_t upload_bytes, void *stream,
+                                 gbhip_slot **out)
+{
+    if (!out) return seterr(GBHIP_EINVAL, "out pointer is NULL");
+    *out = nullptr;
+    if (gbhip_device_count() < 1) return seterr(GBHIP_ENODEV, "no GPU is visible to the HIP runtime");
+    gbhip_slot *sl = new (std::nothrow) gbhip_slot;
+    if (!sl) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    SlotMem &m = sl->m;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    int rc = 0;
+    if (device_bytes && hipMalloc((void **)&m.dbase, device_bytes) != hipSuccess)
+        rc = seterr(GBHIP_ENOMEM, "hipMalloc of %zu bytes failed", device_bytes);
+    else if (device_bytes) {
+        m.dsize = device_bytes;
+        m.own_device = true;
+    }
+    if (!rc && io_bytes && hipHostMalloc((void **)&m.hio, io_bytes, hipHostMallocDefault) != hipSuccess)
+        rc = seterr(GBHIP_ENOMEM, "hipHostMalloc of %zu bytes failed", io_bytes);
+    else if (!rc)
+        m.hio_size = io_bytes;
+    if (!rc && upload_bytes && hipHostMalloc((void **)&m.hup, upload_bytes, hipHostMallocDefault) != hipSuccess)
+        rc = seterr(GBHIP_ENOMEM, "hipHostMalloc of %zu bytes failed", upload_bytes);
+    else if (!rc)
+        m.hup_size = upload_bytes;
+    if (!rc && hipEventCreateWithFlags(&m.ev, hipEventDisableTiming) != hipSuccess)
+        rc = seterr(GBHIP_EHIP, "hipEventCreate failed");
+    m.stream = (hipStream_t)stream;
+    if (rc) {
+        (void)hipGetLastError();
+        if (m.own_device) (void)hipFree(m.dbase);
+        if (m.hio) (void)hipHostFree(m.hio);
+        if (m.hup) (void)hipHostFree(m.hup);
+        if (m.ev) (void)hipEventDestroy(m.ev);
+        delete sl;
+        return rc;
+    }
+    *out = sl;
+    return 0;
+}
+
+extern "C" int gbhip_slot_set_region(gbhip_slot *sl, void *device_base, size_t device_bytes)
+{
+    if (!sl) return seterr(GBHIP_EINVAL, "NULL slot");
+    if (sl->m.in_use) return seterr(GBHIP_ESTATE, "a context lives in this slot");
+    if (sl->m.own_device) return seterr(GBHIP_ESTATE, "the slot owns its device region");
+    sl->m.dbase = (char *)device_base;
+    sl->m.dsize = device_base ? device_bytes : 0;
+    sl->m.dused = 0;
+    return 0;
+}
+
+extern "C" int gbhip_slot_destroy(gbhip_slot *sl)
+{
+    if (!sl) return 0;
+    if (sl->m.in_use) return seterr(GBHIP_ESTATE, "a context lives in this slot");
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    if (sl->m.stream) (void)hipStreamSynchronize(sl->m.stream);
+    if (sl->m.own_device) (void)hipFree(sl->m.dbase);
+    if (sl->m.hio) (void)hipHostFree(sl->m.hio);
+    if (sl->m.hup) (void)hipHostFree(sl->m.hup);
+    if (sl->m.ev) (void)hipEventDestroy(sl->m.ev);
+    delete sl;
+    return 0;
+}
+
+extern "C" int gbhip_ctx_create_in_slot(const gbhip_topology *top, const gbhip_options *opt, const double *x0,
+                                        gbhip_slot *sl, gbhip_ctx **out)
+{
+    if (!out || !sl) return seterr(GBHIP_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (sl->m.in_use) return seterr(GBHIP_ESTATE, "a context already lives in this slot");
+    if (!sl->m.dbase) return seterr(GBHIP_ESTATE, "the slot has no device region");
+    gbhip_ctx *c = new (std::nothrow) gbhip_ctx;
+    if (!c) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    sl->m.dused = 0;
+    sl->m.hup_used = 0;
+    sl->m.in_use = true;
+    c->E.slot = &sl->m;
+    c->E.stream = sl->m.stream;
+    const double *xs[1] = {x0};
+    /* nothing is allocated: the lock is not needed for the set-up itself */
+    int rc = engine_build(c->E, 1, top, opt, xs);
+    if (rc) {
+        delete c; /* hands the region back */
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
 extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
@@ -2089,10 +2203,21 @@ bool engine_stage(Engine &E)
 {
     if (E.h_stage) return true;
     if (E.stage_failed) return false;
+    if (E.slot) {
+        const size_t need = 6 * (size_t)E.natoms + 64;
+        if (!E.slot->hio || E.slot->hio_size < need * sizeof(double)) {
+            E.stage_failed = true;
+            return false;
+        }
+        E.h_stage = (double *)E.slot->hio;
+        E.h_stage_doubles = E.slot->hio_size / sizeof(double);
+        E.stage_ev = E.slot->ev;
+        return true;
+    }
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     void *p = nullptr;
     hipEvent_t ev = nullptr;
-    if (hipHostMalloc(&p, sizeof(double) * (6 * (size_t)E.natoms + 14), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&p, sizeof(double) * (6 * (size_t)E.natoms + 64), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         E.stage_failed = true;
         return false;
@@ -2104,8 +2229,24 @@ bool engine_stage(Engine &E)
         return false;
     }
     E.h_stage = (double *)p;
+    E.h_stage_doubles = 6 * (size_t)E.natoms + 64;
     E.stage_ev = ev;
     return true;
+}
+
+int engine_d2h(Engine &E, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return 0;
+    if (engine_stage(E) && bytes <= E.h_stage_doubles * sizeof(double)) {
+        HIPCHK(hipEventSynchronize(E.stage_ev));
+        HIPCHK(hipMemcpyAsync(E.h_stage, src, bytes, hipMemcpyDeviceToHost, E.stream));
+        HIPCHK(hipStreamSynchronize(E.stream));
+        memcpy(dst, E.h_stage, bytes);
+        return 0;
+    }
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
 }
 
 extern "C" int gbhip_set_coords(gbhip_ctx *ctx, const double *x)
@@ -2144,8 +2285,8 @@ extern "C" int gbhip_npairs(gbhip_ctx *ctx, long long *npairs)
     if (!ctx || !npairs) return seterr(GBHIP_EINVAL, "NULL argument");
     Engine &E = ctx->E;
     unsigned long long v = 0;
-    HIPCHK(hipMemcpyAsync(&v, E.d.npairs, sizeof(v), hipMemcpyDeviceToHost, E.stream));
-    HIPCHK(hipStreamSynchronize(E.stream));
+    const int rc = engine_d2h(E, &v, E.d.npairs, sizeof(v));
+    if (rc) return rc;
     *npairs = (long long)v;
     return 0;
 }

@@ -48,10 +48,14 @@ typedef struct {
 } nabc_model;
 
 struct gbhip_ctx;
+struct gbhip_slot;
 int nabc_model_build(nabc_model *m, PARMSTRUCT_T *prm, const int *frozen, const nabc_opts_t *o);
 void nabc_model_free(nabc_model *m);
 int nabc_model_context(const nabc_model *m, const double *x, const nabc_opts_t *o, void *stream,
                        struct gbhip_ctx **out);
+/* the same in a slot (gbhip_ctx_create_in_slot) when slot is not NULL */
+int nabc_model_context_in(const nabc_model *m, const double *x, const nabc_opts_t *o, void *stream,
+                          struct gbhip_slot *slot, struct gbhip_ctx **out);
 
 void nabc_seterr(const char *fmt, ...);
 int nabc_atomic_number_from_mass(double mass);

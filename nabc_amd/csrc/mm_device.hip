@@ -404,6 +404,16 @@ static int mme_graph_launch(gbhip_ctx *c)
     return 0;
 }
 
+/* one mme evaluation of E.x_dev into E.g_dev / E.ene14_dev, as gbhip_mme
+ * launches it (the captured graph, or kernel by kernel while timing), no
+ * copies */
+int engine_mme(gbhip_ctx *c)
+{
+    Engine &E = c->E;
+    const bool graph = c->G.on && E.tev_cap == 0;
+    return graph ? mme_graph_launch(c) : mme_device(c, E.x_dev, E.g_dev, E.ene14_dev);
+}
+
 extern "C" int gbhip_set_mme_graph(gbhip_ctx *ctx, int on)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
@@ -812,7 +822,7 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
             M.log_cap = log_cap;
         }
     }
-    std::vector<double> minv(n3), zero(n3, 0.0);
+    std::vector<double> minv(n3);
     for (int a = 0; a < n; a++) {
         if (!(masses[a] > 0)) return seterr(GBHIP_EINVAL, "atom %d has mass %g", a + 1, masses[a]);
         minv[3 * a] = minv[3 * a + 1] = minv[3 * a + 2] = 1.0 / masses[a];
@@ -820,10 +830,14 @@ extern "C" int gbhip_md_init(gbhip_ctx *ctx, const double *masses, const double 
     int nfrozen = 0;
     for (int s = 0; s < E.nslots; s++)
         if (E.perm_h[s] >= 0 && E.frz_h[s]) nfrozen++;
-    HIPCHK(hipMemcpyAsync(M.minv, minv.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(M.v, v0 ? v0 : zero.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(M.sqrmass, zero.data(), sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(E.x_dev, x, sizeof(double) * n3, hipMemcpyHostToDevice, E.stream));
+    if ((rc = E.h2d(M.minv, minv.data(), sizeof(double) * n3))) return rc;
+    if (v0) {
+        if ((rc = E.h2d(M.v, v0, sizeof(double) * n3))) return rc;
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M.v, (long long)n3, 0.0);
+    }
+    hipLaunchKernelGGL(k_fill, dim3(grid_of(n3)), dim3(MM_THREADS), 0, E.stream, M.sqrmass, (long long)n3, 0.0);
+    if ((rc = E.h2d(E.x_dev, x, sizeof(double) * n3))) return rc;
     const double dtx = o->dt * 20.455;
     M.dtx = dtx;
     M.dt5 = 0.5 * dtx;
@@ -899,8 +913,7 @@ extern "C" int gbhip_md_run(gbhip_ctx *ctx, int nsteps, int use_graph)
              * (the steps queued so far finish first); once it no longer
              * holds, this run rebuilds as the reference does */
             std::vector<double> xh(3 * (size_t)E.natoms);
-            HIPCHK(hipMemcpyAsync(xh.data(), E.x_dev, xh.size() * sizeof(double), hipMemcpyDeviceToHost, E.stream));
-            HIPCHK(hipStreamSynchronize(E.stream));
+            if ((rc = engine_d2h(E, xh.data(), E.x_dev, xh.size() * sizeof(double)))) return rc;
             md_static_bound(E, M, xh.data(), step - 1);
             recheck = M.static_until > step - 1;
         }
@@ -933,9 +946,10 @@ extern "C" int gbhip_md_state(gbhip_ctx *ctx, double *x, double *v, double *grad
     MDState &M = ctx->M;
     if (!M.ready) return seterr(GBHIP_ESTATE, "gbhip_md_init has not been called");
     const size_t nb = 3 * sizeof(double) * (size_t)E.natoms;
-    if (x) HIPCHK(hipMemcpyAsync(x, E.x_dev, nb, hipMemcpyDeviceToHost, E.stream));
-    if (v) HIPCHK(hipMemcpyAsync(v, M.v, nb, hipMemcpyDeviceToHost, E.stream));
-    if (grad) HIPCHK(hipMemcpyAsync(grad, E.g_dev, nb, hipMemcpyDeviceToHost, E.stream));
+    int rc;
+    if (x && (rc = engine_d2h(E, x, E.x_dev, nb))) return rc;
+    if (v && (rc = engine_d2h(E, v, M.v, nb))) return rc;
+    if (grad && (rc = engine_d2h(E, grad, E.g_dev, nb))) return rc;
     HIPCHK(hipStreamSynchronize(E.stream));
     return 0;
 }
@@ -948,8 +962,8 @@ extern "C" int gbhip_md_log(gbhip_ctx *ctx, double *rows, int max_rows, int *nro
     const int n = std::min(std::min(M.steps_done, M.log_cap), max_rows);
     /* rows of the last n steps in step order */
     std::vector<double> all(16 * (size_t)M.log_cap);
-    HIPCHK(hipMemcpyAsync(all.data(), M.log, all.size() * sizeof(double), hipMemcpyDeviceToHost, E.stream));
-    HIPCHK(hipStreamSynchronize(E.stream));
+    const int rc = engine_d2h(E, all.data(), M.log, all.size() * sizeof(double));
+    if (rc) return rc;
     for (int r = 0; r < n; r++) {
         const int step = M.steps_done - n + 1 + r;
         const double *src = all.data() + 16 * (size_t)((step - 1) % M.log_cap);

@@ -239,6 +239,12 @@ int nabc_model_build(nabc_model *m, PARMSTRUCT_T *prm, const int *frozen, const 
 int nabc_model_context(const nabc_model *m, const double *x, const nabc_opts_t *o, void *stream,
                        gbhip_ctx **out)
 {
+    return nabc_model_context_in(m, x, o, stream, NULL, out);
+}
+
+int nabc_model_context_in(const nabc_model *m, const double *x, const nabc_opts_t *o, void *stream,
+                          gbhip_slot *slot, gbhip_ctx **out)
+{
     const PARMSTRUCT_T *p = m->prm;
     gbhip_topology t;
     gbhip_options op;
@@ -271,7 +277,8 @@ int nabc_model_context(const nabc_model *m, const double *x, const nabc_opts_t *
     op.gbneckscale = m->gbneckscale;
     op.kappa = o->kappa;
     op.epsext = o->epsext;
-    if (gbhip_ctx_create_on(&t, &op, x, stream, &ctx) != 0) {
+    if ((slot ? gbhip_ctx_create_in_slot(&t, &op, x, slot, &ctx) : gbhip_ctx_create_on(&t, &op, x, stream, &ctx)) !=
+        0) {
         nabc_seterr("%s", gbhip_last_error());
         return -1;
     }

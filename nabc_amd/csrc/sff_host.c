@@ -77,6 +77,19 @@ void mme_get_energies(double ene[13], double *frms)
     if (frms) *frms = S.ene[13];
 }
 
+/* the ff: line of an evaluation (eff.c:2899-2927): the header at iteration
+ * 0, then every ntpr iterations */
+static void mme_report(int it, const double *e)
+{
+    const nabc_opts_t *o = &nabc_opts;
+    if (it == 0) fprintf(out(), "      iter    Total       bad      vdW     elect   nonpolar   genBorn      frms\n");
+    if (it > -1 && (it == 0 || it % o->ntpr == 0)) {
+        fprintf(out(), "ff:%6d %9.2f %9.2f %9.2f %9.2f %9.2f %9.2f %9.2e\n", it, e[0], e[3] + e[4] + e[5],
+                e[1] + e[7], e[2] + e[8], e[9] + e[11] + e[12], e[10], e[13]);
+        fflush(out());
+    }
+}
+
 /* mme() / mme34(), sff.c:1854, eff.c:2442-2947 */
 double mme(double *x, double *f, int *iter)
 {
@@ -91,7 +104,6 @@ double mme(double *x, double *f, int *iter)
         return NAN;
     }
     if (ensure_ctx(x) != 0) return NAN;
-    if (it == 0) fprintf(out(), "      iter    Total       bad      vdW     elect   nonpolar   genBorn      frms\n");
     if (gbhip_set_coords(S.ctx, x) != 0) {
         nabc_seterr("%s", gbhip_last_error());
         return NAN;
@@ -109,6 +121,7 @@ double mme(double *x, double *f, int *iter)
         nabc_seterr("%s", gbhip_last_error());
         return NAN;
     }
+    mme_report(it, S.ene);
     if (o->gb_debug) {
         int i, n = S.m.prm->Natom;
         double *reff = (double *)malloc(sizeof(double) * n);
@@ -117,12 +130,6 @@ double mme(double *x, double *f, int *iter)
             for (i = 0; i < n; i++) fprintf(out(), "%d\t%15.7f\t%15.7f\n", i + 1, S.m.prm->Rborn[i], reff[i]);
         }
         free(reff);
-    }
-    if (it > -1 && (it == 0 || it % o->ntpr == 0)) {
-        const double *e = S.ene;
-        fprintf(out(), "ff:%6d %9.2f %9.2f %9.2f %9.2f %9.2f %9.2f %9.2e\n", it, e[0], e[3] + e[4] + e[5],
-                e[1] + e[7], e[2] + e[8], e[9] + e[11] + e[12], e[10], e[13]);
-        fflush(out());
     }
     if (it == -1) {
         const double *e = S.ene;
@@ -359,8 +366,46 @@ int md(int n, int maxstep, double *x, double *f, double *v, double (*func)(doubl
 enum cg_state { CG_EVAL, CG_RESET_DIR, CG_TEST, CG_NEW_SEARCH, CG_STEP, CG_SPLINE, CG_NEXT_STEP,
                 CG_NEW_DIR, CG_FINISH, CG_DONE };
 
+static int conjgrad_host(double x[], int *n, double *f, double (*func)(double *, double *, int *),
+                         double *rmsgrad, double *dfpred, int *maxiter);
+
+/* each evaluation of the device-resident minimiser: what mme() records and prints */
+static void cg_device_report(int it, const double *ene14, void *user)
+{
+    (void)user;
+    memcpy(S.ene, ene14, sizeof(S.ene));
+    mme_report(it, S.ene);
+}
+
+/*
+ * conjgrad over this library's own mme(): the whole minimiser runs with its
+ * vectors on the device (gbhip_conjgrad), only scalars cross per
+ * evaluation; the ff: lines are printed as mme() prints them.
+ */
+static int conjgrad_device(double x[], int n, double *f, double rmsgrad, double dfpred, int maxiter)
+{
+    const nabc_opts_t *o = &nabc_opts;
+    int ret = 0;
+    if (ensure_ctx(x) != 0) return -1;
+    if (gbhip_conjgrad(S.ctx, x, n, f, rmsgrad, dfpred, maxiter, o->nsnb > 0 ? o->nsnb : 1, cg_device_report, NULL,
+                       &ret) != 0) {
+        nabc_seterr("%s", gbhip_last_error());
+        return -1;
+    }
+    S.have_list = 1;
+    return ret;
+}
+
 int conjgrad(double x[], int *n, double *f, double (*func)(double *, double *, int *), double *rmsgrad,
              double *dfpred, int *maxiter)
+{
+    if (func == mme && S.m.prm && *n == 3 * S.m.prm->Natom && !nabc_opts.gb_debug)
+        return conjgrad_device(x, *n, f, *rmsgrad, *dfpred, *maxiter);
+    return conjgrad_host(x, n, f, func, rmsgrad, dfpred, maxiter);
+}
+
+static int conjgrad_host(double x[], int *n, double *f, double (*func)(double *, double *, int *),
+                         double *rmsgrad, double *dfpred, int *maxiter)
 {
     const int N = *n, maxlin = 10, mxfcon = 4;
     double *d = (double *)calloc(N, sizeof(double));      /* search direction (w[1..n]) */

@@ -109,3 +109,26 @@ def test_dispatcher_rejects_oversized_and_reports_failures(gen_dir):
         else:
             assert r["status"] == 0 and r["energy"] == float(k)
     assert any(r["status"] == -2 for r in res) and run.max_mem <= budget
+
+
+@pytest.mark.timeout(60)
+def test_pool_blocks_of_resident_jobs_never_overlap(gen_dir):
+    """Each admitted job holds a contiguous block of its footprint in the
+    device pool (first fit, 4 KiB granules); the blocks of jobs resident at
+    the same time are disjoint and inside the pool."""
+    parms = _parms(gen_dir)
+    jobs = _jobs(parms, 400, rate=3000.0, seed=9)
+    big = max(_footprint(p, na.GBSERVE_MD) for p in parms)
+    budget = int(big * 2.5)
+    rng = np.random.default_rng(10)
+    run = _Runner(jobs, rng.uniform(0.0, 0.003, len(jobs)))
+    res = na.serve_custom(jobs, run, slots=6, hbm_budget=budget)
+    rnd = lambda b: (b + 4095) // 4096 * 4096
+    iv = [(r["start_s"], r["end_s"], r["pool_offset"], r["pool_offset"] + rnd(r["footprint"])) for r in res]
+    for a0, a1, o0, o1 in iv:
+        assert 0 <= o0 < o1 <= budget
+    for i in range(len(iv)):
+        for j in range(i + 1, len(iv)):
+            (s0, e0, p0, q0), (s1, e1, p1, q1) = iv[i], iv[j]
+            if s0 < e1 and s1 < e0:  # resident together
+                assert q0 <= p1 or q1 <= p0, (i, j, iv[i], iv[j])
