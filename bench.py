@@ -364,13 +364,13 @@ def bench_serve(args):
             "ntpr=100000000")
     res = na.serve(jobs, slots=args.slots, hbm_budget=int(args.hbm_gb * (1 << 30)), precision=na.GBHIP_MIXED,
                    check_fp64=True, mm_opts=opts, cg_rmsgrad=1e-4, cg_dfpred=10.0, progress=20)
-    ok = [r for r in res if r["status"] >= -3]
+    ok = [r for r in res if r["status"] >= -4]  # conjgrad's -1..-4 are ordinary endings
     lat = np.array([r["latency_s"] for r in ok])
     span = max(r["end_s"] for r in ok) - min(j["arrival_s"] for j in jobs)
     err = np.array([abs(r["energy_final_eval"] - r["energy_fp64"]) / abs(r["energy_fp64"]) for r in ok
                     if r["energy_fp64"] != 0.0])
-    lat_cg = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 0 and r["status"] >= -3])
-    lat_md = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 1 and r["status"] >= -3])
+    lat_cg = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 0 and r["status"] >= -4])
+    lat_md = np.array([r["latency_s"] for r, j in zip(res, jobs) if j["kind"] == 1 and r["status"] >= -4])
     out = {"metric": "serving p99 latency at a fixed request rate (CG/MD jobs, 300-8000 atoms)",
            "value": round(float(np.percentile(lat, 99)), 4), "unit": "s", "higher_is_better": False,
            "request_rate_per_s": args.rate, "jobs": n, "completed": len(ok), "slots": args.slots,
@@ -380,10 +380,67 @@ def bench_serve(args):
            "p50_cg_s": round(float(np.percentile(lat_cg, 50)), 4) if len(lat_cg) else None,
            "p50_md_s": round(float(np.percentile(lat_md, 50)), 4) if len(lat_md) else None,
            "max_resident_jobs": max(r["max_active"] for r in res),
+           "p50_setup_ms": round(1e3 * float(np.median([r["setup_s"] for r in ok])), 3),
            "mixed_vs_fp64_final_energy_rel_err": {"median": float(np.median(err)), "max": float(err.max())},
            "atoms_mean": float(sizes.mean()), "generate_s": round(t_gen, 1),
            "job_mix": "Bernoulli(1/2) of CG maxiter=50 / MD 500 steps, seed 55"}
     print(json.dumps(out))
+
+
+def bench_cg(args):
+    """Minimiser: conjgrad() over mme() with its vectors on the device
+    (cgdevice=1, gbhip_conjgrad) and on the host (cgdevice=0, the
+    reference's arrangement: coordinates in and gradient out every
+    evaluation) at 600, 2,500 and 8,000 atoms (gb=5, cut = rgbmax = 20 A,
+    nsnb 10, mixed precision), maxiter iterations from the generated start;
+    wall time per evaluation beside the device time of one mme() evaluation
+    (captured graph, back to back)."""
+    import numpy as np
+    import nabc_amd as na
+    maxiter = args.cg_iters
+    rows = []
+    for natom in (600, 2500, 8000):
+        parm, arr, x0 = na.build_system(natom, 4242 + natom)
+        row = {"natom": natom}
+        for dev in (1, 0):
+            na.lib.mm_reset_options()
+            opts = "gb=5, cut=20.0, rgbmax=20.0, nsnb=10, ntpr=100000000, gbprec=mixed, cgdevice=%d" % dev
+            assert na.lib.mm_options(opts.encode()) == 0
+            assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+            best = None
+            for rep in range(3):  # the first run captures graphs and stages memory
+                x = np.ascontiguousarray(x0.copy())
+                n = na.C.c_int(3 * natom)
+                f = na.C.c_double()
+                rms = na.C.c_double(1e-12)
+                dfp = na.C.c_double(10.0)
+                mx = na.C.c_int(maxiter)
+                t0 = time.perf_counter()
+                ret = na.lib.conjgrad(x.ctypes.data_as(na.dp), na.C.byref(n), na.C.byref(f), na.MME_C,
+                                      na.C.byref(rms), na.C.byref(dfp), na.C.byref(mx))
+                dt = time.perf_counter() - t0
+                if rep > 0 and (best is None or dt < best[0]):
+                    best = (dt, ret, f.value)
+            nev = maxiter + 1 if best[1] == -3 else abs(best[1])
+            row["device" if dev else "host"] = {"us_per_eval": round(1e6 * best[0] / nev, 1), "return": best[1],
+                                                "final_energy": best[2]}
+            if dev:
+                # device time of one evaluation: back-to-back captured graphs
+                ctx = na.lib.mme_context(x0.ctypes.data_as(na.dp))
+                for _ in range(3):
+                    na.lib.gbhip_mme(ctx, None, None)
+                na.lib.gbhip_sync(ctx)
+                t0 = time.perf_counter()
+                for _ in range(50):
+                    na.lib.gbhip_mme(ctx, None, None)
+                row["mme_us_with_wait"] = round(1e6 * (time.perf_counter() - t0) / 50, 1)
+            it = na.C.c_int(-3)
+            na.lib.mme(x0.ctypes.data_as(na.dp), x0.copy().ctypes.data_as(na.dp), na.C.byref(it))
+        row["host_over_device"] = round(row["host"]["us_per_eval"] / row["device"]["us_per_eval"], 3)
+        rows.append(row)
+        print("# %s" % json.dumps(row), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "conjgrad wall time per evaluation, device vs host vectors",
+                      "unit": "us/evaluation", "iterations": maxiter, "rows": rows}))
 
 
 def bench_sweep(args):
@@ -438,7 +495,7 @@ def main():
                          "reproduces a tuned run's shapes under a profiler")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy", "sweep"],
+    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
                          "errors against the oracle at C1/C2/C3; sweep: evaluation rate at 5k-100k atoms")
@@ -451,6 +508,7 @@ def main():
     ap.add_argument("--md-steps", type=int, default=0,
                     help="MD steps (default per config: c1 1000, c2 10000, c3 5000)")
     ap.add_argument("--md-min", type=int, default=500, help="CG iterations before dynamics")
+    ap.add_argument("--cg-iters", type=int, default=100, help="conjgrad iterations of --config cg")
     ap.add_argument("--md-no-graph", action="store_true",
                     help="launch each step's kernels on the stream instead of replaying the captured graphs")
     args = ap.parse_args()
@@ -468,6 +526,8 @@ def main():
         return bench_accuracy(args)
     if args.config == "sweep":
         return bench_sweep(args)
+    if args.config == "cg":
+        return bench_cg(args)
 
     import numpy as np
     import nabc_amd as na

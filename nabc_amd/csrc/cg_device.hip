@@ -11,9 +11,9 @@
  * energies of each evaluation and the dot products and step size the
  * line search branches on.  Dot products are reduced in a fixed order
  * (per-block partial sums over a fixed grid, then one block), so a run is
- * reproducible.  Per evaluation the host waits once (energies, g.d and
- * g.g come back together); a line search adds one wait for g.d of the new
- * direction and one for the three restart products.
+ * reproducible.  Per evaluation the host waits once: the energies and
+ * every product of the new gradient the line search and the next
+ * direction can need come back together.
  *
  * A line-search step x = xopt + stepch d is evaluated together with the
  * test that it moved x at all (the reference's "work > 0", conjgrad.c:
@@ -32,10 +32,10 @@
 
 #define CG_THREADS 256
 #define CG_BLOCKS 256 /* fixed grid of the reductions: fixed summation order */
-#define CG_MAXDOT 4
+#define CG_MAXDOT 5
 
 /* slots of CGState::res */
-enum { R_DOT0 = 0, R_WORK = 4, R_NRES = 8 };
+enum { R_DOT0 = 0, R_WORK = 5, R_NRES = 8 };
 
 __device__ __forceinline__ double cg_block_sum(double v, double *s_red)
 {
@@ -69,7 +69,7 @@ struct DotArgs {
 __global__ __launch_bounds__(CG_THREADS) void k_cg_dots(DotArgs A, int nd, long long n, double *part)
 {
     __shared__ double s_red[CG_THREADS / 64];
-    double acc[CG_MAXDOT] = {0.0, 0.0, 0.0, 0.0};
+    double acc[CG_MAXDOT] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (long long i = (long long)blockIdx.x * CG_THREADS + threadIdx.x; i < n; i += (long long)CG_BLOCKS * CG_THREADS)
         for (int k = 0; k < CG_MAXDOT; k++)
             if (k < nd) acc[k] += A.a[k][i] * A.b[k][i];
@@ -204,10 +204,13 @@ struct CGRun {
     /*
      * Evaluation at the current x (after a step when step != nullptr):
      * the pair list rebuilt at the reference's cadence for iteration it,
-     * mme, then d.g and g.g; one wait.  Returns in *work the step's largest
+     * mme, then the products of the new gradient g the minimiser can need
+     * before the next evaluation -- d.g, g.g and, for the next direction,
+     * g.ginit, g.rsdg, g.rsdx (with the vectors as they are during this
+     * line search); one wait.  Returns in *work the step's largest
      * coordinate change (step evaluations only) and whether it rebuilt.
      */
-    int eval(int it, const double *step, double *ene14, double *dg, double *gg, double *work, bool *rebuilt)
+    int eval(int it, const double *step, double *ene14, double *pr, double *work, bool *rebuilt)
     {
         int rc;
         if (step) {
@@ -222,15 +225,15 @@ struct CGRun {
             force_rebuild = false;
         }
         if ((rc = engine_mme(c))) return rc;
-        const double *a[2] = {c->C.d, E.g_dev}, *b[2] = {E.g_dev, E.g_dev};
-        if ((rc = launch_dots(2, a, b))) return rc;
-        hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(CG_THREADS), 0, E.stream, c->C.part, 2, step ? 1 : 0,
+        const double *a[5] = {c->C.d, E.g_dev, E.g_dev, E.g_dev, E.g_dev};
+        const double *b[5] = {E.g_dev, E.g_dev, c->C.ginit, c->C.rsdg, c->C.rsdx};
+        if ((rc = launch_dots(5, a, b))) return rc;
+        hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(CG_THREADS), 0, E.stream, c->C.part, 5, step ? 1 : 0,
                            c->C.res);
         LAUNCHCHK();
         double r[R_NRES];
         if ((rc = read(r, R_NRES, ene14))) return rc;
-        *dg = r[R_DOT0];
-        *gg = r[R_DOT0 + 1];
+        for (int k = 0; k < 5; k++) pr[k] = r[R_DOT0 + k];
         if (work) *work = r[R_WORK];
         return 0;
     }
@@ -278,21 +281,40 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
     double gama = 0, beta = 0, fmin = 0, gmin = 0, dfpr = 0, gnew = 0, step = 0, work = 0, finit = 0, ginit = 0;
     double gsqrd = 0, gspln, stmin = 0, gamden = 0, ddspln = 0, stepch = 0, sbound = 0, fch = 0, sum = 0;
     int niter = 0, nfbeg = 0, iterc = 0, iterfm = 0, iterrs = 0, iretry = 0, nfopt = 0, ier = 0;
-    /* products of the gradient now in g with the current d, and with itself;
-     * the same for the best point's gradient */
-    double cur_dg = 0, cur_gg = 0, opt_dg = 0, opt_gg = 0;
+    /*
+     * Products of the gradient now in g (pc) and of the best point's (po):
+     * [0] d.g, [1] g.g, [2] g.ginit, [3] g.rsdg, [4] g.rsdx.  Each set
+     * carries the epochs of d ([0]) and of ginit/rsdg/rsdx ([2..4]) it was
+     * formed with; a vector update bumps its epoch, d.g of a new direction
+     * follows from the old products (d' = -g + beta d + gama rsdx gives
+     * d'.g = -g.g + beta d.g + gama g.rsdx), and a product whose epoch is
+     * stale when it is needed is formed again on the device.
+     */
+    struct Prod {
+        double v[5];
+        long ed, ev;
+    } pc{}, po{};
+    long ed = 0, ev = 0;
     double ene14[14];
     bool rebuilt = false;
     cg_state st = CG_EVAL;
 
+    auto refresh = [&](bool want_d, bool want_v) -> int {
+        if (!(want_d && pc.ed != ed) && !(want_v && pc.ev != ev)) return 0;
+        const double *a[5] = {C.d, E.g_dev, E.g_dev, E.g_dev, E.g_dev};
+        const double *b[5] = {E.g_dev, E.g_dev, C.ginit, C.rsdg, C.rsdx};
+        const int r2 = R.dots(5, a, b, pc.v);
+        pc.ed = ed;
+        pc.ev = ev;
+        return r2;
+    };
     auto save_best = [&]() -> int {
         int r2;
         fmin = *f;
         gsqrd = sum;
         nfopt = niter;
         if ((r2 = R.copy(C.xopt, E.x_dev)) || (r2 = R.copy(C.gopt, E.g_dev))) return r2;
-        opt_dg = cur_dg;
-        opt_gg = cur_gg;
+        po = pc;
         return 0;
     };
 
@@ -300,7 +322,9 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
         switch (st) {
         case CG_EVAL: { /* L10: the first evaluation (later ones come from CG_STEP) */
             ++niter;
-            if ((rc = R.eval(niter - 1, nullptr, ene14, &cur_dg, &cur_gg, nullptr, &rebuilt))) return rc;
+            if ((rc = R.eval(niter - 1, nullptr, ene14, pc.v, nullptr, &rebuilt))) return rc;
+            pc.ed = ed;
+            pc.ev = ev;
             *f = ene14[0];
             if (report) report(niter - 1, ene14, user);
             st = niter >= 2 ? CG_TEST : CG_RESET_DIR;
@@ -310,13 +334,16 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             hipLaunchKernelGGL(k_cg_dir, dim3(R.nblk), dim3(CG_THREADS), 0, E.stream, C.d, (const double *)E.g_dev,
                                0.0, nullptr, 0.0, (long long)n);
             LAUNCHCHK();
-            cur_dg = -cur_gg; /* (-g).g summed in the order g.g is */
+            ++ed;
+            pc.v[0] = -pc.v[1]; /* (-g).g summed in the order g.g is */
+            pc.ed = ed;
             iterrs = 0;
             st = iterc > 0 ? CG_NEW_SEARCH : CG_TEST;
             break;
         case CG_TEST: /* L30 */
-            gnew = cur_dg;
-            sum = cur_gg;
+            if ((rc = refresh(true, false))) return rc;
+            gnew = pc.v[0];
+            sum = pc.v[1];
             if (niter == 1) {
                 if ((rc = save_best())) return rc;
                 if (sum <= dgrad) { st = CG_DONE; break; }
@@ -336,8 +363,10 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
         case CG_NEW_SEARCH: /* L40 */
             ++iterc;
             finit = *f;
+            if ((rc = refresh(true, false))) return rc;
             if ((rc = R.copy(C.ginit, E.g_dev))) return rc;
-            ginit = cur_dg;
+            ++ev;
+            ginit = pc.v[0];
             if (ginit >= 0.0) { ier = -2; st = CG_FINISH; break; }
             gmin = ginit;
             sbound = -1.0;
@@ -352,12 +381,14 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             break;
         case CG_STEP: { /* L50, with the evaluation of L10 when the step moved x */
             step = stmin + stepch;
-            double e14[14], dg = 0, gg = 0;
-            if ((rc = R.eval(niter, &stepch, e14, &dg, &gg, &work, &rebuilt))) return rc;
+            double e14[14];
+            Prod pn;
+            if ((rc = R.eval(niter, &stepch, e14, pn.v, &work, &rebuilt))) return rc;
             if (work > 0.0) {
                 ++niter;
-                cur_dg = dg;
-                cur_gg = gg;
+                pc = pn;
+                pc.ed = ed;
+                pc.ev = ev;
                 memcpy(ene14, e14, sizeof(ene14));
                 *f = ene14[0];
                 if (report) report(niter - 1, ene14, user);
@@ -368,8 +399,7 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
              * never makes it; the gradient goes back to the best point's */
             if (rebuilt) R.force_rebuild = true;
             if ((rc = R.copy(E.g_dev, C.gopt))) return rc;
-            cur_dg = opt_dg;
-            cur_gg = opt_gg;
+            pc = po;
             if (niter > nfbeg + 1) ier = -1;
             if (fabs(gmin / ginit) > 0.2) ier = -1;
             st = CG_FINISH;
@@ -401,10 +431,8 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             st = CG_STEP;
             break;
         case CG_NEW_DIR: { /* L80 */
-            double pr[3];
-            const double *a[3] = {E.g_dev, E.g_dev, E.g_dev}, *b[3] = {C.ginit, C.rsdg, C.rsdx};
-            if ((rc = R.dots(3, a, b, pr))) return rc;
-            sum = pr[0];
+            if ((rc = refresh(true, true))) return rc;
+            sum = pc.v[2];
             beta = (gsqrd - sum) / (gmin - ginit);
             if (fabs(beta * gmin) > gsqrd * 0.2) {
                 ++iretry;
@@ -419,13 +447,14 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             dfpr = stmin * ginit;
             if (iretry > 0) { st = CG_RESET_DIR; break; }
             if (iterrs != 0 && iterc - iterrs < n && fabs(sum) < gsqrd * 0.2) {
-                gama = pr[1] / gamden;
-                if (fabs(beta * gmin + gama * pr[2]) < gsqrd * 0.2) {
+                gama = pc.v[3] / gamden;
+                if (fabs(beta * gmin + gama * pc.v[4]) < gsqrd * 0.2) {
                     hipLaunchKernelGGL(k_cg_dir, dim3(R.nblk), dim3(CG_THREADS), 0, E.stream, C.d,
                                        (const double *)E.g_dev, beta, (const double *)C.rsdx, gama, (long long)n);
                     LAUNCHCHK();
-                    const double *a1[1] = {C.d}, *b1[1] = {E.g_dev};
-                    if ((rc = R.dots(1, a1, b1, &cur_dg))) return rc;
+                    ++ed;
+                    pc.v[0] = -pc.v[1] + beta * pc.v[0] + gama * pc.v[4];
+                    pc.ed = ed;
                     st = CG_NEW_SEARCH;
                     break;
                 }
@@ -434,10 +463,10 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             hipLaunchKernelGGL(k_cg_restart, dim3(R.nblk), dim3(CG_THREADS), 0, E.stream, C.d, C.rsdx, C.rsdg,
                                (const double *)E.g_dev, (const double *)C.ginit, beta, (long long)n);
             LAUNCHCHK();
-            {
-                const double *a1[1] = {C.d}, *b1[1] = {E.g_dev};
-                if ((rc = R.dots(1, a1, b1, &cur_dg))) return rc;
-            }
+            ++ed;
+            ++ev;
+            pc.v[0] = -pc.v[1] + beta * pc.v[0];
+            pc.ed = ed;
             iterrs = iterc;
             st = CG_NEW_SEARCH;
             break;
@@ -446,8 +475,7 @@ extern "C" int gbhip_conjgrad(gbhip_ctx *ctx, double *x, int n, double *f, doubl
             if (niter != nfopt) {
                 *f = fmin;
                 if ((rc = R.copy(E.x_dev, C.xopt)) || (rc = R.copy(E.g_dev, C.gopt))) return rc;
-                cur_dg = opt_dg;
-                cur_gg = opt_gg;
+                pc = po;
             }
             st = ier == 0 ? CG_NEW_DIR : CG_DONE;
             break;

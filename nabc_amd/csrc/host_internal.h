@@ -28,6 +28,7 @@ typedef struct {
     /* nabc_amd extensions */
     int gbprec;          /* 0 = fp64, 1 = mixed fp32-pair/fp64-accumulate */
     unsigned long long md_seed; /* Langevin noise stream seed */
+    int cgdevice;        /* conjgrad over mme: 1 = vectors on the device (default), 0 = on the host */
 } nabc_opts_t;
 
 extern nabc_opts_t nabc_opts;

@@ -8,8 +8,10 @@
  * unknown key or a malformed value is an error, as there (mm_options.l:101);
  * here the message goes to stderr and the call returns -1 instead of exiting.
  *
- * Two keys are additions of this library: gbprec=fp64|mixed selects the GB
- * arithmetic, md_seed=<int> seeds the device Langevin noise stream.
+ * Three keys are additions of this library: gbprec=fp64|mixed selects the
+ * GB arithmetic, md_seed=<int> seeds the device Langevin noise stream, and
+ * cgdevice=0|1 chooses where conjgrad() over mme() keeps its vectors (1,
+ * the default: on the device; 0: on the host, as the reference does).
  */
 #include <ctype.h>
 #include <stdarg.h>
@@ -53,7 +55,7 @@ void mm_reset_options(void)
     o->temp0 = 300.0; o->boltz2 = 9.93595e-4; o->vlimit = 10.0;
     o->genmass = 10.0; o->ntpr_md = 10; o->ntwx = 0; o->zerov = 0;
     o->tempi = 0.0; o->irattle = 0; o->dim = 3;
-    o->gbprec = 0; o->md_seed = 71277ULL;
+    o->gbprec = 0; o->md_seed = 71277ULL; o->cgdevice = 1;
 }
 
 typedef enum { K_DBL, K_INT, K_POSINT, K_NNINT, K_SET, K_DIEL, K_PREC, K_U64 } kind_t;
@@ -97,6 +99,7 @@ static const keydef_t keys[] = {
     {"dielc", K_DBL, &O->dielc, 0}, {"rattle", K_SET, &O->irattle, "012"},
     {"wdssp", K_DBL, &O->wdssp, 0},
     {"gbprec", K_PREC, &O->gbprec, 0}, {"md_seed", K_U64, &O->md_seed, 0},
+    {"cgdevice", K_SET, &O->cgdevice, "01"},
 };
 #undef O
 

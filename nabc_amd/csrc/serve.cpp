@@ -320,7 +320,7 @@ extern "C" size_t gbhip_footprint_bytes(int natom, int md_steps)
     b += (size_t)natom * (4 + 4 + 12 + 2 * 24 + 8);       /* inverse order, exclusions, x, g, scratch */
     b += (size_t)natom * 220;                              /* bonded / 1-4 tables (chain topology bound) */
     b += (size_t)natom * 4 * 24 + 16 * 8 * (size_t)(md_steps > 0 ? md_steps : 0); /* dynamics state + log */
-    b += (size_t)natom * 6 * 24 + 5 * 256 * 8 + 64;        /* minimiser vectors and reductions */
+    b += (size_t)natom * 6 * 24 + 6 * 256 * 8 + 64;        /* minimiser vectors and reductions */
     b += 3 * 64 * 8;                                       /* energy partial sums */
     b += 96 * 256;                                         /* 256-byte alignment of each array */
     return b;

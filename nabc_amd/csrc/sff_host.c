@@ -399,7 +399,7 @@ static int conjgrad_device(double x[], int n, double *f, double rmsgrad, double 
 int conjgrad(double x[], int *n, double *f, double (*func)(double *, double *, int *), double *rmsgrad,
              double *dfpred, int *maxiter)
 {
-    if (func == mme && S.m.prm && *n == 3 * S.m.prm->Natom && !nabc_opts.gb_debug)
+    if (func == mme && S.m.prm && *n == 3 * S.m.prm->Natom && !nabc_opts.gb_debug && nabc_opts.cgdevice)
         return conjgrad_device(x, *n, f, *rmsgrad, *dfpred, *maxiter);
     return conjgrad_host(x, n, f, func, rmsgrad, dfpred, maxiter);
 }
