@@ -136,6 +136,8 @@ struct Engine {
     int nmol = 0, natoms = 0, nslots = 0, ntiles = 0;
     long long tp_cap = 0;
     int precision = GBHIP_FP64;
+    /* FP64 pass 1 with the logarithmic descreening cases queued (k_born_q) */
+    bool born_queue = true;
     int pair_blocks = 0;       /* workgroups of the list-build kernels (16 per CU, at most the tile pairs) */
     int pass_blocks[3] = {0, 0, 0}; /* per pass: born, pairs, chain */
     int ncu = 0;

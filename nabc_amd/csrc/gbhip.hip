@@ -604,6 +604,194 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
 }
 
 /*
+ * Pass 1 in FP64 with the logarithmic cases and the neck term deferred.  In
+ * FP64 the log and the divisions of the descreening cases A, C, D and E,
+ * and the division of the neck term, are long library sequences; at a 25 A
+ * cutoff about one pair direction in five needs one of them (the pairs
+ * near rgbmax fall in case A, the near pairs in C-E or within the neck
+ * range, the rest are case B, the Taylor series, or skipped), so computed
+ * in the sweep nearly every wavefront step would pay them for a few lanes.
+ * Here each step computes the case-B directions at once and pushes every
+ * other direction that contributes onto a per-workgroup queue in LDS (the
+ * lane and column of the pair, r^2, 1/dij and where its derivative-store
+ * entry goes); whenever 64 are queued the whole wavefront takes one each
+ * and evaluates descreen_d and the neck term with every lane active, and
+ * the rest at the end of the visit.  Sums and store entries are the same
+ * as k_born's: the pair terms are the same functions of the same operands,
+ * only the order of the FP64 additions into the atom sums differs.
+ */
+#define BQ_CAP 128
+struct BornQueue {
+    double r2[BQ_CAP], dij1i[BQ_CAP];
+    double *out[BQ_CAP];   /* derivative-store component, or null */
+    int code[BQ_CAP];      /* row lane | column << 6 | (1 << 12: direction j by i) */
+};
+
+/* per-visit LDS copies of the two tiles' atom data the queue needs */
+struct BornTiles {
+    double ri[TILE], ri1i[TILE], sj[TILE], rb[TILE]; /* column atoms */
+    double r_ri[TILE], r_ri1i[TILE], r_sj[TILE], r_rb[TILE]; /* row atoms */
+    int nk[TILE], r_nk[TILE]; /* neck indices of the column and row atoms */
+};
+
+/* the queued directions at positions [h, h + n) of the ring, one per lane */
+__device__ __forceinline__ void born_queue_run(BornQueue &Q, int h, int n, const BornTiles &T, const double *npos,
+                                               const double *nval, double *s_row, double *s_col, const GBPar &p)
+{
+    const int l = threadIdx.x;
+    if (l >= n) return;
+    const int e = (h + l) & (BQ_CAP - 1);
+    const int code = Q.code[e], lane = code & 63, col = (code >> 6) & 63;
+    const bool ji = code >> 12;
+    /* direction i by j: the row atom descreened by the column atom's
+     * sphere; direction j by i the other way round */
+    const double ri = ji ? T.ri[col] : T.r_ri[lane], ri1i = ji ? T.ri1i[col] : T.r_ri1i[lane];
+    const double sj = ji ? T.r_sj[lane] : T.sj[col];
+    const double r2 = Q.r2[e], dij1i = Q.dij1i[e], dij = r2 * dij1i;
+    bool skip;
+    double dv;
+    double v = descreen_d<double>(ri, ri1i, sj, r2, dij, dij1i, p.rgbmax, p.rgbmax1i, p.rgbmax2i, skip, dv);
+    if (p.has_neck && !skip && dij < T.r_rb[lane] + T.rb[col] + NECK_CUT) {
+        const int nidx = ji ? T.nk[col] * NECK_N + T.r_nk[lane] : T.r_nk[lane] * NECK_N + T.nk[col];
+        double dn;
+        v += neck_d<double>(dij, dij1i, npos[nidx], nval[nidx], p.neckscale, dn);
+        dv += dn;
+    }
+    atomicAdd(ji ? &s_col[col] : &s_row[lane], -v);
+    if (Q.out[e]) *Q.out[e] = dv;
+}
+
+template <bool SPL>
+__global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
+{
+    __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE], s_row[TILE];
+    __shared__ BornTiles T;
+    __shared__ double npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
+    __shared__ BornQueue Q;
+    const int l = threadIdx.x;
+    const int nnz = *d.nnz;
+    const bool store = *d.store_ok;
+    double2 *st = (double2 *)d.dstore;
+    const double rgbmax = p.rgbmax;
+    if (p.has_neck) {
+        for (int k = l; k < NECK_N * NECK_N; k += TILE) {
+            npos[k] = d.neck_pos[k];
+            nval[k] = d.neck_val[k];
+        }
+    }
+    const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
+    for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
+        const QMeta mq = d.qm[q >> sh];
+        const int t = mq.t;
+        const unsigned long long cols = mq.cols;
+        const int m = __popcll(cols);
+        int sb, se;
+        item_steps(q, sh, m, sb, se);
+        if (sb == se) continue;
+        const int i = mq.a * TILE + l, j = mq.b * TILE + l;
+        const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
+        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
+        const double ri = d.ri0[i], si = d.sj[i], rbi = d.rb[i];
+        const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        const bool own = (cols >> l) & 1ull;
+        const int pos = __popcll(cols & ((1ull << l) - 1ull));
+        double2 *sp = store ? st + mq.start + (sb ? entries_before(mr, sb) : 0) : nullptr;
+        __syncthreads();
+        T.r_ri[l] = ri; T.r_ri1i[l] = gbm<double>::rcp(ri); T.r_sj[l] = si; T.r_rb[l] = rbi;
+        T.r_nk[l] = d.neck[i];
+        s_row[l] = 0.0;
+        if (own) {
+            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            const double rj = d.ri0[j];
+            T.ri[pos] = rj; T.ri1i[pos] = gbm<double>::rcp(rj);
+            T.sj[pos] = d.sj[j]; T.rb[pos] = d.rb[j];
+            T.nk[pos] = d.neck[j];
+            s_acc[pos] = 0.0;
+        }
+        __syncthreads();
+        double acc = 0.0;
+        int qh = 0, qn = 0; /* ring head and length, the same in every lane */
+        int k = (l + sb) % m;
+        for (int s = sb; s < se; s++) {
+            const bool listed = (mr >> s) & 1ull;
+            double aij = 0.0, aji = 0.0, r2 = 0.0, dij1i = 0.0;
+            bool defer_ij = false, defer_ji = false;
+            if (listed) {
+                const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
+                r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= rp2) {
+                    dij1i = gbm<double>::rsqrt_(r2);
+                    const double dij = r2 * dij1i;
+                    const double sj = T.sj[k];
+                    const DsCase cij = ds_case<double>(ri, sj, dij, rgbmax);
+                    const DsCase cji = ds_case<double>(T.ri[k], si, dij, rgbmax);
+                    const bool neck = p.has_neck && dij < rbi + T.rb[k] + NECK_CUT;
+                    defer_ij = !cij.skip && (cij.a || cij.c || cij.d || cij.e || neck);
+                    defer_ji = !cji.skip && (cji.a || cji.c || cji.d || cji.e || neck);
+                    const double dij2i = dij1i * dij1i;
+                    double tij = 0.0, tji = 0.0;
+                    if (cij.b && !defer_ij) {
+                        const double sj2 = sj * sj, tmpsd = sj2 * dij2i;
+                        const double dumbo = GB_TA + tmpsd * (GB_TB + tmpsd * (GB_TC + tmpsd * (GB_TD + tmpsd * GB_TDD)));
+                        const double dumbd = GB_TE + tmpsd * (GB_TF + tmpsd * (GB_TG + tmpsd * (GB_TH + tmpsd * GB_THH)));
+                        tij = sj * tmpsd * dij2i * dumbo;
+                        aij = tmpsd * sj * dij2i * dij2i * dumbd;
+                    }
+                    if (cji.b && !defer_ji) {
+                        const double si2 = si * si, tmpsd = si2 * dij2i;
+                        const double dumbo = GB_TA + tmpsd * (GB_TB + tmpsd * (GB_TC + tmpsd * (GB_TD + tmpsd * GB_TDD)));
+                        const double dumbd = GB_TE + tmpsd * (GB_TF + tmpsd * (GB_TG + tmpsd * (GB_TH + tmpsd * GB_THH)));
+                        tji = si * tmpsd * dij2i * dumbo;
+                        aji = tmpsd * si * dij2i * dij2i * dumbd;
+                    }
+                    acc -= tij;
+                    atomicAdd(&s_acc[k], -tji);
+                }
+            }
+            /* derivative-store entries: the immediate ones now, the
+             * deferred ones when their queue entry runs */
+            double2 *slot = nullptr;
+            if (store) {
+                const unsigned long long bal = __ballot(listed);
+                if (listed) slot = sp + lanes_below(bal);
+                sp += __popcll(bal);
+                if (listed) {
+                    if (!defer_ij) slot->x = aij;
+                    if (!defer_ji) slot->y = aji;
+                }
+            }
+            /* queue the deferred directions, running 64 at a time */
+            for (int dir = 0; dir < 2; dir++) {
+                const bool push = dir ? defer_ji : defer_ij;
+                const unsigned long long bal = __ballot(push);
+                if (push) {
+                    const int e = (qh + qn + lanes_below(bal)) & (BQ_CAP - 1);
+                    Q.r2[e] = r2;
+                    Q.dij1i[e] = dij1i;
+                    Q.out[e] = slot ? (dir ? &slot->y : &slot->x) : nullptr;
+                    Q.code[e] = l | (k << 6) | (dir << 12);
+                }
+                qn += __popcll(bal);
+                if (qn >= TILE) {
+                    __syncthreads();
+                    born_queue_run(Q, qh, TILE, T, npos, nval, s_row, s_acc, p);
+                    __syncthreads();
+                    qh = (qh + TILE) & (BQ_CAP - 1);
+                    qn -= TILE;
+                }
+            }
+            k = k + 1 == m ? 0 : k + 1;
+        }
+        __syncthreads();
+        born_queue_run(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
+        __syncthreads();
+        acc += s_row[l];
+        if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
+        if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+    }
+}
+
+/*
  * Per atom (FP64): effective radius (eff.c:1712-1727), the diagonal GB term
  * and its sumdeijda start value (eff.c:1884-1893), and the factor that turns
  * sumdeijda into the pass-3 weight (eff.c:2212-2231; 0 for frozen atoms).
@@ -1661,7 +1849,10 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], E.stream));
     const bool sp1 = E.par.split_sh[0] > 0, sp2 = E.par.split_sh[1] > 0, sp3 = E.par.split_sh[2] > 0;
-    if (E.precision == GBHIP_FP64)
+    if (E.precision == GBHIP_FP64 && E.born_queue)
+        hipLaunchKernelGGL((sp1 ? k_born_q<true> : k_born_q<false>), dim3(E.pass_blocks[0]), dim3(64), 0, E.stream,
+                           E.d, E.par);
+    else if (E.precision == GBHIP_FP64)
         hipLaunchKernelGGL((sp1 ? k_born<double, true> : k_born<double, false>), dim3(E.pass_blocks[0]), dim3(64), 0,
                            E.stream, E.d, E.par);
     else
