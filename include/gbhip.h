@@ -153,6 +153,28 @@ int gbhip_nblist(gbhip_ctx *ctx, long long *nentries);
 int gbhip_egb(gbhip_ctx *ctx, double *grad, double *epol, double *evdw,
               double *eelec);
 
+/*
+ * One GB energy + gradient of one molecule that allocates nothing on the
+ * device: what gbhip_ctx_create + gbhip_nblist + gbhip_egb +
+ * gbhip_ctx_destroy compute (same kernels, same results), with every device
+ * array taken from a static buffer of GBHIP_STATIC_BYTES that the library
+ * holds on each GPU it runs on until the process ends.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code that
+ * must not call hipMalloc/hipFree, harnesses that forbid it, tight serving
+ * loops).  A system whose arrays do not fit (about 20,000 atoms; the pass-3
+ * derivative store takes what is left, else pass 3 recomputes) is rejected
+ * with GBHIP_EINVAL and a message naming the size.  Synchronous, on the
+ * default stream; grad (3*natom AoS, host, may be NULL) is accumulated
+ * into.  Every call on a GPU shares that GPU's buffer, which keeps the
+ * last call's contents (each call writes what it reads before reading
+ * it): calls that overlapped on one GPU would race, so the library runs
+ * them one at a time under its device lock (each process has its own
+ * buffer).
+ */
+#define GBHIP_STATIC_BYTES (64ull << 20)
+int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const double *x, double *grad,
+                     double *epol, double *evdw, double *eelec);
+
 /* Effective Born radii of the last evaluation, original atom order. */
 int gbhip_get_reff(gbhip_ctx *ctx, double *reff);
 

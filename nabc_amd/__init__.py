@@ -127,6 +127,7 @@ _sig("gbhip_set_coords", C.c_int, [vp, dp])
 _sig("gbhip_nblist", C.c_int, [vp, C.POINTER(C.c_longlong)])
 _sig("gbhip_nblist_async", C.c_int, [vp])
 _sig("gbhip_egb", C.c_int, [vp, dp, dp, dp, dp])
+_sig("gbhip_egb_static", C.c_int, [C.POINTER(Topology), C.POINTER(Options), dp, dp, dp, dp, dp])
 _sig("gbhip_get_reff", C.c_int, [vp, dp])
 _sig("gbhip_get_sumdeijda", C.c_int, [vp, dp])
 _sig("gbhip_set_stream", C.c_int, [vp, vp])
@@ -428,6 +429,20 @@ def make_options(gb, cut, rgbmax, kappa, epsext, gboffset, neckscale, precision=
     o.gboffset, o.gbneckscale = gboffset, neckscale
     o.kappa, o.epsext = kappa, epsext
     return o
+
+
+def egb_static(top, x, gb=5, cut=999.0, rgbmax=999.0, kappa=0.0, epsext=78.5, precision=GBHIP_FP64,
+               frozen=None, atomic_num=None):
+    """One evaluation through gbhip_egb_static (no device allocation).
+    Returns (epol, evdw, eelec, grad[3N])."""
+    keep = _Keep()
+    t, gboffset, neckscale = make_topology(keep, top, gb, frozen, atomic_num)
+    o = make_options(gb, cut, rgbmax, kappa, epsext, gboffset, neckscale, precision)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros(3 * top["natom"])
+    e = [C.c_double() for _ in range(3)]
+    _check(lib.gbhip_egb_static(C.byref(t), C.byref(o), _p(x), _p(g), *[C.byref(v) for v in e]))
+    return e[0].value, e[1].value, e[2].value, g
 
 
 class GBContext:

@@ -43,6 +43,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -2267,6 +2269,116 @@ extern "C" int gbhip_ctx_destroy(gbhip_ctx *ctx)
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     if (!ctx->E.stream) (void)hipDeviceSynchronize();
     delete ctx;
+    return 0;
+}
+
+/* --------------------------------------------- evaluation without allocation */
+
+/*
+ * One GB energy + gradient with every device array in a static buffer of
+ * the library: `scratch`, GBHIP_STATIC_BYTES per GPU, held until the
+ * process ends.  Host code never passes `scratch` itself to the device (its
+ * host-side name is not a device address): the address of each GPU's copy
+ * is looked up once per device with hipGetSymbolAddress and kept in
+ * g_static_ptr.  The context's arrays (atom data, coordinates, tile pairs
+ * and bit matrices, accumulators, energy sums, and the pass-3 derivative
+ * store when room is left) are carved from that address by the slot
+ * mechanism; everything a call reads is written by the same call first
+ * (set-up copies and clears, list build, the gathered coordinates and
+ * cleared accumulators of the evaluation), so nothing of an earlier call
+ * leaks in.  Within the process the calls hold the device lock, so they
+ * run one at a time (and never inside another thread's graph capture);
+ * processes sharing a GPU each have their own copy.
+ */
+static __device__ __align__(256) unsigned char scratch[GBHIP_STATIC_BYTES];
+
+#define STATIC_MAX_DEVICES 64
+static void *g_static_ptr[STATIC_MAX_DEVICES];
+
+/* device bytes engine_build takes for one molecule (every array rounded up
+ * to the 256-byte alignment of the slot allocator), without the store */
+static size_t static_build_bytes(size_t natom, size_t ntypes, size_t nexcl)
+{
+    const size_t NS = (natom + TILE - 1) / TILE * TILE, NT = NS / TILE, cap = NT * (NT + 1) / 2;
+    size_t b = 0;
+    auto add = [&b](size_t bytes) { b += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255; };
+    for (int k = 0; k < 6; k++) add(NS * sizeof(int));      /* perm slot_mol neck frz ljrow ljcol */
+    for (int k = 0; k < 7; k++) add(NS * sizeof(double));   /* q rb ri0 sj ga gbb gg */
+    for (int k = 0; k < 3; k++) add(NS * sizeof(float));    /* ri0_f sj_f rb_f */
+    for (int k = 0; k < 11; k++) add(NS * sizeof(double));  /* xs ys zs sumi reff psi dfac sumdeijda gx gy gz */
+    add(NT * sizeof(int)); add(NT * sizeof(int)); add(NT * sizeof(long long));   /* tile_mol tile_end rowbase */
+    add((natom + 1) * sizeof(int)); add(nexcl * sizeof(int)); add(natom * sizeof(int)); /* ex_ptr ex_j inv */
+    add(ntypes * ntypes * sizeof(double2)); add(sizeof(int2)); add(sizeof(double));    /* ljab mol_lj rgbpsmax2 */
+    add(NECK_N * NECK_N * sizeof(double)); add(NECK_N * NECK_N * sizeof(double));
+    add(NECK_N * NECK_N * sizeof(float)); add(NECK_N * NECK_N * sizeof(float));
+    add(3 * sizeof(double)); add(3 * ENE_SLOTS * sizeof(double));                     /* ene eslot */
+    add(cap * sizeof(int)); add(NT * sizeof(int)); add(NT * sizeof(int)); add(sizeof(int)); /* cand rowcnt rowstart ntp */
+    add(cap * sizeof(int2)); add(cap * sizeof(int2)); add(cap * sizeof(int));          /* tp tpo tflip */
+    add(cap * sizeof(unsigned long long)); add(cap * sizeof(int)); add(2 * TILE * sizeof(int)); /* tcol tord tbin */
+    add(sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(QMeta)); /* nnz tnp tnq qm */
+    add(sizeof(long long)); add(sizeof(int));                                          /* store_need store_ok */
+    add(cap * TILE * sizeof(unsigned long long)); add(cap * TILE * sizeof(unsigned long long)); /* mask excl */
+    add(sizeof(unsigned long long)); add(3 * NT * sizeof(double)); add(3 * NT * sizeof(double)); /* npairs bblo bbhi */
+    add(3 * natom * sizeof(double)); add(3 * natom * sizeof(double)); add(natom * sizeof(double)); /* x g tmp */
+    add(14 * sizeof(double));
+    return b;
+}
+
+extern "C" int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const double *x, double *grad,
+                                double *epol, double *evdw, double *eelec)
+{
+    if (!top || !opt || !x) return seterr(GBHIP_EINVAL, "NULL argument");
+    if (top->natom < 1) return seterr(GBHIP_EINVAL, "natom must be positive");
+    if (!top->iblo) return seterr(GBHIP_EINVAL, "topology without exclusion counts");
+    long long nexcl = 0;
+    for (int i = 0; i < top->natom; i++) nexcl += top->iblo[i];
+    const size_t need = static_build_bytes((size_t)top->natom, (size_t)std::max(0, top->ntypes), (size_t)nexcl);
+    if (need > GBHIP_STATIC_BYTES)
+        return seterr(GBHIP_EINVAL,
+                      "gbhip_egb_static: %d atoms need %zu bytes of device memory, more than the %llu-byte "
+                      "static buffer; evaluate this system through a context (gbhip_ctx_create)",
+                      top->natom, need, (unsigned long long)GBHIP_STATIC_BYTES);
+    if (gbhip_device_count() < 1) return seterr(GBHIP_ENODEV, "no GPU is visible to the HIP runtime");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) return seterr(GBHIP_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+    if (!g_static_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return seterr(GBHIP_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_static_ptr[dev] = ptr;
+    }
+    SlotMem sm;
+    sm.dbase = (char *)g_static_ptr[dev];
+    sm.dsize = GBHIP_STATIC_BYTES;
+    sm.in_use = true;
+    std::unique_ptr<gbhip_ctx> c(new (std::nothrow) gbhip_ctx);
+    if (!c) return seterr(GBHIP_ENOMEM, "host allocation failed");
+    Engine &E = c->E;
+    E.slot = &sm;
+    E.stream = nullptr;
+    const double *xs[1] = {x};
+    int rc;
+    if ((rc = engine_build(E, 1, top, opt, xs)) || (rc = engine_nblist(E)) || (rc = engine_store_reserve(E)))
+        return rc;
+    const size_t n3 = 3 * (size_t)E.natoms;
+    if (grad) {
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, E.stream, E.g_dev, (long long)n3);
+        LAUNCHCHK();
+    }
+    if ((rc = engine_eval(E, E.x_dev, grad ? E.g_dev : nullptr))) return rc;
+    double ene[3];
+    std::vector<double> g(grad ? n3 : 0);
+    HIPCHK(hipMemcpyAsync(ene, E.d.ene, sizeof(ene), hipMemcpyDeviceToHost, E.stream));
+    if (grad) HIPCHK(hipMemcpyAsync(g.data(), E.g_dev, n3 * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (grad)
+        for (size_t k = 0; k < n3; k++) grad[k] += g[k];
+    if (epol) *epol = ene[0];
+    if (evdw) *evdw = ene[1];
+    if (eelec) *eelec = ene[2];
     return 0;
 }
 
