@@ -68,8 +68,10 @@ def sha256(path):
     return h.hexdigest()
 
 
-def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None):
+def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None, det=False):
     ctx = na.GBContext(arr, x, gb=GB, cut=CUT, rgbmax=RGBMAX, kappa=KAPPA, precision=precision)
+    if det:
+        ctx.set_deterministic(1)
     ctx.nblist()
     npairs = ctx.npairs()
     ntiles, ntp = ctx.tile_stats()
@@ -113,7 +115,13 @@ def run_gpu(na, arr, x, precision, steps, warmup, tune=True, fixed_launch=None):
 
 def cpu_baseline(parm, natom, x, budget_s=15.0):
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    # every CPU this process may run on, unless the environment allots fewer
+    # OpenMP threads (a share of a shared host)
+    visible = len(os.sched_getaffinity(0))
+    threads = visible
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and 0 < int(omp) < visible:
+        threads = int(omp)
     oracle.set_threads(threads)
     o = oracle.Oracle(parm, natom, gb=GB, cut=CUT, rgbmax=RGBMAX, kappa=KAPPA)
     t0 = time.perf_counter()
@@ -137,9 +145,12 @@ def cpu_baseline(parm, natom, x, budget_s=15.0):
     except OSError:
         pass
     return dict(value=1.0 / per_eval, unit="evals/s", cores=threads, kind="port",
+                host_cpus={"affinity": visible, "os_cpu_count": os.cpu_count(),
+                           "omp_num_threads": omp or None, "model": cpu},
                 sample="%d oracle egb() evaluations + 1 pair-list build (amortised over nsnb=%d) "
-                       "of the same 25,000-atom igb=8 system; OpenMP, %d threads, %s" %
-                       (n, NSNB, threads, cpu or "cpu model unknown")), ref
+                       "of the same 25,000-atom igb=8 system; OpenMP, %d threads (%d CPUs in this "
+                       "process's affinity, OMP_NUM_THREADS=%s), %s" %
+                       (n, NSNB, threads, visible, omp or "unset", cpu or "cpu model unknown")), ref
 
 
 def bench_batch(args):
@@ -509,6 +520,8 @@ def main():
                     help="MD steps (default per config: c1 1000, c2 10000, c3 5000)")
     ap.add_argument("--md-min", type=int, default=500, help="CG iterations before dynamics")
     ap.add_argument("--cg-iters", type=int, default=100, help="conjgrad iterations of --config cg")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="--config c3 with fixed-order reductions (gbhip_set_deterministic)")
     ap.add_argument("--md-no-graph", action="store_true",
                     help="launch each step's kernels on the stream instead of replaying the captured graphs")
     args = ap.parse_args()
@@ -552,8 +565,8 @@ def main():
     fixed = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     if fixed is not None and len(fixed) != 3:
         raise SystemExit("--launch takes three workgroup counts")
-    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune, fixed)
-    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune, fixed)
+    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune, fixed, args.deterministic)
+    mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune, fixed, args.deterministic)
 
     # accuracy of the mixed kernels against the FP64 kernels (same list)
     acc = {k: abs(mixed["energies"][k] - fp64["energies"][k]) / abs(fp64["energies"][k])
@@ -617,7 +630,8 @@ def main():
                    "tile_pairs": mixed["tilepairs"], "sha256": digests,
                    "launch_blocks": {"mixed": mixed["launch"], "fp64": fp64["launch"]},
                    "step_split": {"mixed": list(mixed["split"]), "fp64": list(fp64["split"])},
-                   "derivative_store": {"mixed": mixed["store"], "fp64": fp64["store"]}},
+                   "derivative_store": {"mixed": mixed["store"], "fp64": fp64["store"]},
+                   "deterministic": bool(args.deterministic)},
         "fp64": {"value": round(fp64["evals_per_s"], 2), "ms_per_step": round(fp64["ms_per_eval"], 4),
                  "pass_ms": {k: round(v, 4) for k, v in fp64["pass_ms"].items()},
                  "dominant": kernel_name(dom64, "double", fp64["split"]), "achieved_tflops": round(a64, 3),

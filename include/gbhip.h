@@ -268,6 +268,23 @@ int gbhip_get_split(gbhip_ctx *ctx, int *split3);
  */
 int gbhip_set_derivative_store(gbhip_ctx *ctx, int mode);
 int gbhip_store_stats(gbhip_ctx *ctx, long long *need, long long *cap, int *in_use);
+
+/*
+ * Deterministic reductions (on = 1).  By default the per-atom sums of the
+ * three passes (Born sums, sumdeijda, gradient) and the energies are
+ * accumulated with FP64 atomics, whose order varies from run to run (last
+ * bits of the results).  With the mode on, every tile-pair visit stores its
+ * sums per tile pair and small kernels add them up per atom and per
+ * molecule in a fixed order, so the same input and list give bit-identical
+ * results on every run, whatever the launch shapes, like the reference's
+ * fixed-order reduction of its per-thread copies (eff.c:2768-2797,
+ * eff.c:2077-2098).  Costs 4 KiB of device memory per tile pair of the list
+ * capacity and the step split (each tile pair is one work item).  Not for a
+ * context in a slot; not while dynamics graphs exist (GBHIP_ESTATE).
+ * Covers the GB evaluation (gbhip_egb, gbhip_egb_device, batches); the
+ * bonded terms of gbhip_mme still add with atomics.
+ */
+int gbhip_set_deterministic(gbhip_ctx *ctx, int on);
 int gbhip_get_launch(gbhip_ctx *ctx, int *blocks3);
 int gbhip_autotune(gbhip_ctx *ctx, int reps, int *chosen3);
 
@@ -447,6 +464,9 @@ int gbhip_batch_autotune(gbhip_batch *b, int reps, int *chosen3);
 
 /* Derivative-store mode and statistics of a batch (as for a context). */
 int gbhip_batch_set_derivative_store(gbhip_batch *b, int mode);
+
+/* Deterministic reductions of a batch (as gbhip_set_deterministic). */
+int gbhip_batch_set_deterministic(gbhip_batch *b, int on);
 int gbhip_batch_store_stats(gbhip_batch *b, long long *need, long long *cap, int *in_use);
 
 /* Total unordered pairs and atoms of the batch. */

@@ -157,6 +157,8 @@ _sig("gbhip_set_launch", C.c_int, [vp, ip])
 _sig("gbhip_valu_peak", C.c_int, [C.c_int, dp])
 _sig("gbhip_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
+_sig("gbhip_set_deterministic", C.c_int, [vp, C.c_int])
+_sig("gbhip_batch_set_deterministic", C.c_int, [vp, C.c_int])
 _sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_batch_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_get_launch", C.c_int, [vp, ip])
@@ -539,6 +541,10 @@ class GBContext:
         """1: pass 1 stores the pass-3 derivatives when they fit; 0: never."""
         _check(lib.gbhip_set_derivative_store(self.h, mode))
 
+    def set_deterministic(self, on):
+        """Fixed-order reductions (bit-identical results run to run)."""
+        _check(lib.gbhip_set_deterministic(self.h, int(on)))
+
     def set_split(self, born=0, pairs=0, chain=0):
         """Step chunks per tile pair for the three pair passes (1, 2, 4, 8, 16;
         0 = the default for the system's size); resets the launch shapes to
@@ -647,6 +653,9 @@ class Batch:
 
     def set_derivative_store(self, mode):
         _check(lib.gbhip_batch_set_derivative_store(self.h, mode))
+
+    def set_deterministic(self, on):
+        _check(lib.gbhip_batch_set_deterministic(self.h, int(on)))
 
     def store_stats(self):
         need, cap, ok = C.c_longlong(), C.c_longlong(), C.c_int()

@@ -100,14 +100,30 @@ struct DevPtrs {
     const int2 *mol_lj;   /* per molecule: offset of its ntypes x ntypes block in ljab, ntypes */
     const double *neck_pos, *neck_val;
     const float *neck_pos_f, *neck_val_f;
+    /* deterministic mode (GBPar::det): per tile pair t and side (0: tile I
+     * of (I, J), 1: tile J), DET_NC components of 64 per-atom visit sums;
+     * per tile pair its three pass-2 energies; per tile its self energy;
+     * per molecule its first tile (nmol + 1 entries); per tile its row's
+     * pass-2 energies (3) and the tile pairs (I, T), I < T, of its column
+     * by increasing I (count per tile; the list of column T of a molecule
+     * whose first tile is I0 starts at its tile-pair base + (T - I0)(T - I0
+     * + 1) / 2, a triangle as large as its tile-pair capacity) */
+    double *part, *pe, *pself, *prow;
+    const int *mol_tile0;
+    int *dcol, *dcolcnt;
 };
+
+#define DET_NC 4
 
 struct GBPar {
     int gb, has_neck, nslots, nmol;
     /* per pass (born, pairs, chain): each tile pair's steps are dealt out
      * in 1 << split_sh chunks, one work item each (more waves than tile
      * pairs for small systems) */
-    int split_sh[3], pad_;
+    int split_sh[3];
+    /* 1: deterministic reductions (every visit's sums stored per tile pair
+     * and added up in a fixed order instead of by atomics; no step split) */
+    int det;
     double cut2, rgbmax, rgbmax1i, rgbmax2i;
     double neckscale, kappa, epsext;
 };
@@ -161,6 +177,8 @@ struct Engine {
     int tev_cap = 0, tev_used = 0;
     /* pass-3 derivative store (separate from allocs: it is resized) */
     void *store_mem = nullptr;
+    /* the deterministic mode's arrays (allocated when the mode is first set) */
+    void *det_mem = nullptr;
     int store_mode = 1;        /* 1: used when it fits, 0: always recompute */
     bool store_frozen = false; /* a captured graph holds the current pointer */
     /* the host knows that the store holds the current list (sized for every
@@ -202,6 +220,7 @@ struct Engine {
         if (stage_ev) (void)hipEventDestroy(stage_ev);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         if (store_mem) (void)hipFree(store_mem);
+        if (det_mem) (void)hipFree(det_mem);
         if (own_stream && stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);

@@ -492,6 +492,18 @@ __device__ __forceinline__ int lanes_below(unsigned long long bal)
 }
 
 /*
+ * Deterministic mode: a visit of tile pair t leaves the sums of its lane
+ * tile A and of its column tile B in two rows of DET_NC * 64 doubles (side
+ * 0 belongs to tile I of t = (I, J), side 1 to tile J; tflip says which of
+ * them is A), and k_det_sum adds the rows up per atom in a fixed order.
+ * Lane l's element of component c of a side: [c * TILE].
+ */
+__device__ __forceinline__ double *det_side(const DevPtrs &d, int t, int side)
+{
+    return d.part + ((size_t)t * 2 + side) * (DET_NC * TILE) + threadIdx.x;
+}
+
+/*
  * Pass 1 (eff.c:1566-1733): for every listed pair, the descreening of i by
  * j (added to i's sum) and of j by i (added to j's sum).  The r^2 test
  * against (rgbmax+Fsmax)^2 and the FP64 sums are kept in both variants.
@@ -600,8 +612,14 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             k = k + 1 == m ? 0 : k + 1;
         }
         __syncthreads();
-        if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
-        if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+        if (p.det) {
+            const int fl = d.tflip[t];
+            *det_side(d, t, fl) = acc;
+            *det_side(d, t, 1 - fl) = own ? s_acc[pos] : 0.0;
+        } else {
+            if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
+            if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+        }
     }
 }
 
@@ -788,8 +806,14 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         born_queue_run(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
         __syncthreads();
         acc += s_row[l];
-        if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
-        if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+        if (p.det) {
+            const int fl = d.tflip[t];
+            *det_side(d, t, fl) = acc;
+            *det_side(d, t, 1 - fl) = own ? s_acc[pos] : 0.0;
+        } else {
+            if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
+            if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+        }
     }
 }
 
@@ -839,7 +863,9 @@ __global__ __launch_bounds__(64) void k_radii(DevPtrs d, GBPar p)
     }
     /* a wave is one tile, so one molecule */
     for (int off = 32; off > 0; off >>= 1) eself += __shfl_xor(eself, off);
-    if (threadIdx.x == 0 && eself != 0.0)
+    if (threadIdx.x == 0 && p.det)
+        d.pself[blockIdx.x] = eself;
+    else if (threadIdx.x == 0 && eself != 0.0)
         unsafeAtomicAdd(&d.eslot[3 * (size_t)d.slot_mol[s] * ENE_SLOTS + (blockIdx.x & (ENE_SLOTS - 1))], eself);
 }
 
@@ -948,7 +974,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const int mol = d.slot_mol[i];
         if (mol != cur_mol) {
-            if (cur_mol >= 0) {
+            if (cur_mol >= 0 && !p.det) {
                 for (int off = 32; off > 0; off >>= 1) {
                     epol += __shfl_xor(epol, off);
                     eel += __shfl_xor(eel, off);
@@ -1090,20 +1116,42 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         if (tab_lds) sweep(std::true_type{});
         else sweep(std::false_type{});
         __syncthreads();
-        if (mr) {
-            unsafeAtomicAdd(&d.gx[i], fxi);
-            unsafeAtomicAdd(&d.gy[i], fyi);
-            unsafeAtomicAdd(&d.gz[i], fzi);
-            unsafeAtomicAdd(&d.sumdeijda[i], sdi);
-        }
-        if (own) {
-            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
-            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
-            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
-            unsafeAtomicAdd(&d.sumdeijda[j], s_sd[pos]);
+        if (p.det) {
+            /* the visit's energies per tile pair, its sums per side */
+            for (int off = 32; off > 0; off >>= 1) {
+                epol += __shfl_xor(epol, off);
+                eel += __shfl_xor(eel, off);
+                evdw += __shfl_xor(evdw, off);
+            }
+            if (l == 0) {
+                d.pe[3 * (size_t)t] = epol;
+                d.pe[3 * (size_t)t + 1] = evdw;
+                d.pe[3 * (size_t)t + 2] = eel;
+            }
+            epol = eel = evdw = 0.0;
+            const int fl = d.tflip[t];
+            double *pa = det_side(d, t, fl), *pb = det_side(d, t, 1 - fl);
+            pa[0] = sdi; pa[TILE] = fxi; pa[2 * TILE] = fyi; pa[3 * TILE] = fzi;
+            pb[0] = own ? s_sd[pos] : 0.0;
+            pb[TILE] = own ? s_fx[pos] : 0.0;
+            pb[2 * TILE] = own ? s_fy[pos] : 0.0;
+            pb[3 * TILE] = own ? s_fz[pos] : 0.0;
+        } else {
+            if (mr) {
+                unsafeAtomicAdd(&d.gx[i], fxi);
+                unsafeAtomicAdd(&d.gy[i], fyi);
+                unsafeAtomicAdd(&d.gz[i], fzi);
+                unsafeAtomicAdd(&d.sumdeijda[i], sdi);
+            }
+            if (own) {
+                unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+                unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+                unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+                unsafeAtomicAdd(&d.sumdeijda[j], s_sd[pos]);
+            }
         }
     }
-    if (cur_mol >= 0) {
+    if (cur_mol >= 0 && !p.det) {
         for (int off = 32; off > 0; off >>= 1) {
             epol += __shfl_xor(epol, off);
             eel += __shfl_xor(eel, off);
@@ -1213,15 +1261,24 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
             }
         }
         __syncthreads();
-        if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
-            unsafeAtomicAdd(&d.gx[i], fxi);
-            unsafeAtomicAdd(&d.gy[i], fyi);
-            unsafeAtomicAdd(&d.gz[i], fzi);
-        }
-        if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
-            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
-            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
-            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+        if (p.det) {
+            const int fl = d.tflip[t];
+            double *pa = det_side(d, t, fl), *pb = det_side(d, t, 1 - fl);
+            pa[0] = fxi; pa[TILE] = fyi; pa[2 * TILE] = fzi;
+            pb[0] = own ? s_fx[pos] : 0.0;
+            pb[TILE] = own ? s_fy[pos] : 0.0;
+            pb[2 * TILE] = own ? s_fz[pos] : 0.0;
+        } else {
+            if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
+                unsafeAtomicAdd(&d.gx[i], fxi);
+                unsafeAtomicAdd(&d.gy[i], fyi);
+                unsafeAtomicAdd(&d.gz[i], fzi);
+            }
+            if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
+                unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+                unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+                unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+            }
         }
     }
 }
@@ -1313,16 +1370,180 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
             k = k + 1 == m ? 0 : k + 1;
         }
         __syncthreads();
-        if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
-            unsafeAtomicAdd(&d.gx[i], fxi);
-            unsafeAtomicAdd(&d.gy[i], fyi);
-            unsafeAtomicAdd(&d.gz[i], fzi);
+        if (p.det) {
+            const int fl = d.tflip[t];
+            double *pa = det_side(d, t, fl), *pb = det_side(d, t, 1 - fl);
+            pa[0] = fxi; pa[TILE] = fyi; pa[2 * TILE] = fzi;
+            pb[0] = own ? s_fx[pos] : 0.0;
+            pb[TILE] = own ? s_fy[pos] : 0.0;
+            pb[2 * TILE] = own ? s_fz[pos] : 0.0;
+        } else {
+            if (fxi != 0.0 || fyi != 0.0 || fzi != 0.0) {
+                unsafeAtomicAdd(&d.gx[i], fxi);
+                unsafeAtomicAdd(&d.gy[i], fyi);
+                unsafeAtomicAdd(&d.gz[i], fzi);
+            }
+            if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
+                unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
+                unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
+                unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+            }
         }
-        if (own && (s_fx[pos] != 0.0 || s_fy[pos] != 0.0 || s_fz[pos] != 0.0)) {
-            unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
-            unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
-            unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+    }
+}
+
+/*
+ * Deterministic mode, after a list build: the column lists, one wavefront
+ * per tile J finding, for every earlier tile I of its molecule, the tile
+ * pair (I, J) in row I by binary search (rows are sorted by J) and writing
+ * the ones present in increasing I.
+ */
+__global__ __launch_bounds__(64) void k_det_cols(DevPtrs d)
+{
+    const int J = blockIdx.x, l = threadIdx.x;
+    const int I0 = d.mol_tile0[d.tile_mol[J]];
+    int *col = d.dcol + d.rowbase[I0] + (size_t)(J - I0) * (J - I0 + 1) / 2;
+    int n = 0;
+    for (int b = I0; b < J; b += TILE) {
+        const int I = b + l;
+        int t = -1;
+        if (I < J) {
+            const int r0 = d.rowstart[I], r1 = r0 + d.rowcnt[I];
+            int lo = r0, hi = r1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (d.tp[mid].y < J) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < r1 && d.tp[lo].y == J) t = lo;
         }
+        const unsigned long long bal = __ballot(t >= 0);
+        if (t >= 0) col[n + lanes_below(bal)] = t;
+        n += __popcll(bal);
+    }
+    if (l == 0) d.dcolcnt[J] = n;
+}
+
+/*
+ * Deterministic mode, after a pass: per slot of tile T, the visit sums its
+ * tile received, added in a fixed order whichever workgroup made them and
+ * when.  The contributions are listed as the tile pairs (T, J) of row T by
+ * increasing J (side 0), then the tile pairs (I, T) by increasing I (side
+ * 1), then the column side of (T, T); four wavefronts take fixed quarters
+ * of the list, each adding its quarter in order, and the quarters are
+ * joined in order.  mode 0: pass 1, into sumi; mode 1: pass 2, added to
+ * sumdeijda (which holds the self term) and gx, gy, gz, and the row's
+ * tile-pair energies summed into prow; mode 2: pass 3, added to gx, gy, gz.
+ * Tile pairs without listed pairs (not visited) are skipped.
+ */
+__global__ __launch_bounds__(256) void k_det_sum(DevPtrs d, int mode)
+{
+    __shared__ double red[4][DET_NC][TILE];
+    __shared__ double ered[3][256];
+    const int T = blockIdx.x, tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int nc = mode == 0 ? 1 : mode == 1 ? 4 : 3;
+    const int r0 = d.rowstart[T], nr = d.rowcnt[T];
+    const int I0 = d.mol_tile0[d.tile_mol[T]];
+    const int *col = d.dcol + d.rowbase[I0] + (size_t)(T - I0) * (T - I0 + 1) / 2;
+    const int ncol = d.dcolcnt[T];
+    const bool diag = nr > 0 && d.tp[r0].y == T;
+    const int n = nr + ncol + (diag ? 1 : 0);
+    const int kb = (int)(((long long)n * w) >> 2), ke = (int)(((long long)n * (w + 1)) >> 2);
+    double v[DET_NC] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k = kb; k < ke; k++) {
+        int t, side;
+        if (k < nr) {
+            t = r0 + k;
+            side = 0;
+        } else if (k < nr + ncol) {
+            t = col[k - nr];
+            side = 1;
+        } else {
+            t = r0;
+            side = 1;
+        }
+        const bool vis = d.tcol[t] != 0ull;
+        const double *pr = d.part + ((size_t)t * 2 + side) * (DET_NC * TILE) + l;
+#pragma unroll
+        for (int c = 0; c < DET_NC; c++)
+            if (c < nc) {
+                const double x = pr[c * TILE];
+                v[c] += vis ? x : 0.0;
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < DET_NC; c++) red[w][c][l] = v[c];
+    if (mode == 1) {
+        /* the row's tile-pair energies: fixed strides, fixed tree */
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        for (int t = r0 + tid; t < r0 + nr; t += 256)
+            if (d.tcol[t]) {
+                e0 += d.pe[3 * (size_t)t];
+                e1 += d.pe[3 * (size_t)t + 1];
+                e2 += d.pe[3 * (size_t)t + 2];
+            }
+        ered[0][tid] = e0; ered[1][tid] = e1; ered[2][tid] = e2;
+    }
+    __syncthreads();
+    if (mode == 1) {
+        for (int h = 128; h > 0; h >>= 1) {
+            if (tid < h)
+                for (int c = 0; c < 3; c++) ered[c][tid] += ered[c][tid + h];
+            __syncthreads();
+        }
+        if (tid < 3) d.prow[3 * (size_t)T + tid] = ered[tid][0];
+    }
+    if (w) return;
+    const int s = T * TILE + l;
+    double u[DET_NC];
+#pragma unroll
+    for (int c = 0; c < DET_NC; c++) u[c] = ((red[0][c][l] + red[1][c][l]) + red[2][c][l]) + red[3][c][l];
+    if (mode == 0) {
+        d.sumi[s] = u[0];
+    } else if (mode == 1) {
+        d.sumdeijda[s] += u[0];
+        d.gx[s] += u[1];
+        d.gy[s] += u[2];
+        d.gz[s] += u[3];
+    } else {
+        d.gx[s] += u[0];
+        d.gy[s] += u[1];
+        d.gz[s] += u[2];
+    }
+}
+
+/*
+ * Deterministic mode, after pass 2: per molecule (one workgroup each), the
+ * row energies and self energies of its tiles, each thread summing a fixed
+ * stride and a fixed tree joining the threads; the totals go to the first
+ * energy slot of each entry (the others hold zero), where k_post's slot
+ * sums pick them up unchanged.
+ */
+__global__ __launch_bounds__(256) void k_det_energy(DevPtrs d)
+{
+    __shared__ double sh[4][256];
+    const int m = blockIdx.x, tid = threadIdx.x;
+    const int T0 = d.mol_tile0[m], T1 = d.mol_tile0[m + 1];
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, es = 0.0;
+    for (int T = T0 + tid; T < T1; T += 256) {
+        e0 += d.prow[3 * (size_t)T];
+        e1 += d.prow[3 * (size_t)T + 1];
+        e2 += d.prow[3 * (size_t)T + 2];
+        es += d.pself[T];
+    }
+    sh[0][tid] = e0; sh[1][tid] = e1; sh[2][tid] = e2; sh[3][tid] = es;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int c = 0; c < 4; c++) sh[c][tid] += sh[c][tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double *e = d.eslot + 3 * (size_t)m * ENE_SLOTS;
+        e[0] = sh[3][0] + sh[0][0];
+        e[ENE_SLOTS] = sh[1][0];
+        e[2 * ENE_SLOTS] = sh[2][0];
     }
 }
 
@@ -1477,6 +1698,7 @@ static const int split_items_per_cu[3] = {64, 6, 6};
 static int default_split(const Engine &E, int pass)
 {
     int sh = 0;
+    if (E.par.det) return 0;
     const long long cap = std::max<long long>(1, E.tp_cap);
     const int most = pass == 0 && cap >= 128 ? 4 : SPLIT_DEFAULT_MAX;
     while ((1 << sh) < most && (cap << sh) < (long long)split_items_per_cu[pass] * E.ncu) sh++;
@@ -1708,6 +1930,7 @@ int engine_nblist(Engine &E)
     if (E.natoms > 0)
         hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_steporder, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d);
+    if (E.par.det) hipLaunchKernelGGL(k_det_cols, dim3(NT), dim3(64), 0, E.stream, E.d);
     LAUNCHCHK();
     E.have_list = true;
     /* a store sized for every tile pair full holds any list; otherwise the
@@ -1850,7 +2073,9 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
         E.tev_used += 5;
     }
     if (ev) HIPCHK(hipEventRecord(ev[0], E.stream));
-    const bool sp1 = E.par.split_sh[0] > 0, sp2 = E.par.split_sh[1] > 0, sp3 = E.par.split_sh[2] > 0;
+    const bool det = E.par.det != 0;
+    const bool sp1 = !det && E.par.split_sh[0] > 0, sp2 = !det && E.par.split_sh[1] > 0,
+               sp3 = !det && E.par.split_sh[2] > 0;
     if (E.precision == GBHIP_FP64 && E.born_queue)
         hipLaunchKernelGGL((sp1 ? k_born_q<true> : k_born_q<false>), dim3(E.pass_blocks[0]), dim3(64), 0, E.stream,
                            E.d, E.par);
@@ -1860,6 +2085,7 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
     else
         hipLaunchKernelGGL((sp1 ? k_born<float, true> : k_born<float, false>), dim3(E.pass_blocks[0]), dim3(64), 0,
                            E.stream, E.d, E.par);
+    if (det) hipLaunchKernelGGL(k_det_sum, dim3(E.ntiles), dim3(256), 0, E.stream, E.d, 0);
     if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
     hipLaunchKernelGGL(k_radii, dim3(nb64), dim3(64), 0, E.stream, E.d, E.par);
     if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
@@ -1869,6 +2095,10 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
     else
         hipLaunchKernelGGL((sp2 ? k_pairs<float, true> : k_pairs<float, false>), dim3(E.pass_blocks[1]), dim3(64), 0,
                            E.stream, E.d, E.par);
+    if (det) {
+        hipLaunchKernelGGL(k_det_sum, dim3(E.ntiles), dim3(256), 0, E.stream, E.d, 1);
+        hipLaunchKernelGGL(k_det_energy, dim3(E.nmol), dim3(256), 0, E.stream, E.d);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
     /* k_chain_st and k_chain each read the device flag and one of them
      * works; when the host knows the store holds the list, k_chain (which
@@ -1887,6 +2117,7 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
             hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]),
                                dim3(64), 0, E.stream, E.d, E.par);
     }
+    if (det) hipLaunchKernelGGL(k_det_sum, dim3(E.ntiles), dim3(256), 0, E.stream, E.d, 2);
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (!(flags & EV_NO_POST)) {
         /* a wavefront per energy entry, a thread per slot */
@@ -1958,6 +2189,8 @@ static int set_split(Engine &E, const int *split3)
         }
         if ((std::max<long long>(1, E.tp_cap) << sh[k]) > (1LL << 30))
             return seterr(GBHIP_EINVAL, "too many tile pairs for a step split of %d", 1 << sh[k]);
+        if (E.par.det && sh[k] != 0)
+            return seterr(GBHIP_EINVAL, "the deterministic mode evaluates each tile pair in one piece (step split 1)");
     }
     for (int k = 0; k < 3; k++) E.par.split_sh[k] = sh[k];
     default_grids(E);
@@ -2037,7 +2270,7 @@ static int autotune(Engine &E, int reps, int *chosen3)
     }
     /* 2. step split per pass, for systems with few tile pairs (each at one
      * workgroup per work item, the grid that suits short work lists) */
-    if (E.tp_cap < (long long)split_items_per_cu[0] * E.ncu) {
+    if (!E.par.det && E.tp_cap < (long long)split_items_per_cu[0] * E.ncu) {
         double tbest[3] = {1e300, 1e300, 1e300};
         int spbest[3] = {1 << E.par.split_sh[0], 1 << E.par.split_sh[1], 1 << E.par.split_sh[2]};
         for (int sh = 0; (1 << sh) <= SPLIT_MAX; sh++) {
@@ -2074,6 +2307,78 @@ static int autotune(Engine &E, int reps, int *chosen3)
     if (chosen3)
         for (int k = 0; k < 3; k++) chosen3[k] = pick[k];
     return engine_eval(E, E.x_dev, nullptr);
+}
+
+/*
+ * Deterministic reductions on or off.  The arrays (about 4 KiB per tile
+ * pair of the list capacity, 32 bytes per tile) are allocated
+ * the first time the mode is set and kept; the step split returns to 1 per
+ * pass while the mode is on, and to the default for the system when it is
+ * turned off.
+ */
+static int set_det(Engine &E, int on)
+{
+    if (on != 0 && on != 1) return seterr(GBHIP_EINVAL, "deterministic mode must be 0 or 1");
+    if (on == E.par.det) return 0;
+    if (E.store_frozen) return seterr(GBHIP_ESTATE, "deterministic mode cannot change while dynamics graphs exist");
+    if (on && !E.det_mem) {
+        if (E.slot) return seterr(GBHIP_ESTATE, "deterministic mode is not available to a context in a slot");
+        const size_t cap = (size_t)std::max<long long>(1, E.tp_cap);
+        const size_t b_part = cap * 2 * DET_NC * TILE * sizeof(double), b_pe = cap * 3 * sizeof(double);
+        const size_t b_ps = (size_t)E.ntiles * sizeof(double), b_mt = (size_t)(E.nmol + 1) * sizeof(int);
+        const size_t b_pr = (size_t)E.ntiles * 3 * sizeof(double), b_dc = cap * sizeof(int),
+                     b_dn = (size_t)E.ntiles * sizeof(int);
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t total = al(b_part) + al(b_pe) + al(b_ps) + al(b_mt) + al(b_pr) + al(b_dc) + al(b_dn);
+        std::vector<int> mt(E.nmol + 1);
+        for (int m = 0; m <= E.nmol; m++) mt[m] = E.mol_slot_off[m] / TILE;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        HIPCHK(hipStreamSynchronize(E.stream));
+        void *p = nullptr;
+        if (hipMalloc(&p, total) != hipSuccess) {
+            (void)hipGetLastError();
+            return seterr(GBHIP_ENOMEM, "hipMalloc of %zu bytes for the deterministic mode failed", total);
+        }
+        char *c = (char *)p;
+        E.det_mem = p;
+        E.d.part = (double *)c;
+        c += al(b_part);
+        E.d.pe = (double *)c;
+        c += al(b_pe);
+        E.d.pself = (double *)c;
+        c += al(b_ps);
+        int *mtd = (int *)c;
+        c += al(b_mt);
+        E.d.prow = (double *)c;
+        c += al(b_pr);
+        E.d.dcol = (int *)c;
+        c += al(b_dc);
+        E.d.dcolcnt = (int *)c;
+        HIPCHK(hipMemcpyAsync(mtd, mt.data(), b_mt, hipMemcpyHostToDevice, E.stream));
+        HIPCHK(hipStreamSynchronize(E.stream));
+        E.d.mol_tile0 = mtd;
+    }
+    E.par.det = on;
+    for (int k = 0; k < 3; k++) E.par.split_sh[k] = default_split(E, k);
+    default_grids(E);
+    /* the column lists of the current list */
+    if (on && E.have_list) {
+        hipLaunchKernelGGL(k_det_cols, dim3(E.ntiles), dim3(64), 0, E.stream, E.d);
+        LAUNCHCHK();
+    }
+    return 0;
+}
+
+extern "C" int gbhip_set_deterministic(gbhip_ctx *ctx, int on)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    return set_det(ctx->E, on);
+}
+
+extern "C" int gbhip_batch_set_deterministic(gbhip_batch *b, int on)
+{
+    if (!b) return seterr(GBHIP_EINVAL, "NULL batch");
+    return set_det(b->E, on);
 }
 
 extern "C" int gbhip_set_derivative_store(gbhip_ctx *ctx, int mode)
