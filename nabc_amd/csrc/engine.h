@@ -161,6 +161,9 @@ struct Engine {
     DevPtrs d{};
     hipStream_t stream = nullptr;
     bool own_stream = false;   /* created by the engine, destroyed with it */
+    /* streams the engine created and then left for a caller's stream: kept
+     * until the engine goes, since graphs captured on them still exist */
+    std::vector<hipStream_t> retired_streams;
     std::vector<int> mol_atom_off, mol_natom, mol_slot_off;
     std::vector<int> perm_h;  /* slot -> global atom */
     std::vector<int> frz_h;   /* slot -> frozen flag */
@@ -224,6 +227,10 @@ struct Engine {
         if (own_stream && stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
+        }
+        for (hipStream_t s : retired_streams) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
         }
         for (void *p : allocs) (void)hipFree(p);
     }

@@ -2791,9 +2791,11 @@ extern "C" int gbhip_set_stream(gbhip_ctx *ctx, void *stream)
      * one can start (the context's calls assume one ordered stream) */
     if ((hipStream_t)stream != ctx->E.stream) {
         HIPCHK(hipStreamSynchronize(ctx->E.stream));
-        /* a stream the engine created for dynamics goes now; the caller's
-         * stream is never destroyed by the engine */
-        if (ctx->E.own_stream && ctx->E.stream) (void)hipStreamDestroy(ctx->E.stream);
+        /* a stream the engine created (for dynamics or the mme graph) is
+         * kept until the context goes: the graphs captured on it live on
+         * and are replayed on the new stream; the caller's stream is never
+         * destroyed by the engine */
+        if (ctx->E.own_stream && ctx->E.stream) ctx->E.retired_streams.push_back(ctx->E.stream);
         ctx->E.own_stream = false;
     }
     ctx->E.stream = (hipStream_t)stream;

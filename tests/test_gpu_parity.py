@@ -409,3 +409,48 @@ def test_gbrna_old_format_gb1(gpu, precision):
     arr = na.parm_arrays(parm)
     x, _ = na.read_rst7(GOLDEN + "/gbrna.rst7", arr["natom"])
     _compare(na, parm, arr, x, gb=1, cut=99.0, rgbmax=99.0, kappa=0.10395, precision=precision)
+
+
+@pytest.mark.parametrize("diel,dielc", [("R", 1.0), ("C", 2.0), ("R", 2.0)])
+def test_dielectric_options_match_oracle(gpu, gen_dir, diel, dielc):
+    """diel=R (1-4 electrostatics q_i q_j / (scee r^2)) and dielc != 1
+    (charges scaled by 1/sqrt(dielc) at mme_init_sff): the device mme()
+    per term and gradient against the oracle's mme34 to 1e-9, and for
+    diel=R the device's 1-4 electrostatics scale by 1/4 when every distance
+    doubles."""
+    na = gpu
+    parm, arr, x = generated_system(600, 12, gen_dir)
+    n = arr["natom"]
+    # the oracle copies the charges before mme_init_sff scales them in place
+    orc = oracle.Oracle(parm, n, gb=5, cut=16.0, rgbmax=16.0, kappa=0.1, dielc=dielc, dield=0 if diel == "R" else 1)
+    r_ene, r_frms, r_g = orc.mme(x)
+    orc.close()
+    na.lib.mm_reset_options()
+    opts = "gb=5, cut=16.0, rgbmax=16.0, kappa=0.1, diel=%s, dielc=%.1f, ntpr=100000" % (diel, dielc)
+    assert na.lib.mm_options(opts.encode()) == 0
+    assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+
+    def run(xc):
+        xc = np.ascontiguousarray(xc)
+        f = np.zeros(3 * n)
+        it = na.C.c_int(-2)
+        e = na.lib.mme(xc.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), na.C.byref(it))
+        ene = np.zeros(13)
+        fr = na.C.c_double()
+        na.lib.mme_get_energies(ene.ctypes.data_as(na.dp), na.C.byref(fr))
+        return e, ene, fr.value, f
+
+    try:
+        e0, ene, frms, f = run(x)
+        assert rel(e0, r_ene[0]) <= 1e-9
+        for k in (1, 2, 3, 4, 5, 7, 8, 10):
+            assert abs(ene[k] - r_ene[k]) <= 1e-9 * max(1.0, abs(r_ene[k])), (k, ene[k], r_ene[k])
+        assert rel_rms(f, r_g) <= 1e-9
+        assert abs(frms - r_frms) <= 1e-9 * r_frms
+        if diel == "R":
+            _, ene2, _, _ = run(2.0 * x)
+            assert abs(ene[8] / ene2[8] - 4.0) <= 1e-11, (ene[8], ene2[8])
+    finally:
+        it = na.C.c_int(-3)
+        xc = np.ascontiguousarray(x)
+        na.lib.mme(xc.ctypes.data_as(na.dp), np.zeros(3 * n).ctypes.data_as(na.dp), na.C.byref(it))

@@ -13,7 +13,13 @@ These pin the oracle where the reference's fixtures leave it open:
   - the gradient against central finite differences of the total energy,
     for gb 1/2/5/7/8 with salt and with the rgbmax switch;
   - translation and rotation invariance, zero net force and torque;
-  - the pair list against a brute-force count.
+  - the pair list against a brute-force count;
+  - the 1-4 electrostatics of diel=R (eff.c:1032-1040: q_i q_j / (scee
+    r^2)) and diel=C (q_i q_j / (scee r)) by how they scale when every
+    distance doubles (1/4 and 1/2; the 1-4 pairs are topological), the
+    diel=R gradient against finite differences, and dielc (sff.c:1153-1162:
+    charges divided by sqrt(dielc)) by halving every charge product (eel,
+    eel14 and the GB energy) at dielc = 2 and leaving the other terms alone.
 A dropped term, a wrong sign or a swapped radius in any of these fails one
 of them.  Inputs are generated systems whose radii and screening factors
 are overwritten in memory to reach each case.
@@ -239,3 +245,65 @@ def test_pair_list_brute_force():
     o.close()
     iu, ju = np.nonzero(np.triu(keep, 1))
     assert np.array_equal(got, np.stack([iu, ju], 1))
+
+
+EEL14, ENB14, EEL, EGB = 8, 7, 2, 10
+
+
+@pytest.mark.parametrize("dield,ratio", [(0, 4.0), (1, 2.0)])
+def test_14_electrostatics_scale_with_dielectric_form(dield, ratio):
+    import nabc_amd as na
+    p, a, x = na.build_system(120, 31)
+    o = oracle.Oracle(p, 120, gb=5, cut=999.0, rgbmax=999.0, dield=dield)
+    e1, _, _ = o.mme(x)
+    e2, _, _ = o.mme(2.0 * x)
+    o.close()
+    assert abs(e1[EEL14]) > 1.0
+    assert abs(e1[EEL14] / e2[EEL14] - ratio) <= 1e-12 * ratio, (e1[EEL14], e2[EEL14])
+
+
+def test_14_terms_depend_on_dielectric_only_in_electrostatics():
+    import nabc_amd as na
+    p, a, x = na.build_system(120, 32)
+    oc = oracle.Oracle(p, 120, gb=5, cut=999.0, rgbmax=999.0, dield=1)
+    orr = oracle.Oracle(p, 120, gb=5, cut=999.0, rgbmax=999.0, dield=0)
+    ec, _, _ = oc.mme(x)
+    er, _, _ = orr.mme(x)
+    oc.close()
+    orr.close()
+    # (the oracle's OpenMP sums may differ in the last bit)
+    for k in (1, 2, 3, 4, 5, ENB14, 10):
+        assert abs(ec[k] - er[k]) <= 1e-13 * abs(ec[k]), k
+    assert ec[EEL14] != er[EEL14]
+
+
+def test_diel_r_gradient_matches_finite_differences():
+    import nabc_amd as na
+    p, a, x = na.build_system(60, 33)
+    o = oracle.Oracle(p, 60, gb=2, cut=999.0, rgbmax=999.0, dield=0)
+    o.nblist(x)
+    e0, g = _energy(o, x)
+    h = 1e-5
+    fd = np.zeros_like(x)
+    for k in range(len(x)):
+        xp = x.copy(); xp[k] += h
+        xm = x.copy(); xm[k] -= h
+        fd[k] = (_energy(o, xp)[0] - _energy(o, xm)[0]) / (2 * h)
+    o.close()
+    err = np.sqrt(np.mean((fd - g) ** 2) / np.mean(g ** 2))
+    assert err <= 1e-6, err
+
+
+def test_dielc_halves_every_charge_product():
+    import nabc_amd as na
+    p, a, x = na.build_system(150, 34)
+    o1 = oracle.Oracle(p, 150, gb=5, cut=999.0, rgbmax=999.0, kappa=0.1, dielc=1.0)
+    o2 = oracle.Oracle(p, 150, gb=5, cut=999.0, rgbmax=999.0, kappa=0.1, dielc=2.0)
+    e1, _, _ = o1.mme(x)
+    e2, _, _ = o2.mme(x)
+    o1.close()
+    o2.close()
+    for k in (EEL, EEL14, EGB):
+        assert abs(e2[k] - 0.5 * e1[k]) <= 1e-12 * abs(e1[k]), (k, e1[k], e2[k])
+    for k in (1, 3, 4, 5, ENB14):
+        assert abs(e2[k] - e1[k]) <= 1e-13 * abs(e1[k]), k
