@@ -49,16 +49,25 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gb_25k_igb8_traffic.json")
 
 def kernel_name(pass_, real, split):
     k = ("born", "pairs", "chain").index(pass_)
+    sp = "true" if split[k] > 1 else "false"
+    if pass_ == "born" and real == "double":
+        # FP64 pass 1 is the queued variant
+        return "k_born_q<%s>" % sp
     base = ("k_born", "k_pairs", "k_chain_st")[k]
-    return "%s<%s, %s>" % (base, real, "true" if split[k] > 1 else "false")
+    return "%s<%s, %s>" % (base, real, sp)
+
+
+def profiled(kernel, key):
+    """A figure of the committed profile summary for this workload."""
+    try:
+        with open(TRAFFIC_FILE) as fp:
+            return json.load(fp)["kernels"].get(kernel, {}).get(key)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def measured_traffic(kernel):
-    try:
-        with open(TRAFFIC_FILE) as fp:
-            return json.load(fp)["kernels"].get(kernel, {}).get("bytes_per_launch")
-    except (OSError, ValueError, KeyError):
-        return None
+    return profiled(kernel, "bytes_per_launch")
 
 
 def sha256(path):
@@ -504,6 +513,8 @@ def main():
     ap.add_argument("--launch", default="", metavar="B1,B2,B3",
                     help="fixed workgroup counts of passes 1-3 (both precisions), no autotuning; "
                          "reproduces a tuned run's shapes under a profiler")
+    ap.add_argument("--launch64", default="", metavar="B1,B2,B3",
+                    help="fixed workgroup counts for the FP64 run only (else --launch applies to both)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg"],
@@ -565,7 +576,10 @@ def main():
     fixed = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     if fixed is not None and len(fixed) != 3:
         raise SystemExit("--launch takes three workgroup counts")
-    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune, fixed, args.deterministic)
+    fixed64 = tuple(int(v) for v in args.launch64.split(",")) if args.launch64 else fixed
+    if fixed64 is not None and len(fixed64) != 3:
+        raise SystemExit("--launch64 takes three workgroup counts")
+    fp64 = run_gpu(na, arr, x, na.GBHIP_FP64, args.steps, args.warmup, not args.no_tune, fixed64, args.deterministic)
     mixed = run_gpu(na, arr, x, na.GBHIP_MIXED, args.steps, args.warmup, not args.no_tune, fixed, args.deterministic)
 
     # accuracy of the mixed kernels against the FP64 kernels (same list)
@@ -581,7 +595,16 @@ def main():
     kname = kernel_name(dom, "float", mixed["split"])
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK["mixed"], unit="TFLOP/s",
                     frac=round(achieved / PEAK["mixed"], 4), traffic=measured_traffic(kname), kernel=kname,
-                    kernel_ms=round(mixed["pass_ms"][dom], 4))
+                    kernel_ms=round(mixed["pass_ms"][dom], 4),
+                    kernel_ms_source="HIP events around the kernel on the engine stream, this run's timed region")
+    # the same figure at the kernel's average duration in the committed
+    # rocprofv3 kernel trace (the profiler serialises launches, so it is longer)
+    prof_us = profiled(kname, "avg_us_kernel_trace")
+    if prof_us:
+        a_prof = FLOPS[dom] * P / (prof_us * 1e-6) / 1e12
+        roofline["profile_kernel_ms"] = round(prof_us / 1e3, 4)
+        roofline["achieved_at_profile_ms"] = round(a_prof, 3)
+        roofline["frac_at_profile_ms"] = round(a_prof / PEAK["mixed"], 4)
     dom64 = max(("born", "pairs", "chain"), key=lambda k: fp64["pass_ms"][k])
     a64 = FLOPS[dom64] * P / (fp64["pass_ms"][dom64] * 1e-3) / 1e12
     # whole-evaluation figure: W = 240 P flops per evaluation at the measured rate
