@@ -595,8 +595,10 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                             }
                         }
                     }
+                    /* the column sums are kept with the opposite sign (one
+                     * LDS add without a negation) and negated at the flush */
                     acc -= (double)tij;
-                    atomicAdd(&s_acc[k], -(double)tji);
+                    atomicAdd(&s_acc[k], (double)tji);
                 }
             }
             if (store) {
@@ -615,10 +617,10 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         if (p.det) {
             const int fl = d.tflip[t];
             *det_side(d, t, fl) = acc;
-            *det_side(d, t, 1 - fl) = own ? s_acc[pos] : 0.0;
+            *det_side(d, t, 1 - fl) = own ? -s_acc[pos] : 0.0;
         } else {
             if (acc != 0.0) unsafeAtomicAdd(&d.sumi[i], acc);
-            if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], s_acc[pos]);
+            if (own && s_acc[pos] != 0.0) unsafeAtomicAdd(&d.sumi[j], -s_acc[pos]);
         }
     }
 }
@@ -875,7 +877,15 @@ struct PairOut {
     double epol, eel, evdw, sdi, sdj, gx, gy, gz;
 };
 
-/* the pass-2 terms of k_pairs below for one pair, branch-free */
+/*
+ * The pass-2 terms of k_pairs below for one pair, branch-free.  Instead of
+ * a select per output, two operands are selected: the GB terms take a zero
+ * charge product on a lane without a listed pair (every GB output is then
+ * an exact zero; the GB arithmetic stays finite even at r = 0, the lane's
+ * own diagonal entry), and the Coulomb and LJ terms take 1/r = 0 unless the
+ * pair is listed and not excluded (their outputs, and their share of the
+ * force, are then exact zeros whatever 1/r would have been).
+ */
 template <typename R>
 __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, double xi, double yi, double zi,
                                            double qi, R ri, double xj, double yj, double zj, double qj, R rj,
@@ -885,7 +895,7 @@ __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, doub
     const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
     const R r2 = (R)r2d;
     const double qiqj_d = qi * qj;
-    const R qiqj = (R)qiqj_d;
+    const R qiqj = act ? (R)qiqj_d : R(0);
     const R rb2 = ri * rj;
     const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
     const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
@@ -903,20 +913,20 @@ __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, doub
     const R de = temp6 * (R(1) - R(0.25) * efac);
     const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
     /* Coulomb and LJ in FP64 (see k_pairs) */
-    const double rinv = rsqrt_pair<R>(r2d);
+    const double rinv = coul ? rsqrt_pair<R>(r2d) : 0.0;
     const double r2inv = rinv * rinv;
     const double el = qiqj_d * rinv;
     const double r6inv = r2inv * r2inv * r2inv;
     const double f6 = ab.y * r6inv;
     const double f12 = ab.x * r6inv * r6inv;
     const double dcl = -(el + 12.0 * f12 - 6.0 * f6) * r2inv;
-    const double ded = act ? (double)de + (coul ? dcl : 0.0) : 0.0;
+    const double ded = (double)de + dcl;
     o.act = act;
-    o.epol = act ? -(double)(qiqj * dielfac * fgbi) : 0.0;
-    o.eel = coul ? el : 0.0;
-    o.evdw = coul ? f12 - f6 : 0.0;
-    o.sdi = act ? (double)(ri * temp5) : 0.0;
-    o.sdj = act ? (double)(rj * temp5) : 0.0;
+    o.epol = -(double)(qiqj * dielfac * fgbi);
+    o.eel = el;
+    o.evdw = f12 - f6;
+    o.sdi = (double)(ri * temp5);
+    o.sdj = (double)(rj * temp5);
     o.gx = ded * dxd;
     o.gy = ded * dyd;
     o.gz = ded * dzd;
@@ -1048,9 +1058,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     fxi += P[u].gx; fyi += P[u].gy; fzi += P[u].gz;
                     if (P[u].act) {
                         atomicAdd(&s_sd[kq], P[u].sdj);
-                        atomicAdd(&s_fx[kq], -P[u].gx);
-                        atomicAdd(&s_fy[kq], -P[u].gy);
-                        atomicAdd(&s_fz[kq], -P[u].gz);
+                        atomicAdd(&s_fx[kq], P[u].gx);
+                        atomicAdd(&s_fy[kq], P[u].gy);
+                        atomicAdd(&s_fz[kq], P[u].gz);
                     }
                 }
                 k = two ? (k1 + 1 == m ? 0 : k1 + 1) : k1;
@@ -1105,9 +1115,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     }
                     const double gxv = ded * dxd, gyv = ded * dyd, gzv = ded * dzd;
                     fxi += gxv; fyi += gyv; fzi += gzv;
-                    atomicAdd(&s_fx[k], -gxv);
-                    atomicAdd(&s_fy[k], -gyv);
-                    atomicAdd(&s_fz[k], -gzv);
+                    atomicAdd(&s_fx[k], gxv);
+                    atomicAdd(&s_fy[k], gyv);
+                    atomicAdd(&s_fz[k], gzv);
                 }
                 k = k + 1 == m ? 0 : k + 1;
             }
@@ -1133,9 +1143,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             double *pa = det_side(d, t, fl), *pb = det_side(d, t, 1 - fl);
             pa[0] = sdi; pa[TILE] = fxi; pa[2 * TILE] = fyi; pa[3 * TILE] = fzi;
             pb[0] = own ? s_sd[pos] : 0.0;
-            pb[TILE] = own ? s_fx[pos] : 0.0;
-            pb[2 * TILE] = own ? s_fy[pos] : 0.0;
-            pb[3 * TILE] = own ? s_fz[pos] : 0.0;
+            pb[TILE] = own ? -s_fx[pos] : 0.0;
+            pb[2 * TILE] = own ? -s_fy[pos] : 0.0;
+            pb[3 * TILE] = own ? -s_fz[pos] : 0.0;
         } else {
             if (mr) {
                 unsafeAtomicAdd(&d.gx[i], fxi);
@@ -1143,10 +1153,12 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                 unsafeAtomicAdd(&d.gz[i], fzi);
                 unsafeAtomicAdd(&d.sumdeijda[i], sdi);
             }
+            /* the column atoms' direct forces were summed with the row's
+             * sign (one LDS add without a negation per pair) */
             if (own) {
-                unsafeAtomicAdd(&d.gx[j], s_fx[pos]);
-                unsafeAtomicAdd(&d.gy[j], s_fy[pos]);
-                unsafeAtomicAdd(&d.gz[j], s_fz[pos]);
+                unsafeAtomicAdd(&d.gx[j], -s_fx[pos]);
+                unsafeAtomicAdd(&d.gy[j], -s_fy[pos]);
+                unsafeAtomicAdd(&d.gz[j], -s_fz[pos]);
                 unsafeAtomicAdd(&d.sumdeijda[j], s_sd[pos]);
             }
         }
