@@ -888,8 +888,8 @@ struct PairOut {
  */
 template <typename R>
 __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, double xi, double yi, double zi,
-                                           double qi, R ri, double xj, double yj, double zj, double qj, R rj,
-                                           double2 ab, R epsinv, R kks, bool screened)
+                                           double qi, R ri, R ci, double xj, double yj, double zj, double qj, R rj,
+                                           R irj, double2 ab, R epsinv, R kks, bool screened)
 {
     const double dxd = xi - xj, dyd = yi - yj, dzd = zi - zj;
     const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
@@ -897,11 +897,15 @@ __device__ __forceinline__ void pair_terms(PairOut &o, bool act, bool coul, doub
     const double qiqj_d = qi * qj;
     const R qiqj = act ? (R)qiqj_d : R(0);
     const R rb2 = ri * rj;
-    const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
-    const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
+    /* exp(-r^2 / 4 Ri Rj) = 2^(r^2 ci / Rj) with ci = -log2(e) / 4 Ri of
+     * the row and 1/Rj of the column staged once per visit; 1/fgbi =
+     * sqrt(D) = D fgbi: no reciprocal per pair */
+    const R efac = __builtin_amdgcn_exp2f(r2 * (ci * irj));
+    const R dd = r2 + rb2 * efac;
+    const R fgbi = gbm<R>::rsqrt_(dd);
     R expmkf, fgbk;
     if (screened) {
-        fgbk = kks * gbm<R>::rcp(fgbi);
+        fgbk = kks * dd * fgbi;
         expmkf = gbm<R>::exp_(fgbk) * epsinv;
     } else {
         fgbk = R(0);
@@ -956,7 +960,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
     /* the molecule's LJ (A, B) table when it has at most LJ_LDS entries
      * (16 types): a per-pair global load leaves the dependency chain */
     __shared__ double2 s_tab[LJ_LDS];
-    __shared__ R s_r[TILE];
+    __shared__ R s_r[TILE], s_ir[TILE];
     __shared__ int s_lj[TILE];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
@@ -1005,6 +1009,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double qi = d.q[i];
         const R ri = (R)d.reff[i];
+        const R ci = R(-1.44269504088896341) / (R(4) * ri);
         const int row = d.ljrow[i];
         const double2 *gtab = d.ljab;
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
@@ -1014,7 +1019,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         __syncthreads();
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
-            s_q[pos] = d.q[j]; s_r[pos] = (R)d.reff[j];
+            s_q[pos] = d.q[j]; s_r[pos] = (R)d.reff[j]; s_ir[pos] = R(1) / (R)d.reff[j];
             s_lj[pos] = d.ljcol[j];
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0; s_sd[pos] = 0.0;
         }
@@ -1048,8 +1053,8 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     const int b = (s + u) & 63;
                     const bool act = (u == 0 || two) && ((mr >> b) & 1ull);
                     const bool coul = act && !((e >> b) & 1ull);
-                    pair_terms<R>(P[u], act, coul, xi, yi, zi, qi, ri, s_x[kq], s_y[kq], s_z[kq], s_q[kq], s_r[kq],
-                                  lj(row + s_lj[kq]), epsinv, kks, screened);
+                    pair_terms<R>(P[u], act, coul, xi, yi, zi, qi, ri, ci, s_x[kq], s_y[kq], s_z[kq], s_q[kq],
+                                  s_r[kq], s_ir[kq], lj(row + s_lj[kq]), epsinv, kks, screened);
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
