@@ -643,22 +643,23 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
  * only the order of the FP64 additions into the atom sums differs.
  */
 #define BQ_CAP 128
-struct BornQueue {
-    double r2[BQ_CAP], dij1i[BQ_CAP];
-    double *out[BQ_CAP];   /* derivative-store component, or null */
+template <typename R> struct BornQueue {
+    R r2[BQ_CAP], dij1i[BQ_CAP];
+    R *out[BQ_CAP];        /* derivative-store component, or null */
     int code[BQ_CAP];      /* row lane | column << 6 | (1 << 12: direction j by i) */
 };
 
 /* per-visit LDS copies of the two tiles' atom data the queue needs */
-struct BornTiles {
-    double ri[TILE], ri1i[TILE], sj[TILE], rb[TILE]; /* column atoms */
-    double r_ri[TILE], r_ri1i[TILE], r_sj[TILE], r_rb[TILE]; /* row atoms */
+template <typename R> struct BornTiles {
+    R ri[TILE], ri1i[TILE], sj[TILE], rb[TILE]; /* column atoms */
+    R r_ri[TILE], r_ri1i[TILE], r_sj[TILE], r_rb[TILE]; /* row atoms */
     int nk[TILE], r_nk[TILE]; /* neck indices of the column and row atoms */
 };
 
 /* the queued directions at positions [h, h + n) of the ring, one per lane */
-__device__ __forceinline__ void born_queue_run(BornQueue &Q, int h, int n, const BornTiles &T, const double *npos,
-                                               const double *nval, double *s_row, double *s_col, const GBPar &p)
+template <typename R>
+__device__ __forceinline__ void born_queue_run(BornQueue<R> &Q, int h, int n, const BornTiles<R> &T, const R *npos,
+                                               const R *nval, double *s_row, double *s_col, const GBPar &p)
 {
     const int l = threadIdx.x;
     if (l >= n) return;
@@ -667,38 +668,39 @@ __device__ __forceinline__ void born_queue_run(BornQueue &Q, int h, int n, const
     const bool ji = code >> 12;
     /* direction i by j: the row atom descreened by the column atom's
      * sphere; direction j by i the other way round */
-    const double ri = ji ? T.ri[col] : T.r_ri[lane], ri1i = ji ? T.ri1i[col] : T.r_ri1i[lane];
-    const double sj = ji ? T.r_sj[lane] : T.sj[col];
-    const double r2 = Q.r2[e], dij1i = Q.dij1i[e], dij = r2 * dij1i;
+    const R ri = ji ? T.ri[col] : T.r_ri[lane], ri1i = ji ? T.ri1i[col] : T.r_ri1i[lane];
+    const R sj = ji ? T.r_sj[lane] : T.sj[col];
+    const R r2 = Q.r2[e], dij1i = Q.dij1i[e], dij = r2 * dij1i;
     bool skip;
-    double dv;
-    double v = descreen_d<double>(ri, ri1i, sj, r2, dij, dij1i, p.rgbmax, p.rgbmax1i, p.rgbmax2i, skip, dv);
-    if (p.has_neck && !skip && dij < T.r_rb[lane] + T.rb[col] + NECK_CUT) {
+    R dv;
+    R v = descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, (R)p.rgbmax, (R)p.rgbmax1i, (R)p.rgbmax2i, skip, dv);
+    if (p.has_neck && !skip && dij < T.r_rb[lane] + T.rb[col] + (R)NECK_CUT) {
         const int nidx = ji ? T.nk[col] * NECK_N + T.r_nk[lane] : T.r_nk[lane] * NECK_N + T.nk[col];
-        double dn;
-        v += neck_d<double>(dij, dij1i, npos[nidx], nval[nidx], p.neckscale, dn);
+        R dn;
+        v += neck_d<R>(dij, dij1i, npos[nidx], nval[nidx], (R)p.neckscale, dn);
         dv += dn;
     }
-    atomicAdd(ji ? &s_col[col] : &s_row[lane], -v);
+    atomicAdd(ji ? &s_col[col] : &s_row[lane], -(double)v);
     if (Q.out[e]) *Q.out[e] = dv;
 }
 
-template <bool SPL>
+template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
 {
+    typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE], s_row[TILE];
-    __shared__ BornTiles T;
-    __shared__ double npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
-    __shared__ BornQueue Q;
+    __shared__ BornTiles<R> T;
+    __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
+    __shared__ BornQueue<R> Q;
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const bool store = *d.store_ok;
-    double2 *st = (double2 *)d.dstore;
-    const double rgbmax = p.rgbmax;
+    R2 *st = (R2 *)d.dstore;
+    const R rgbmax = (R)p.rgbmax;
     if (p.has_neck) {
         for (int k = l; k < NECK_N * NECK_N; k += TILE) {
-            npos[k] = d.neck_pos[k];
-            nval[k] = d.neck_val[k];
+            npos[k] = neckpos<R>(d)[k];
+            nval[k] = neckval<R>(d)[k];
         }
     }
     const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
@@ -713,20 +715,23 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         const int i = mq.a * TILE + l, j = mq.b * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
-        const double ri = d.ri0[i], si = d.sj[i], rbi = d.rb[i];
+        const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
-        double2 *sp = store ? st + mq.start + (sb ? entries_before(mr, sb) : 0) : nullptr;
+        R2 *sp = store ? st + mq.start + (sb ? entries_before(mr, sb) : 0) : nullptr;
         __syncthreads();
-        T.r_ri[l] = ri; T.r_ri1i[l] = gbm<double>::rcp(ri); T.r_sj[l] = si; T.r_rb[l] = rbi;
+        /* 1/Ri: the Newton-refined reciprocal in FP64, the IEEE quotient in
+         * FP32 (what the sweep kernel k_born uses) */
+        T.r_ri[l] = ri; T.r_ri1i[l] = sizeof(R) == sizeof(double) ? gbm<R>::rcp(ri) : R(1) / ri;
+        T.r_sj[l] = si; T.r_rb[l] = rbi;
         T.r_nk[l] = d.neck[i];
         s_row[l] = 0.0;
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
-            const double rj = d.ri0[j];
-            T.ri[pos] = rj; T.ri1i[pos] = gbm<double>::rcp(rj);
-            T.sj[pos] = d.sj[j]; T.rb[pos] = d.rb[j];
+            const R rj = ldri0<R>(d, j);
+            T.ri[pos] = rj; T.ri1i[pos] = sizeof(R) == sizeof(double) ? gbm<R>::rcp(rj) : R(1) / rj;
+            T.sj[pos] = ldsj<R>(d, j); T.rb[pos] = ldrb<R>(d, j);
             T.nk[pos] = d.neck[j];
             s_acc[pos] = 0.0;
         }
@@ -736,43 +741,44 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
             const bool listed = (mr >> s) & 1ull;
-            double aij = 0.0, aji = 0.0, r2 = 0.0, dij1i = 0.0;
+            R aij = R(0), aji = R(0), r2 = R(0), dij1i = R(0);
             bool defer_ij = false, defer_ji = false;
             if (listed) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
-                r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= rp2) {
-                    dij1i = gbm<double>::rsqrt_(r2);
-                    const double dij = r2 * dij1i;
-                    const double sj = T.sj[k];
-                    const DsCase cij = ds_case<double>(ri, sj, dij, rgbmax);
-                    const DsCase cji = ds_case<double>(T.ri[k], si, dij, rgbmax);
-                    const bool neck = p.has_neck && dij < rbi + T.rb[k] + NECK_CUT;
+                const double r2d = dx * dx + dy * dy + dz * dz;
+                if (r2d <= rp2) {
+                    r2 = (R)r2d;
+                    dij1i = gbm<R>::rsqrt_(r2);
+                    const R dij = r2 * dij1i;
+                    const R sj = T.sj[k];
+                    const DsCase cij = ds_case<R>(ri, sj, dij, rgbmax);
+                    const DsCase cji = ds_case<R>(T.ri[k], si, dij, rgbmax);
+                    const bool neck = p.has_neck && dij < rbi + T.rb[k] + (R)NECK_CUT;
                     defer_ij = !cij.skip && (cij.a || cij.c || cij.d || cij.e || neck);
                     defer_ji = !cji.skip && (cji.a || cji.c || cji.d || cji.e || neck);
-                    const double dij2i = dij1i * dij1i;
-                    double tij = 0.0, tji = 0.0;
+                    const R dij2i = dij1i * dij1i;
+                    R tij = R(0), tji = R(0);
                     if (cij.b && !defer_ij) {
-                        const double sj2 = sj * sj, tmpsd = sj2 * dij2i;
-                        const double dumbo = GB_TA + tmpsd * (GB_TB + tmpsd * (GB_TC + tmpsd * (GB_TD + tmpsd * GB_TDD)));
-                        const double dumbd = GB_TE + tmpsd * (GB_TF + tmpsd * (GB_TG + tmpsd * (GB_TH + tmpsd * GB_THH)));
+                        const R sj2 = sj * sj, tmpsd = sj2 * dij2i;
+                        const R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
+                        const R dumbd = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
                         tij = sj * tmpsd * dij2i * dumbo;
                         aij = tmpsd * sj * dij2i * dij2i * dumbd;
                     }
                     if (cji.b && !defer_ji) {
-                        const double si2 = si * si, tmpsd = si2 * dij2i;
-                        const double dumbo = GB_TA + tmpsd * (GB_TB + tmpsd * (GB_TC + tmpsd * (GB_TD + tmpsd * GB_TDD)));
-                        const double dumbd = GB_TE + tmpsd * (GB_TF + tmpsd * (GB_TG + tmpsd * (GB_TH + tmpsd * GB_THH)));
+                        const R si2 = si * si, tmpsd = si2 * dij2i;
+                        const R dumbo = R(GB_TA) + tmpsd * (R(GB_TB) + tmpsd * (R(GB_TC) + tmpsd * (R(GB_TD) + tmpsd * R(GB_TDD))));
+                        const R dumbd = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
                         tji = si * tmpsd * dij2i * dumbo;
                         aji = tmpsd * si * dij2i * dij2i * dumbd;
                     }
-                    acc -= tij;
-                    atomicAdd(&s_acc[k], -tji);
+                    acc -= (double)tij;
+                    atomicAdd(&s_acc[k], -(double)tji);
                 }
             }
             /* derivative-store entries: the immediate ones now, the
              * deferred ones when their queue entry runs */
-            double2 *slot = nullptr;
+            R2 *slot = nullptr;
             if (store) {
                 const unsigned long long bal = __ballot(listed);
                 if (listed) slot = sp + lanes_below(bal);
@@ -796,7 +802,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
                 qn += __popcll(bal);
                 if (qn >= TILE) {
                     __syncthreads();
-                    born_queue_run(Q, qh, TILE, T, npos, nval, s_row, s_acc, p);
+                    born_queue_run<R>(Q, qh, TILE, T, npos, nval, s_row, s_acc, p);
                     __syncthreads();
                     qh = (qh + TILE) & (BQ_CAP - 1);
                     qn -= TILE;
@@ -805,7 +811,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
             k = k + 1 == m ? 0 : k + 1;
         }
         __syncthreads();
-        born_queue_run(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
+        born_queue_run<R>(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
         __syncthreads();
         acc += s_row[l];
         if (p.det) {
@@ -2094,8 +2100,8 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
     const bool sp1 = !det && E.par.split_sh[0] > 0, sp2 = !det && E.par.split_sh[1] > 0,
                sp3 = !det && E.par.split_sh[2] > 0;
     if (E.precision == GBHIP_FP64 && E.born_queue)
-        hipLaunchKernelGGL((sp1 ? k_born_q<true> : k_born_q<false>), dim3(E.pass_blocks[0]), dim3(64), 0, E.stream,
-                           E.d, E.par);
+        hipLaunchKernelGGL((sp1 ? k_born_q<double, true> : k_born_q<double, false>), dim3(E.pass_blocks[0]), dim3(64),
+                           0, E.stream, E.d, E.par);
     else if (E.precision == GBHIP_FP64)
         hipLaunchKernelGGL((sp1 ? k_born<double, true> : k_born<double, false>), dim3(E.pass_blocks[0]), dim3(64), 0,
                            E.stream, E.d, E.par);
