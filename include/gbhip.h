@@ -181,6 +181,19 @@ int gbhip_get_reff(gbhip_ctx *ctx, double *reff);
 /* sumdeijda (dE/dR_i * R_i^2) of the last evaluation, original order. */
 int gbhip_get_sumdeijda(gbhip_ctx *ctx, double *sumdeijda);
 
+/*
+ * Hessian of the mme energy (bonded, 1-4 and GB terms; gbhip_set_bonded
+ * must have been called) at the context's coordinates and current list, by
+ * central differences of the analytic gradient with step h (A): column c =
+ * (g(x + h e_c) - g(x - h e_c)) / 2h, two device evaluations per column,
+ * then symmetrised.  hess: host n3 x n3 (n3 = 3 natom); grad (n3) and
+ * ene14 (mme34 energies + rms gradient) at x, either may be NULL.  The list
+ * is not rebuilt.  Replaces the analytic second derivatives of sff2.c
+ * (egb2 at sff2.c:2271 and the bonded terms' ebond2/eangl2/ephi2); use an
+ * FP64 context (differences of the mixed gradient are not accurate).
+ */
+int gbhip_hessian(gbhip_ctx *ctx, double h, double *hess, double *grad, double *ene14);
+
 /* ---- device-resident use (no host copies; asynchronous on the stream) ---- */
 
 /* Stream for all later work of this context (a hipStream_t; NULL =

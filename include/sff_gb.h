@@ -11,6 +11,9 @@
  *   mme           sff.c:1854         energy + gradient, returns the total
  *   md            sff.c:2071         velocity-Verlet / Langevin dynamics
  *   conjgrad      conjgrad.c:34      Polak-Ribiere CG with Powell restarts
+ *   mme2          sff2.c:6176        energy, gradient and Hessian
+ *   newton        newton.c:357       Newton-Raphson minimiser with line search
+ *   nmode         nmode.c:1596       normal modes and thermochemistry
  *   getxv, putxv  sff.c:651, :733    rst7 coordinate / velocity files
  *
  * The generalized-Born pair work (egb() and nblist() of eff.c/nblist.c) runs
@@ -134,6 +137,44 @@ int md(int n, int maxstep, double *x, double *f, double *v,
 int conjgrad(double x[], int *n, double *f,
              double (*func)(double *, double *, int *),
              double *rmsgrad, double *dfpred, int *maxiter);
+
+/*
+ * Second derivatives (sff.h:142-162; the non-ScaLAPACK meaning of the
+ * arguments: the descriptor and context pointers are not read).
+ *
+ * mme2(x, f, g, m, grad, ..., natom, iter, name): energy (returned),
+ * gradient f[3N] (also copied to grad when not NULL), Hessian g[3N*3N]
+ * (symmetric), atomic masses m[N], *natom, and name[i] the first letter of
+ * atom i's name.  iter -3 releases its state.  The Hessian is the device's
+ * (gbhip_hessian): central differences (step 1e-4 A) of the analytic FP64
+ * gradient of every term, where sff2.c forms it analytically; the
+ * reference's normal modes of test/nab/asp_nm1 reproduce at their printed
+ * precision.  Whatever gbprec says, mme2 evaluates in FP64.
+ *
+ * newton(x, n, f, func1, func2, rmsgrad, nradd, maxiter): Newton-Raphson
+ * (newton.c:357-1371): Hessian from func2, translations and rotations
+ * level-shifted (3500 d d^T), nradd added to the diagonal, the step from a
+ * Cholesky solve (rocSOLVER on the GPU), then a cubic line search along it
+ * with func1 unless the step is already below 1e-4 rms.  Returns the
+ * iteration count on convergence (sum g^2 < rmsgrad^2 n), -1 after maxiter
+ * iterations, -2 when the Hessian is not positive definite.
+ *
+ * nmode(x, n, func, eigp, ntrun, eta, hrmax, ioseen): mass-weighted Hessian
+ * from func, its eigenvalues (rocSOLVER) as frequencies in cm^-1 and the
+ * thermochemistry of nmode.c's thermo() at 298.15 K and 1 atm.  ntrun 0
+ * and 1 print the full table (and with eigp > 0 write the first eigp modes
+ * to the file "vecs"); ntrun 2 prints the lowest eigp frequencies (all
+ * eigenvalues are computed, where the reference uses ARPACK).  ntrun 3
+ * (Langevin modes) is not provided (returns -1).
+ */
+double mme2(double *x, double *f, double *g, double *m, double *grad, int *descF_PxQ, int *descF_1x1,
+            int *descG_PxQ, int *context_PxQ, int *context_1x1, int *context_Nx1, int *gridim, int *natom,
+            int *iter, char *name);
+typedef double (*nabc_mme2_fn)(double *, double *, double *, double *, double *, int *, int *, int *, int *, int *,
+                               int *, int *, int *, int *, char *);
+int newton(double x[], int *n, double *f, double (*func1)(double *, double *, int *), nabc_mme2_fn func2,
+           double *rmsgrad, double *nradd, int *maxiter);
+int nmode(double *x, int n, nabc_mme2_fn func, int eigp, int ntrun, double eta, double hrmax, int ioseen);
 
 /* Output stream of the ff:/md: lines and mm_options echo (libsff's
  * nabout, sff.h:119; NULL means stdout); mm_set_output sets it too. */

@@ -127,6 +127,7 @@ _sig("gbhip_set_coords", C.c_int, [vp, dp])
 _sig("gbhip_nblist", C.c_int, [vp, C.POINTER(C.c_longlong)])
 _sig("gbhip_nblist_async", C.c_int, [vp])
 _sig("gbhip_egb", C.c_int, [vp, dp, dp, dp, dp])
+_sig("gbhip_hessian", C.c_int, [vp, C.c_double, dp, dp, dp])
 _sig("gbhip_egb_static", C.c_int, [C.POINTER(Topology), C.POINTER(Options), dp, dp, dp, dp, dp])
 _sig("gbhip_get_reff", C.c_int, [vp, dp])
 _sig("gbhip_get_sumdeijda", C.c_int, [vp, dp])
@@ -269,6 +270,11 @@ _sig("conjgrad", C.c_int, [dp, ip, dp, MMEFUNC, dp, dp, ip])
 _sig("nabc_md_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])
 
 MME_C = MMEFUNC(("mme", lib))
+MME2FUNC = C.CFUNCTYPE(C.c_double, dp, dp, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip, ip, C.c_char_p)
+_sig("mme2", C.c_double, [dp, dp, dp, dp, dp, ip, ip, ip, ip, ip, ip, ip, ip, ip, C.c_char_p])
+_sig("newton", C.c_int, [dp, ip, dp, MMEFUNC, MME2FUNC, dp, dp, ip])
+_sig("nmode", C.c_int, [dp, C.c_int, MME2FUNC, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int])
+MME2_C = MME2FUNC(("mme2", lib))
 
 
 class GBHipError(RuntimeError):

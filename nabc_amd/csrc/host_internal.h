@@ -59,6 +59,10 @@ int nabc_model_context_in(const nabc_model *m, const double *x, const nabc_opts_
                           struct gbhip_slot *slot, struct gbhip_ctx **out);
 
 void nabc_seterr(const char *fmt, ...);
+
+/* dense symmetric solvers on the device (dense.cpp, rocSOLVER) */
+int nabc_dense_eigh(int n, double *A, double *w, int vectors);
+int nabc_dense_posv(int n, const double *A, double *b);
 int nabc_atomic_number_from_mass(double mass);
 void nabc_build_14_list(PARMSTRUCT_T *prm);
 void nabc_parm_complete(PARMSTRUCT_T *prm);

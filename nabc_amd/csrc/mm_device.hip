@@ -453,6 +453,106 @@ extern "C" int gbhip_mme(gbhip_ctx *ctx, double *grad, double *ene14)
     return 0;
 }
 
+/* ------------------------------------------------------ second derivatives */
+
+/* coordinate c of x set to x0[c] + delta (the others are untouched) */
+__global__ void k_hess_set(double *x, const double *x0, long long c, double delta)
+{
+    if (threadIdx.x == 0) x[c] = x0[c] + delta;
+}
+
+/* one column of the Hessian: (g(x + h e_c) - g(x - h e_c)) / 2h */
+__global__ __launch_bounds__(MM_THREADS) void k_hess_col(double *__restrict__ col, const double *__restrict__ gp,
+                                                         const double *__restrict__ gm, double inv2h, long long n)
+{
+    const long long r = (long long)blockIdx.x * MM_THREADS + threadIdx.x;
+    if (r < n) col[r] = (gp[r] - gm[r]) * inv2h;
+}
+
+/* H <- (H + H^T) / 2 over 16 x 16 blocks of the upper triangle */
+__global__ __launch_bounds__(256) void k_hess_sym(double *H, long long n)
+{
+    const long long r = (long long)blockIdx.y * 16 + threadIdx.y, c = (long long)blockIdx.x * 16 + threadIdx.x;
+    if (r < n && c < n && r < c) {
+        const double a = 0.5 * (H[r * n + c] + H[c * n + r]);
+        H[r * n + c] = a;
+        H[c * n + r] = a;
+    }
+}
+
+/*
+ * Hessian of the mme energy (bonded, 1-4 and GB terms) at the context's
+ * coordinates and pair list, by central differences of the analytic
+ * gradient: every column is two device evaluations at x +- h e_c (the
+ * captured mme graph when it is on), differenced on the device, and the
+ * result symmetrised.  The coordinates are restored and evaluated once
+ * more, so grad / ene14 are those at x.  The list is not rebuilt.
+ */
+extern "C" int gbhip_hessian(gbhip_ctx *ctx, double h, double *hess, double *grad, double *ene14)
+{
+    if (!ctx || !hess) return seterr(GBHIP_EINVAL, "NULL argument");
+    if (!(h > 0.0) || !std::isfinite(h)) return seterr(GBHIP_EINVAL, "difference step must be positive");
+    Engine &E = ctx->E;
+    if (!ctx->B.set) return seterr(GBHIP_ESTATE, "gbhip_set_bonded has not been called");
+    if (!E.have_list) return seterr(GBHIP_ESTATE, "hessian called before the pair list was built");
+    const long long n3 = 3LL * E.natoms;
+    double *x0 = nullptr, *gp = nullptr, *dH = nullptr;
+    {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        if (hipMalloc(&x0, n3 * sizeof(double)) != hipSuccess || hipMalloc(&gp, n3 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&dH, (size_t)n3 * (size_t)n3 * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (x0) (void)hipFree(x0);
+            if (gp) (void)hipFree(gp);
+            return seterr(GBHIP_ENOMEM, "hipMalloc of the %lld x %lld Hessian failed", n3, n3);
+        }
+    }
+    int rc = 0;
+    auto fail = [&](int code) {
+        rc = code;
+        return code;
+    };
+    const unsigned nb = (unsigned)((n3 + MM_THREADS - 1) / MM_THREADS);
+    if (hipMemcpyAsync(x0, E.x_dev, n3 * sizeof(double), hipMemcpyDeviceToDevice, E.stream) != hipSuccess)
+        fail(seterr(GBHIP_EHIP, "coordinate copy failed"));
+    for (long long c = 0; c < n3 && !rc; c++) {
+        hipLaunchKernelGGL(k_hess_set, dim3(1), dim3(64), 0, E.stream, E.x_dev, x0, c, h);
+        if (engine_mme(ctx)) { fail(GBHIP_EHIP); break; }
+        if (hipMemcpyAsync(gp, E.g_dev, n3 * sizeof(double), hipMemcpyDeviceToDevice, E.stream) != hipSuccess) {
+            fail(seterr(GBHIP_EHIP, "gradient copy failed"));
+            break;
+        }
+        hipLaunchKernelGGL(k_hess_set, dim3(1), dim3(64), 0, E.stream, E.x_dev, x0, c, -h);
+        if (engine_mme(ctx)) { fail(GBHIP_EHIP); break; }
+        hipLaunchKernelGGL(k_hess_col, dim3(nb), dim3(MM_THREADS), 0, E.stream, dH + c * n3, gp, E.g_dev,
+                           0.5 / h, n3);
+        hipLaunchKernelGGL(k_hess_set, dim3(1), dim3(64), 0, E.stream, E.x_dev, x0, c, 0.0);
+    }
+    if (!rc) {
+        const unsigned nt = (unsigned)((n3 + 15) / 16);
+        hipLaunchKernelGGL(k_hess_sym, dim3(nt, nt), dim3(16, 16), 0, E.stream, dH, n3);
+        if (hipGetLastError() != hipSuccess) fail(seterr(GBHIP_EHIP, "kernel launch failed"));
+    }
+    /* the gradient and energies at x itself */
+    if (!rc && engine_mme(ctx)) fail(GBHIP_EHIP);
+    if (!rc && hipMemcpyAsync(hess, dH, (size_t)n3 * (size_t)n3 * sizeof(double), hipMemcpyDeviceToHost,
+                              E.stream) != hipSuccess)
+        fail(seterr(GBHIP_EHIP, "Hessian copy failed"));
+    if (!rc && grad && hipMemcpyAsync(grad, E.g_dev, n3 * sizeof(double), hipMemcpyDeviceToHost, E.stream) != hipSuccess)
+        fail(seterr(GBHIP_EHIP, "gradient copy failed"));
+    if (!rc && ene14 && hipMemcpyAsync(ene14, E.ene14_dev, 14 * sizeof(double), hipMemcpyDeviceToHost, E.stream) !=
+                            hipSuccess)
+        fail(seterr(GBHIP_EHIP, "energy copy failed"));
+    if (hipStreamSynchronize(E.stream) != hipSuccess && !rc) fail(seterr(GBHIP_EHIP, "stream synchronisation failed"));
+    {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
+        (void)hipFree(x0);
+        (void)hipFree(gp);
+        (void)hipFree(dH);
+    }
+    return rc;
+}
+
 /* ------------------------------------------------------------- dynamics */
 
 /*

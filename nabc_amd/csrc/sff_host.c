@@ -29,6 +29,7 @@ static struct {
     int have_list;
     long long nb_pairs;
     double ene[14];     /* last mme(): mme34 energies + rms gradient */
+    gbhip_ctx *ctx2;    /* mme2(): an FP64 context of its own */
     nabc_nctraj traj;   /* md() trajectory, when mme_init_sff was given a file name */
 } S;
 
@@ -39,6 +40,8 @@ static void release_ctx(void)
     if (S.ctx) gbhip_ctx_destroy(S.ctx);
     S.ctx = NULL;
     S.have_list = 0;
+    if (S.ctx2) gbhip_ctx_destroy(S.ctx2);
+    S.ctx2 = NULL;
 }
 
 int mme_init_sff(PARMSTRUCT_T *prm, int *frozen, int *constrained, double *x0, char *ncname)
@@ -145,6 +148,65 @@ double mme(double *x, double *f, int *iter)
         fprintf(out(), "    esurf:  %15.9f\n", e[11]);
         fprintf(out(), "    Total:  %15.9f\n", e[0]);
         fprintf(out(), "     frms:  %15.9f\n", e[13]);
+    }
+    return S.ene[0];
+}
+
+/* ------------------------------------------------------------------ mme2 */
+
+#define HESS_STEP 1.0e-4 /* A, central-difference step of the Hessian columns */
+
+/*
+ * mme2() (sff2.c:6176-6835, non-ScaLAPACK): energy, gradient, Hessian,
+ * masses and atom-name letters.  The pair list is rebuilt at every call
+ * (the Hessian needs it at x) and the evaluation is FP64 whatever gbprec
+ * says: the columns are differences of gradients.  The header line at
+ * iterations 0 and 1 and the ff: line at iteration 1 and every ntpr-th
+ * (sff2.c:6770-6775) go out as in the reference.
+ */
+double mme2(double *x, double *f, double *g, double *m, double *grad, int *descF_PxQ, int *descF_1x1,
+            int *descG_PxQ, int *context_PxQ, int *context_1x1, int *context_Nx1, int *gridim, int *natom,
+            int *iter, char *name)
+{
+    const nabc_opts_t *o = &nabc_opts;
+    const int it = iter ? *iter : 0;
+    int i, n;
+    long long np = 0;
+    (void)descF_PxQ; (void)descF_1x1; (void)descG_PxQ;
+    (void)context_PxQ; (void)context_1x1; (void)context_Nx1; (void)gridim;
+    if (it == -3) {
+        if (S.ctx2) gbhip_ctx_destroy(S.ctx2);
+        S.ctx2 = NULL;
+        return 0.0;
+    }
+    if (!S.m.prm) {
+        nabc_seterr("mme2: mme_init_sff has not been called");
+        return NAN;
+    }
+    n = S.m.prm->Natom;
+    if (natom) *natom = n;
+    for (i = 0; i < n; i++) {
+        if (m) m[i] = S.m.prm->Masses[i];
+        if (name) name[i] = S.m.prm->AtomNames ? S.m.prm->AtomNames[i][0] : ' ';
+    }
+    if (!S.ctx2) {
+        nabc_opts_t o64 = *o;
+        o64.gbprec = 0;
+        if (nabc_model_context(&S.m, x, &o64, NULL, &S.ctx2) != 0) return NAN;
+    }
+    if (gbhip_set_coords(S.ctx2, x) != 0 || gbhip_nblist(S.ctx2, &np) != 0 ||
+        gbhip_hessian(S.ctx2, HESS_STEP, g, f, S.ene) != 0) {
+        nabc_seterr("%s", gbhip_last_error());
+        return NAN;
+    }
+    if (grad && grad != f) memcpy(grad, f, sizeof(double) * 3 * (size_t)n);
+    if (it == 0 || it == 1)
+        fprintf(out(), "      iter    Total       bad      vdW     elect   nonpolar   genBorn      frms\n");
+    if (it > -1 && (it == 1 || it % o->ntpr == 0)) {
+        const double *e = S.ene;
+        printf("ff:%6d %9.2f %9.2f %9.2f %9.2f %9.2f %9.2f %9.2e\n", it, e[0], e[3] + e[4] + e[5], e[1] + e[7],
+               e[2] + e[8], e[9] + e[11] + e[12], e[10], e[13]);
+        fflush(stdout);
     }
     return S.ene[0];
 }

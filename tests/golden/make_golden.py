@@ -20,6 +20,13 @@ golden.json, each with its source file:line.
            old-format prmtop (GB radii from the atom names); gbrna.pdb's
            coordinates are also written as gbrna.rst7 (file order, the
            order of the prmtop's atoms).
+  asp_nm1  test/nab/asp_nm1.nab, asp_nm1.out.check:1-66: normal modes of
+           asp.top (gb=1, cut = rgbmax = 100) after 500 CG and 15
+           Newton-Raphson iterations: modes 7-72, frequency (cm^-1) and
+           the per-mode E, Cv, S of thermo(), 3 decimals.
+  asp_nm2  test/nab/asp_nm2.nab, asp_nm2.out.check:1-15: the same minimum's
+           lowest 20 frequencies (modes 6-20 kept by the test script), 5
+           decimals, compared at 1e-4 (Run.nmode2: dacdif -a 1.e-4).
 """
 import json
 import os
@@ -102,6 +109,24 @@ def main():
                       options=dict(gb=1, kappa=0.10395, rgbmax=99.0, cut=99.0, epsext=78.5, diel="C",
                                    nsnb=10, cg=dict(maxiter=10, dfpred=10.0, rmsgrad_per_dof=0.0001)),
                       ff=ff_lines(os.path.join(REF, "gbrna.out.check")))
+    # asp_nm1 / asp_nm2: normal modes (asp_nm1.nab:14-30, asp_nm2.nab:14-30)
+    nm = dict(gb=1, cut=100.0, rgbmax=100.0, kappa=0.0, diel="C", nsnb=999,
+              cg=dict(rmsgrad=0.005, dfpred=1.0, maxiter=500), newton=dict(rmsgrad=1e-12, nradd=0.0, maxiter=15))
+    rows = []
+    for k, line in enumerate(open(os.path.join(REF, "asp_nm1.out.check")), 1):
+        f = line.split()
+        if len(f) == 5 and f[0].isdigit():
+            rows.append(dict(line=k, mode=int(f[0]), freq=float(f[1]), e=float(f[2]), cv=float(f[3]),
+                             s=float(f[4])))
+    g["asp_nm1"] = dict(source="test/nab/asp_nm1.out.check", prmtop="asp.top", pdb="asp.pdb", options=nm,
+                        nmode=dict(eigp=0, ntrun=1), modes=rows)
+    rows = []
+    for k, line in enumerate(open(os.path.join(REF, "asp_nm2.out.check")), 1):
+        f = line.split()
+        if len(f) == 2 and f[0].isdigit():
+            rows.append(dict(line=k, mode=int(f[0]), freq=float(f[1])))
+    g["asp_nm2"] = dict(source="test/nab/asp_nm2.out.check", prmtop="asp.top", pdb="asp.pdb", options=nm,
+                        nmode=dict(eigp=20, ntrun=2), tolerance=1e-4, modes=rows)
     with open(os.path.join(HERE, "golden.json"), "w") as fp:
         json.dump(g, fp, indent=1)
 
