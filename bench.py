@@ -463,6 +463,59 @@ def bench_cg(args):
                       "unit": "us/evaluation", "iterations": maxiter, "rows": rows}))
 
 
+def bench_hessian(args):
+    """Second derivatives: mme2() (the device Hessian of the full mme energy
+    by central differences of the FP64 gradient, 6N device evaluations)
+    and nmode() (mass weighting, the eigenproblem on the GPU and the
+    thermochemistry) at 300, 1,000 and 2,500 atoms (gb=5, cut = rgbmax = 20
+    A), after one untimed mme2 call that creates the context."""
+    import ctypes
+    import numpy as np
+    import nabc_amd as na
+    libc = ctypes.CDLL(None)
+    libc.fopen.restype = ctypes.c_void_p
+    libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    libc.fclose.argtypes = [ctypes.c_void_p]
+    na.lib.mm_set_output.argtypes = [ctypes.c_void_p]
+    rows = []
+    for natom in (300, 1000, 2500):
+        parm, arr, x = na.build_system(natom, 5150 + natom)
+        x = np.ascontiguousarray(x)
+        na.lib.mm_reset_options()
+        assert na.lib.mm_options(b"gb=5, cut=20.0, rgbmax=20.0, ntpr=100000000") == 0
+        assert na.lib.mme_init_sff(parm, None, None, None, None) == 0
+        n3 = 3 * natom
+        f, H, m = np.zeros(n3), np.zeros(n3 * n3), np.zeros(natom)
+        name = ctypes.create_string_buffer(natom + 1)
+        nat, it = na.C.c_int(), na.C.c_int(-2)
+        dummy = [na.C.c_int(0) for _ in range(7)]
+
+        def mme2():
+            return na.lib.mme2(x.ctypes.data_as(na.dp), f.ctypes.data_as(na.dp), H.ctypes.data_as(na.dp),
+                               m.ctypes.data_as(na.dp), None, *[na.C.byref(d) for d in dummy], na.C.byref(nat),
+                               na.C.byref(it), name)
+        mme2()
+        t0 = time.perf_counter()
+        e = mme2()
+        t_h = time.perf_counter() - t0
+        fp = libc.fopen(b"/dev/null", b"w")
+        na.lib.mm_set_output(fp)
+        t0 = time.perf_counter()
+        ret = na.lib.nmode(x.ctypes.data_as(na.dp), n3, na.MME2_C, 0, 1, 0.0, 0.0, 0)
+        t_nm = time.perf_counter() - t0
+        na.lib.mm_set_output(None)
+        libc.fclose(fp)
+        itr = na.C.c_int(-3)
+        na.lib.mme(x.ctypes.data_as(na.dp), x.copy().ctypes.data_as(na.dp), na.C.byref(itr))
+        row = {"natom": natom, "hessian_s": round(t_h, 4), "evaluations": 2 * n3 + 1,
+               "us_per_evaluation": round(1e6 * t_h / (2 * n3 + 1), 1), "nmode_s": round(t_nm, 4),
+               "nmode_return": ret, "energy": e}
+        rows.append(row)
+        print("# %s" % json.dumps(row), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "second derivatives: mme2 Hessian and nmode wall time", "unit": "s",
+                      "rows": rows}))
+
+
 def bench_sweep(args):
     """Size sweep of the headline workload (igb=8, cut = rgbmax = 25 A,
     kappa 0.127315, list rebuilt every 20 evaluations): evaluations/s and
@@ -517,7 +570,8 @@ def main():
                     help="fixed workgroup counts for the FP64 run only (else --launch applies to both)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
-    ap.add_argument("--config", default="c3", choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg"],
+    ap.add_argument("--config", default="c3",
+                    choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg", "hessian"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
                          "errors against the oracle at C1/C2/C3; sweep: evaluation rate at 5k-100k atoms")
@@ -552,6 +606,8 @@ def main():
         return bench_sweep(args)
     if args.config == "cg":
         return bench_cg(args)
+    if args.config == "hessian":
+        return bench_hessian(args)
 
     import numpy as np
     import nabc_amd as na
