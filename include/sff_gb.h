@@ -154,13 +154,14 @@ int conjgrad(double x[], int *n, double *f,
  * newton(x, n, f, func1, func2, rmsgrad, nradd, maxiter): Newton-Raphson
  * (newton.c:357-1371): Hessian from func2, translations and rotations
  * level-shifted (3500 d d^T), nradd added to the diagonal, the step from a
- * Cholesky solve (rocSOLVER on the GPU), then a cubic line search along it
+ * Cholesky solve, then a cubic line search along it
  * with func1 unless the step is already below 1e-4 rms.  Returns the
  * iteration count on convergence (sum g^2 < rmsgrad^2 n), -1 after maxiter
  * iterations, -2 when the Hessian is not positive definite.
  *
  * nmode(x, n, func, eigp, ntrun, eta, hrmax, ioseen): mass-weighted Hessian
- * from func, its eigenvalues (rocSOLVER) as frequencies in cm^-1 and the
+ * from func, its eigenvalues (Householder reduction, on the GPU from order
+ * 1024, and QR iteration) as frequencies in cm^-1 and the
  * thermochemistry of nmode.c's thermo() at 298.15 K and 1 atm.  ntrun 0
  * and 1 print the full table (and with eigp > 0 write the first eigp modes
  * to the file "vecs"); ntrun 2 prints the lowest eigp frequencies (all

@@ -60,7 +60,7 @@ int nabc_model_context_in(const nabc_model *m, const double *x, const nabc_opts_
 
 void nabc_seterr(const char *fmt, ...);
 
-/* dense symmetric solvers on the device (dense.cpp, rocSOLVER) */
+/* dense symmetric solvers (dense.hip) */
 int nabc_dense_eigh(int n, double *A, double *w, int vectors);
 int nabc_dense_posv(int n, const double *A, double *b);
 int nabc_atomic_number_from_mass(double mass);

@@ -10,8 +10,9 @@
  *
  * The Hessian comes from the caller's mme2-like function (normally the
  * library's mme2, the device's central differences); the dense Cholesky
- * solve and the symmetric eigenproblem run on the GPU (dense.cpp,
- * rocSOLVER) where the reference calls LAPACK's dposv_/dsyev_/dsyevd_.
+ * solve and the symmetric eigenproblem are dense.hip's (Householder
+ * reduction on the GPU for large matrices, QR iteration on the host) where
+ * the reference calls LAPACK's dposv_/dsyev_/dsyevd_.
  */
 #include <math.h>
 #include <stdio.h>

@@ -6,11 +6,13 @@
   300-atom gb=5 system: 1e-7 of the largest element.  The gradient and
   energy mme2 returns against the oracle's to 1e-9.
 - test/nab/asp_nm1 and asp_nm2 run through the library: conjgrad over mme
-  (device vectors), newton over mme/mme2 (dense Cholesky on the GPU), then
-  nmode (mass-weighted eigenproblem on the GPU): the printed table matches
+  (device vectors), newton over mme/mme2 (the library's dense Cholesky),
+  then nmode (the library's symmetric eigensolver): the printed table matches
   asp_nm1.out.check (66 rows x 4 columns at their printed precision) and
   the lowest frequencies match asp_nm2.out.check (modes 7-20 to 1e-4; mode
   6 is a zero mode, see tests/test_oracle_nmode.py).
+- The eigensolver's GPU path (Householder reduction on the device, used
+  from order 1024 on) against numpy's LAPACK on a random symmetric matrix.
 """
 import ctypes
 import os
@@ -157,3 +159,26 @@ def test_asp_nm2_through_the_library(gpu, tmp_path, capfd):
             continue
         assert abs(got[r["mode"]] - r["freq"]) <= g["tolerance"], (r, got[r["mode"]])
     assert vecs.startswith(" modes from NAB") and vecs.count("****") == g["nmode"]["eigp"]
+
+
+@pytest.mark.parametrize("vectors", [0, 1])
+def test_device_tridiagonalization_eigenvalues(gpu, vectors):
+    """Matrices of order >= 1024 are reduced on the GPU (dense.hip):
+    eigenvalues against numpy's LAPACK, eigenvectors by their residual."""
+    na = gpu
+    f = na.lib.nabc_dense_eigh
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, na.dp, na.dp, ctypes.c_int]
+    n = 1200
+    rng = np.random.default_rng(12)
+    B = rng.normal(size=(n, n))
+    A = np.ascontiguousarray(B + B.T)
+    a = A.copy()
+    w = np.zeros(n)
+    assert f(n, a.ctypes.data_as(na.dp), w.ctypes.data_as(na.dp), vectors) == 0
+    ref = np.linalg.eigvalsh(A)
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(w - ref)) <= 1e-11 * scale
+    if vectors:
+        V = a.reshape(n, n)
+        assert np.max(np.abs(V @ A - w[:, None] * V)) <= 1e-9 * scale
