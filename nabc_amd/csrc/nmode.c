@@ -445,7 +445,11 @@ int nmode(double *x, int n, nabc_mme2_fn func, int eigp, int ntrun, double eta, 
     }
     energy = func(x, g, h, m, grad, &dummy[0], &dummy[1], &dummy[2], &dummy[3], &dummy[4], &dummy[5], &dummy[6],
                   &natom, &niter, name);
-    if (isnan(energy) || natom * 3 != n) goto done;
+    if (isnan(energy)) goto done; /* the energy function has set the message */
+    if (natom * 3 != n) {
+        nabc_seterr("nmode: n must be 3 * natom");
+        goto done;
+    }
     for (i = 0; i < n; i++) sumg += g[i] * g[i];
     sumg = sqrt(sumg / n);
     fprintf(nm_out(), "\nEnergy       = %20.10e\n", energy);
