@@ -115,13 +115,24 @@ struct DsCase {
     bool skip, a, b, c, d, e;
 };
 
-template <typename R>
+/*
+ * NEAR = false: the caller guarantees dij > 4 sj for every pair it passes
+ * (a far tile pair, see k_born), so the near cases C, D, E cannot occur;
+ * they and u1 are compiled out and the log takes case A's argument.  For
+ * such pairs the results equal those of NEAR = true.
+ */
+template <typename R, bool NEAR = true>
 __device__ __forceinline__ DsCase ds_case(R ri, R sj, R dij, R rgbmax)
 {
     DsCase k;
     k.skip = dij > rgbmax + sj;
     k.a = !k.skip && dij > rgbmax - sj;
     const bool rest = !k.skip && !k.a;
+    if constexpr (!NEAR) {
+        k.b = rest;
+        k.c = k.d = k.e = false;
+        return k;
+    }
     k.b = rest && dij > R(4) * sj;
     k.c = rest && !k.b && dij > ri + sj;
     k.d = rest && !k.b && !k.c && dij > fabs(ri - sj);
@@ -129,12 +140,19 @@ __device__ __forceinline__ DsCase ds_case(R ri, R sj, R dij, R rgbmax)
     return k;
 }
 
-template <typename R>
+template <typename R, bool NEAR = true>
 __device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R dij, R rgbmax1i,
                                          R &u1, R &u2, R &L)
 {
     typedef gbm<R> M;
     const R dm = dij - sj;
+    if constexpr (!NEAR) {
+        /* only case A uses them */
+        u1 = R(0);
+        u2 = M::rcp(dm);
+        L = M::log_(dm * rgbmax1i);
+        return;
+    }
     u1 = M::rcp(dij + sj);
     u2 = M::rcp(k.a ? dm : r2 - sj * sj);
     R arg;
@@ -160,11 +178,11 @@ __device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R di
  * the other.  skip is set when dij > rgbmax + sj (the pair is dropped,
  * neck included, in both passes).
  */
-template <typename R>
+template <typename R, bool NEAR = true>
 __device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
                                         R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip, R &dv)
 {
-    const DsCase k = ds_case<R>(ri, sj, dij, rgbmax);
+    const DsCase k = ds_case<R, NEAR>(ri, sj, dij, rgbmax);
     skip = k.skip;
     R u1 = R(0), u2 = R(0), L = R(0);
     if (sizeof(R) == sizeof(double)) {
@@ -172,11 +190,11 @@ __device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i
          * wavefront step whose lanes are all in case B (or skipped) jumps
          * over them (measured: pass 1 of 1,024 molecules without cutoff
          * 9.7 -> 6.4 ms) */
-        if (k.a || k.c || k.d || k.e) ds_trans<R>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
+        if (k.a || k.c || k.d || k.e) ds_trans<R, NEAR>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
     } else {
         /* FP32: the hardware instructions are cheaper than the branch
          * (measured: pass 1 at 25k atoms, 25 A, 4 % slower with it) */
-        ds_trans<R>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
+        ds_trans<R, NEAR>(k, ri, sj, r2, dij, rgbmax1i, u1, u2, L);
     }
     const R sj2 = sj * sj;
     const R dij2i = dij1i * dij1i;

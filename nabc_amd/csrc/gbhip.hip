@@ -504,6 +504,73 @@ __device__ __forceinline__ double *det_side(const DevPtrs &d, int t, int side)
 }
 
 /*
+ * Kinds of a pass-1 visit.  The descreening's near cases C-E need
+ * dij <= 4 sj and the neck term dij < rbi + rbj + 6.8 A (eff.c:1680-1708);
+ * most tile pairs at a 25 A cutoff lie farther apart than either bound, and
+ * for them the sweep is compiled without those branches and the extra
+ * reciprocal and selects they need (BORN_FAR, or BORN_NECK when only the
+ * neck range is reachable).  The kind is decided per visit from the current
+ * coordinates: the box of the visit's column atoms, and for every lane with
+ * a listed pair its distance to that box, a lower bound of its pair
+ * distances; a pair can only be near if that bound is.  The margin covers
+ * the FP32 rounding of coordinates and of dij; results equal those of the
+ * full sweep.
+ */
+#define BORN_NEAR 0
+#define BORN_NECK 1
+#define BORN_FAR 2
+#define BORN_MARGIN 0.05f
+
+/*
+ * Wavefront minimum and maximum of a float (every lane active), on keys
+ * that order as the floats do when compared as signed integers (the
+ * magnitude bits of negative values flipped): DPP min/max within rows of
+ * 16 lanes (quad swaps, half-row and row mirrors), then the four row
+ * results in scalar registers.
+ */
+__device__ __forceinline__ int fkey(float f)
+{
+    const int b = __float_as_int(f);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float fkey_value(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+template <bool MAX> __device__ __forceinline__ float wave_ext(float f)
+{
+    int v = fkey(f);
+    v = MAX ? max(v, dpp_i<0xB1>(v)) : min(v, dpp_i<0xB1>(v));   /* quad_perm [1,0,3,2] */
+    v = MAX ? max(v, dpp_i<0x4E>(v)) : min(v, dpp_i<0x4E>(v));   /* quad_perm [2,3,0,1] */
+    v = MAX ? max(v, dpp_i<0x141>(v)) : min(v, dpp_i<0x141>(v)); /* row_half_mirror */
+    v = MAX ? max(v, dpp_i<0x140>(v)) : min(v, dpp_i<0x140>(v)); /* row_mirror */
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return fkey_value(MAX ? max(max(r0, r1), max(r2, r3)) : min(min(r0, r1), min(r2, r3)));
+}
+
+__device__ __forceinline__ int born_visit_kind(bool row, bool col, double xi, double yi, double zi, float si,
+                                               float rbi, double xj, double yj, double zj, float sj, float rbj,
+                                               bool has_neck)
+{
+    /* the box of the column atoms and their largest sj and rb */
+    const float fxj = (float)xj, fyj = (float)yj, fzj = (float)zj;
+    const float lox = wave_ext<false>(col ? fxj : 3e30f), hix = wave_ext<true>(col ? fxj : -3e30f);
+    const float loy = wave_ext<false>(col ? fyj : 3e30f), hiy = wave_ext<true>(col ? fyj : -3e30f);
+    const float loz = wave_ext<false>(col ? fzj : 3e30f), hiz = wave_ext<true>(col ? fzj : -3e30f);
+    const float smax = wave_ext<true>(col ? sj : 0.0f), rbmax = wave_ext<true>(col ? rbj : 0.0f);
+    const float gx = fmaxf(0.0f, fmaxf(lox - (float)xi, (float)xi - hix));
+    const float gy = fmaxf(0.0f, fmaxf(loy - (float)yi, (float)yi - hiy));
+    const float gz = fmaxf(0.0f, fmaxf(loz - (float)zi, (float)zi - hiz));
+    const float dist = sqrtf(gx * gx + gy * gy + gz * gz);
+    const bool near = row && dist < 4.0f * fmaxf(si, smax) + BORN_MARGIN;
+    const bool neck = row && has_neck && dist < rbi + rbmax + (float)NECK_CUT + BORN_MARGIN;
+    if (__ballot(near)) return BORN_NEAR;
+    return __ballot(neck) ? BORN_NECK : BORN_FAR;
+}
+
+/*
  * Pass 1 (eff.c:1566-1733): for every listed pair, the descreening of i by
  * j (added to i's sum) and of j by i (added to j's sum).  The r^2 test
  * against (rgbmax+Fsmax)^2 and the FP64 sums are kept in both variants.
@@ -555,17 +622,27 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
         R2 *sp = store ? st + mq.start + (sb ? entries_before(mr, sb) : 0) : nullptr;
+        double xj = 0.0, yj = 0.0, zj = 0.0;
+        R sjj = R(0), rbj = R(0);
         __syncthreads();
         if (own) {
-            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            xj = d.xs[j]; yj = d.ys[j]; zj = d.zs[j];
+            s_x[pos] = xj; s_y[pos] = yj; s_z[pos] = zj;
             const R rj = ldri0<R>(d, j);
+            sjj = ldsj<R>(d, j); rbj = ldrb<R>(d, j);
             s_ri[pos] = rj; s_ri1i[pos] = R(1) / rj;
-            s_sj[pos] = ldsj<R>(d, j); s_rb[pos] = ldrb<R>(d, j);
+            s_sj[pos] = sjj; s_rb[pos] = rbj;
             s_nk[pos] = d.neck[j] | (d.neck[j] * NECK_N) << 8;
             s_acc[pos] = 0.0;
         }
+        const int kind = born_visit_kind(mr != 0ull, own, xi, yi, zi, (float)si, (float)rbi, xj, yj, zj,
+                                         (float)sjj, (float)rbj, p.has_neck != 0);
         __syncthreads();
         double acc = 0.0;
+        /* the step loop, compiled for the three visit kinds */
+        auto sweep = [&](auto kind_c) {
+        constexpr int KIND = decltype(kind_c)::value;
+        constexpr bool NEAR = KIND == BORN_NEAR, NECK = KIND != BORN_FAR;
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
             R aij = R(0), aji = R(0);
@@ -579,9 +656,9 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                     const R dij = r2 * dij1i;
                     const R rj = s_ri[k], sj = s_sj[k];
                     bool skip_ij, skip_ji;
-                    R tij = descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, aij);
-                    R tji = descreen_d<R>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji, aji);
-                    if (p.has_neck) {
+                    R tij = descreen_d<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, aij);
+                    R tji = descreen_d<R, NEAR>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji, aji);
+                    if (NECK && p.has_neck) {
                         const int nk = s_nk[k], nij = niN + (nk & 255), nji = (nk >> 8) + ni;
                         if (dij < rbi + s_rb[k] + neckcut) {
                             R dn;
@@ -613,6 +690,10 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             }
             k = k + 1 == m ? 0 : k + 1;
         }
+        };
+        if (kind == BORN_FAR) sweep(std::integral_constant<int, BORN_FAR>{});
+        else if (kind == BORN_NECK) sweep(std::integral_constant<int, BORN_NECK>{});
+        else sweep(std::integral_constant<int, BORN_NEAR>{});
         __syncthreads();
         if (p.det) {
             const int fl = d.tflip[t];
