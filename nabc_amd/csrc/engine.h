@@ -80,6 +80,7 @@ struct DevPtrs {
     int *tflip;           /* 1 when (A, B) = (J, I) */
     unsigned long long *tcol; /* non-empty columns (B atoms) of the bit matrix */
     unsigned long long *mask, *excl; /* bit matrices in (A, B) orientation */
+    unsigned long long *sbal; /* per tile pair and step: lanes whose derivative-store entry pass 1 wrote */
     int *tord;            /* non-empty tile pairs, most non-empty columns first */
     int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
     int *nnz;             /* number of non-empty tile pairs */

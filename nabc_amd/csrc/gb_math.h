@@ -176,14 +176,15 @@ __device__ __forceinline__ void ds_trans(const DsCase &k, R ri, R sj, R r2, R di
  * derivative numerator datmp (eff.c:2289-2334) of the same pair and
  * direction; callers that need only one of the two let the compiler drop
  * the other.  skip is set when dij > rgbmax + sj (the pair is dropped,
- * neck included, in both passes).
+ * neck included, in both passes), taylor when the pair is in case B.
  */
 template <typename R, bool NEAR = true>
-__device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
-                                        R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip, R &dv)
+__device__ __forceinline__ R descreen_dc(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
+                                         R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip, bool &taylor, R &dv)
 {
     const DsCase k = ds_case<R, NEAR>(ri, sj, dij, rgbmax);
     skip = k.skip;
+    taylor = k.b;
     R u1 = R(0), u2 = R(0), L = R(0);
     if (sizeof(R) == sizeof(double)) {
         /* FP64: log and division are long library sequences, so a
@@ -223,6 +224,28 @@ __device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i
     }
     dv = w;
     return v;
+}
+
+template <typename R, bool NEAR = true>
+__device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i,
+                                        R rgbmax, R rgbmax1i, R rgbmax2i, bool &skip, R &dv)
+{
+    bool taylor;
+    return descreen_dc<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip, taylor, dv);
+}
+
+/*
+ * The case-B (Taylor) derivative numerator alone, for a pair direction
+ * known to be in case B without a neck term: the same expression
+ * descreen_dc evaluates for it, from dij2i = 1/dij^2.
+ */
+template <typename R>
+__device__ __forceinline__ R taylor_dv(R sj, R dij2i)
+{
+    const R sj2 = sj * sj;
+    const R tmpsd = sj2 * dij2i;
+    const R dumbd = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
+    return tmpsd * sj * dij2i * dij2i * dumbd;
 }
 
 template <typename R>

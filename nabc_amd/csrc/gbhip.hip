@@ -491,6 +491,14 @@ __device__ __forceinline__ int lanes_below(unsigned long long bal)
     return __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
 }
 
+/* a 64-bit value of lane n (n the same in every lane) */
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int n)
+{
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, n);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), n);
+    return (unsigned long long)hi << 32 | lo;
+}
+
 /*
  * Deterministic mode: a visit of tile pair t leaves the sums of its lane
  * tile A and of its column tile B in two rows of DET_NC * 64 doubles (side
@@ -639,6 +647,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                                          (float)sjj, (float)rbj, p.has_neck != 0);
         __syncthreads();
         double acc = 0.0;
+        unsigned long long mybal = 0ull; /* lane l: the store mask of step sb + l */
         /* the step loop, compiled for the three visit kinds */
         auto sweep = [&](auto kind_c) {
         constexpr int KIND = decltype(kind_c)::value;
@@ -646,6 +655,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
             R aij = R(0), aji = R(0);
+            bool plain = false; /* both directions case B, no neck term: pass 3 recomputes */
             const bool listed = (mr >> s) & 1ull;
             if (listed) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
@@ -655,13 +665,17 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                     const R dij1i = gbm<R>::rsqrt_(r2);
                     const R dij = r2 * dij1i;
                     const R rj = s_ri[k], sj = s_sj[k];
-                    bool skip_ij, skip_ji;
-                    R tij = descreen_d<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, aij);
-                    R tji = descreen_d<R, NEAR>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji, aji);
+                    bool skip_ij, skip_ji, b_ij, b_ji;
+                    R tij = descreen_dc<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij,
+                                                 b_ij, aij);
+                    R tji = descreen_dc<R, NEAR>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i,
+                                                 skip_ji, b_ji, aji);
+                    plain = b_ij && b_ji;
                     if (NECK && p.has_neck) {
                         const int nk = s_nk[k], nij = niN + (nk & 255), nji = (nk >> 8) + ni;
                         if (dij < rbi + s_rb[k] + neckcut) {
                             R dn;
+                            plain = false;
                             if (!skip_ij) {
                                 tij += neck_d<R>(dij, dij1i, npos[nij], nval[nij], neckscale, dn);
                                 aij += dn;
@@ -679,14 +693,16 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
                 }
             }
             if (store) {
-                const unsigned long long bal = __ballot(listed);
-                if (listed) {
+                const bool need = listed && !plain;
+                const unsigned long long bal = __ballot(need);
+                if (need) {
                     R2 v;
                     v.x = aij;
                     v.y = aji;
                     sp[lanes_below(bal)] = v;
                 }
                 sp += __popcll(bal);
+                mybal = l == s - sb ? bal : mybal;
             }
             k = k + 1 == m ? 0 : k + 1;
         }
@@ -694,6 +710,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         if (kind == BORN_FAR) sweep(std::integral_constant<int, BORN_FAR>{});
         else if (kind == BORN_NECK) sweep(std::integral_constant<int, BORN_NECK>{});
         else sweep(std::integral_constant<int, BORN_NEAR>{});
+        if (store && l < se - sb) d.sbal[(size_t)t * TILE + sb + l] = mybal;
         __syncthreads();
         if (p.det) {
             const int fl = d.tflip[t];
@@ -818,12 +835,13 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         }
         __syncthreads();
         double acc = 0.0;
+        unsigned long long mybal = 0ull; /* lane l: the store mask of step sb + l */
         int qh = 0, qn = 0; /* ring head and length, the same in every lane */
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
             const bool listed = (mr >> s) & 1ull;
             R aij = R(0), aji = R(0), r2 = R(0), dij1i = R(0);
-            bool defer_ij = false, defer_ji = false;
+            bool defer_ij = false, defer_ji = false, plain = false;
             if (listed) {
                 const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
                 const double r2d = dx * dx + dy * dy + dz * dz;
@@ -837,6 +855,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
                     const bool neck = p.has_neck && dij < rbi + T.rb[k] + (R)NECK_CUT;
                     defer_ij = !cij.skip && (cij.a || cij.c || cij.d || cij.e || neck);
                     defer_ji = !cji.skip && (cji.a || cji.c || cji.d || cji.e || neck);
+                    plain = cij.b && cji.b && !defer_ij && !defer_ji;
                     const R dij2i = dij1i * dij1i;
                     R tij = R(0), tji = R(0);
                     if (cij.b && !defer_ij) {
@@ -861,10 +880,14 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
              * deferred ones when their queue entry runs */
             R2 *slot = nullptr;
             if (store) {
-                const unsigned long long bal = __ballot(listed);
-                if (listed) slot = sp + lanes_below(bal);
+                /* entries of the pairs that are not plain case B; those
+                 * pass 3 recomputes */
+                const bool need = listed && !plain;
+                const unsigned long long bal = __ballot(need);
+                if (need) slot = sp + lanes_below(bal);
                 sp += __popcll(bal);
-                if (listed) {
+                mybal = l == s - sb ? bal : mybal;
+                if (need) {
                     if (!defer_ij) slot->x = aij;
                     if (!defer_ji) slot->y = aji;
                 }
@@ -891,6 +914,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
             }
             k = k + 1 == m ? 0 : k + 1;
         }
+        if (store && l < se - sb) d.sbal[(size_t)t * TILE + sb + l] = mybal;
         __syncthreads();
         born_queue_run<R>(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
         __syncthreads();
@@ -1282,13 +1306,14 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
-    __shared__ R s_w[TILE];
+    __shared__ R s_w[TILE], s_s[TILE];
     if (!*d.store_ok) return;
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R2 *st = (const R2 *)d.dstore;
-    /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
-    const int sh = SPL ? p.split_sh[2] : 0, nit = nnz << sh;
+    /* the chunks of pass 1's split: pass 1 packed each chunk's entries from
+     * the chunk's first step (SPL is chosen from pass 1's split) */
+    const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
     QMeta nxt;
     if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
     for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
@@ -1305,7 +1330,10 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
+        const R si = ldsj<R>(d, i);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
+        /* lane l: the store mask of step sb + l (which lanes have an entry) */
+        const unsigned long long mybal = l < se - sb ? d.sbal[(size_t)t * TILE + sb + l] : 0ull;
         const R2 *sp = st + mq.start + (sb ? entries_before(mr, sb) : 0);
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
@@ -1313,26 +1341,28 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
             s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
+            s_s[pos] = ldsj<R>(d, j);
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
         }
         __syncthreads();
-        /* the packed entries of the next CHAIN_DEPTH steps are in flight
+        /* the stored entries of the next CHAIN_DEPTH steps are in flight
          * while a step is processed: sp walks ahead of k */
         const int k0 = (l + sb) % m;
         R2 win[CHAIN_DEPTH];
-        bool act[CHAIN_DEPTH];
+        bool act[CHAIN_DEPTH], sto[CHAIN_DEPTH];
 #pragma unroll
         for (int u = 0; u < CHAIN_DEPTH; u++) {
-            act[u] = false;
+            act[u] = sto[u] = false;
             if (sb + u < se) {
-                /* every lane loads (lanes without a pair read the step's
-                 * first entry, in bounds: the store holds more than the
+                /* every lane loads (lanes without an entry read the step's
+                 * first position, in bounds: the store holds more than the
                  * list needs), so no branch separates a load from its use
                  * and the loads stay in flight */
-                const bool a = (mr >> (sb + u)) & 1ull;
-                const unsigned long long bal = __ballot(a);
-                win[u] = sp[a ? lanes_below(bal) : 0];
-                act[u] = a;
+                const unsigned long long bal = readlane64(mybal, u);
+                const bool e = (bal >> l) & 1ull;
+                win[u] = sp[e ? lanes_below(bal) : 0];
+                act[u] = (mr >> (sb + u)) & 1ull;
+                sto[u] = e;
                 sp += __popcll(bal);
             }
         }
@@ -1344,17 +1374,30 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                 const int s = s0 + u;
                 if (s < se) {
                     const R2 a = win[u];
-                    const bool on = act[u];
+                    const bool on = act[u], stored = sto[u];
                     if (s + CHAIN_DEPTH < se) {
-                        const bool an = (mr >> (s + CHAIN_DEPTH)) & 1ull;
-                        const unsigned long long bal = __ballot(an);
-                        win[u] = sp[an ? lanes_below(bal) : 0];
-                        act[u] = an;
+                        const unsigned long long bal = readlane64(mybal, s + CHAIN_DEPTH - sb);
+                        const bool e = (bal >> l) & 1ull;
+                        win[u] = sp[e ? lanes_below(bal) : 0];
+                        act[u] = (mr >> (s + CHAIN_DEPTH)) & 1ull;
+                        sto[u] = e;
                         sp += __popcll(bal);
                     }
                     if (on) {
-                        const double f = (double)(a.x * wi + a.y * s_w[k]);
-                        const double gxv = f * (xi - s_x[k]), gyv = f * (yi - s_y[k]), gzv = f * (zi - s_z[k]);
+                        const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
+                        R aij = a.x, aji = a.y;
+                        if (!stored) {
+                            /* both directions plain case B (pass 1 wrote no
+                             * entry): the Taylor derivative from 1/dij^2 as
+                             * pass 1 forms it */
+                            const R r2 = (R)(dx * dx + dy * dy + dz * dz);
+                            const R dij1i = gbm<R>::rsqrt_(r2);
+                            const R dij2i = dij1i * dij1i;
+                            aij = taylor_dv<R>(s_s[k], dij2i);
+                            aji = taylor_dv<R>(si, dij2i);
+                        }
+                        const double f = (double)(aij * wi + aji * s_w[k]);
+                        const double gxv = f * dx, gyv = f * dy, gzv = f * dz;
                         fxi -= gxv; fyi -= gyv; fzi -= gzv;
                         atomicAdd(&s_fx[k], gxv);
                         atomicAdd(&s_fy[k], gyv);
@@ -1809,16 +1852,18 @@ static int default_split(const Engine &E, int pass)
     return sh;
 }
 
-/* evaluation grids until autotuned (measured at 25k atoms, 25 A): pass 1
- * fastest with one workgroup per work item, pass 2 near 64 per CU, pass 3
- * near 40 per CU */
+/* evaluation grids until autotuned (measured at 25k atoms, 25 A, where the
+ * tile-pair capacity is 2.6 times the 28,713 non-empty tile pairs): passes
+ * 1 and 3 fastest with one workgroup per work item up to 128 per CU (pass
+ * 3 from the store takes pass 1's chunks), pass 2 near 64 per CU */
 static void default_grids(Engine &E)
 {
     long long items[3];
     for (int k = 0; k < 3; k++) items[k] = std::max<long long>(1, E.tp_cap) << E.par.split_sh[k];
-    E.pass_blocks[0] = (int)std::min<long long>(items[0], MAX_LAUNCH);
+    const long long wide = std::min<long long>((long long)E.ncu * 128, MAX_LAUNCH);
+    E.pass_blocks[0] = (int)std::min<long long>(items[0], wide);
     E.pass_blocks[1] = (int)std::min<long long>(items[1], (long long)E.ncu * 64);
-    E.pass_blocks[2] = (int)std::min<long long>(items[2], (long long)E.ncu * 40);
+    E.pass_blocks[2] = (int)std::min<long long>(std::max(items[0], items[2]), wide);
 }
 
 int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_options *o,
@@ -1969,7 +2014,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.qm, (size_t)cap)) ||
         (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
-        (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
+        (rc = E.dalloc(&D.sbal, (size_t)cap * TILE)) || (rc = E.dalloc(&D.npairs, 1)) || (rc = E.dalloc(&D.bblo, 3 * (size_t)NT)) ||
         (rc = E.dalloc(&D.bbhi, 3 * (size_t)NT)) || (rc = E.dalloc(&E.x_dev, 3 * (size_t)E.natoms)) ||
         (rc = E.dalloc(&E.g_dev, 3 * (size_t)E.natoms)) || (rc = E.dalloc(&E.tmp_dev, (size_t)E.natoms)) ||
         (rc = E.dalloc(&E.ene14_dev, 14)))
@@ -2209,13 +2254,13 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
      * would exit at once) is not launched */
     const bool recompute = !engine_store_holds_list(E);
     if (E.precision == GBHIP_FP64) {
-        hipLaunchKernelGGL((sp3 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
+        hipLaunchKernelGGL((sp1 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
         if (recompute)
             hipLaunchKernelGGL((sp3 ? k_chain<double, true> : k_chain<double, false>), dim3(E.pass_blocks[2]),
                                dim3(64), 0, E.stream, E.d, E.par);
     } else {
-        hipLaunchKernelGGL((sp3 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
+        hipLaunchKernelGGL((sp1 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
         if (recompute)
             hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]),
@@ -2726,7 +2771,7 @@ static size_t static_build_bytes(size_t natom, size_t ntypes, size_t nexcl)
     add(cap * sizeof(unsigned long long)); add(cap * sizeof(int)); add(2 * TILE * sizeof(int)); /* tcol tord tbin */
     add(sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(QMeta)); /* nnz tnp tnq qm */
     add(sizeof(long long)); add(sizeof(int));                                          /* store_need store_ok */
-    add(cap * TILE * sizeof(unsigned long long)); add(cap * TILE * sizeof(unsigned long long)); /* mask excl */
+    for (int k = 0; k < 3; k++) add(cap * TILE * sizeof(unsigned long long)); /* mask excl sbal */
     add(sizeof(unsigned long long)); add(3 * NT * sizeof(double)); add(3 * NT * sizeof(double)); /* npairs bblo bbhi */
     add(3 * natom * sizeof(double)); add(3 * natom * sizeof(double)); add(natom * sizeof(double)); /* x g tmp */
     add(14 * sizeof(double));

@@ -312,8 +312,9 @@ extern "C" size_t gbhip_footprint_bytes(int natom, int md_steps)
     size_t b = 0;
     b += NS * (6 * 4 + 7 * 8 + 4 * 4 + 11 * 8);          /* per-slot constants and work arrays */
     b += NT * (4 * 4 + 8 + 2 * 24);                       /* tile arrays */
-    b += cap * (4 + 8 + 8 + 4 + 8 + 4 + 2 * TILE_ATOMS * 8); /* candidates, tile pairs (both orders, flags,
-                                                              column sets, queue), bit matrices */
+    b += cap * (4 + 8 + 8 + 4 + 8 + 4 + 3 * TILE_ATOMS * 8); /* candidates, tile pairs (both orders, flags,
+                                                              column sets, queue), bit matrices, store
+                                                              step masks */
     /* pass-3 derivative store: at most every tile pair full, 16 bytes per
      * entry (FP64), 1.25x headroom */
     b += (cap * TILE_ATOMS * TILE_ATOMS + TILE_ATOMS * TILE_ATOMS) * 20;
@@ -322,7 +323,7 @@ extern "C" size_t gbhip_footprint_bytes(int natom, int md_steps)
     b += (size_t)natom * 4 * 24 + 16 * 8 * (size_t)(md_steps > 0 ? md_steps : 0); /* dynamics state + log */
     b += (size_t)natom * 6 * 24 + 6 * 256 * 8 + 64;        /* minimiser vectors and reductions */
     b += 3 * 64 * 8;                                       /* energy partial sums */
-    b += 96 * 256;                                         /* 256-byte alignment of each array */
+    b += 97 * 256;                                         /* 256-byte alignment of each array */
     return b;
 }
 
