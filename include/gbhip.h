@@ -268,6 +268,19 @@ int gbhip_set_split(gbhip_ctx *ctx, const int *split3);
 int gbhip_get_split(gbhip_ctx *ctx, int *split3);
 
 /*
+ * Queued visits of pass 1 (mixed precision): a work item whose listed
+ * pairs fill less than `fill` of its 64-lane x steps grid is evaluated from
+ * a queue of its listed pairs, 64 at a time, instead of by the step sweep
+ * (pairs at the rim of the cutoff leave most lanes of such a sweep idle).
+ * pass must be 0; fill in [0, 2]: 0 never queues, above 1 always; the
+ * default 0.7 was the fastest at 25,000 atoms, 25 A.  The results are the
+ * same pair terms summed in another order.  The FP64 kernels keep their
+ * own queue for the logarithmic cases and ignore this.  No counterpart in
+ * the reference (an evaluation-order choice).
+ */
+int gbhip_set_queue_fill(gbhip_ctx *ctx, int pass, double fill);
+
+/*
  * Pass-3 derivative store.  Pass 1 can write, per listed pair, the two
  * derivative numerators pass 3 needs (8 bytes per pair slot in the mixed
  * variant, 16 in FP64), so that pass 3 does not recompute them.  mode 1

@@ -165,6 +165,7 @@ _sig("gbhip_batch_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER
 _sig("gbhip_get_launch", C.c_int, [vp, ip])
 _sig("gbhip_set_split", C.c_int, [vp, ip])
 _sig("gbhip_get_split", C.c_int, [vp, ip])
+_sig("gbhip_set_queue_fill", C.c_int, [vp, C.c_int, C.c_double])
 _sig("gbhip_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_batch_autotune", C.c_int, [vp, C.c_int, ip])
 _sig("gbhip_set_bonded", C.c_int, [vp, vp])
@@ -562,6 +563,11 @@ class GBContext:
         b = np.zeros(3, np.int32)
         _check(lib.gbhip_get_split(self.h, _p(b, ip)))
         return tuple(int(v) for v in b)
+
+    def set_queue_fill(self, fill, pass_=0):
+        """Pass-1 visits filled below `fill` run from a queue of their listed
+        pairs (0: never, above 1: always; mixed precision)."""
+        _check(lib.gbhip_set_queue_fill(self.h, pass_, fill))
 
     def store_stats(self):
         """(entries needed by the current list, entries allocated, in use)."""

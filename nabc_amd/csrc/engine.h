@@ -125,6 +125,11 @@ struct GBPar {
     /* 1: deterministic reductions (every visit's sums stored per tile pair
      * and added up in a fixed order instead of by atomics; no step split) */
     int det;
+    /* per pass: a visit whose listed pairs fill less than this fraction of
+     * its 64 x steps lane-steps is evaluated from a queue of its listed
+     * pairs, 64 at a time, instead of by the step sweep */
+    float qfill[3];
+    int pad_;
     double cut2, rgbmax, rgbmax1i, rgbmax2i;
     double neckscale, kappa, epsext;
 };

@@ -528,6 +528,9 @@ __device__ __forceinline__ double *det_side(const DevPtrs &d, int t, int side)
 #define BORN_NECK 1
 #define BORN_FAR 2
 #define BORN_MARGIN 0.05f
+#ifndef BORN_QFILL
+#define BORN_QFILL 0.7f /* pass 1: queue visits filled below this */
+#endif
 
 /*
  * Wavefront minimum and maximum of a float (every lane active), on keys
@@ -592,19 +595,20 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE], s_acc[TILE];
     __shared__ R s_ri[TILE], s_ri1i[TILE], s_sj[TILE], s_rb[TILE];
     __shared__ int s_nk[TILE]; /* neck index n | (n * NECK_N) << 8 */
-    __shared__ R npos[NECK_N * NECK_N], nval[NECK_N * NECK_N];
+    /* the neck tables are read through the cache: in LDS they took the
+     * space of a sixth wavefront per SIMD */
+    const R *__restrict__ npos = neckpos<R>(d);
+    const R *__restrict__ nval = neckval<R>(d);
+    /* queued visits: row sums, and a ring of listed pairs (row lane | column
+     * << 6 | store offset << 12) */
+    __shared__ double s_row[TILE];
+    __shared__ int s_q[2 * TILE];
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const bool store = *d.store_ok;
     R2 *st = (R2 *)d.dstore;
     const R rgbmax = (R)p.rgbmax, rgbmax1i = (R)p.rgbmax1i, rgbmax2i = (R)p.rgbmax2i;
     const R neckscale = (R)p.neckscale, neckcut = (R)NECK_CUT;
-    if (p.has_neck) {
-        for (int k = l; k < NECK_N * NECK_N; k += TILE) {
-            npos[k] = neckpos<R>(d)[k];
-            nval[k] = neckval<R>(d)[k];
-        }
-    }
     /* without a split (SPL false) the chunk is the whole sweep, folded at compile time */
     const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
     QMeta nxt;
@@ -645,52 +649,65 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         }
         const int kind = born_visit_kind(mr != 0ull, own, xi, yi, zi, (float)si, (float)rbi, xj, yj, zj,
                                          (float)sjj, (float)rbj, p.has_neck != 0);
+        /* the tile pair's fill, listed pairs over lane-steps (of the whole
+         * tile pair, so that every chunk of it, whatever the split, takes
+         * the same form and computes the same FP32 terms) */
+        const bool use_queue = (float)d.tnp[t] < p.qfill[0] * (float)(TILE * m);
+        s_row[l] = 0.0;
         __syncthreads();
         double acc = 0.0;
         unsigned long long mybal = 0ull; /* lane l: the store mask of step sb + l */
+        /* one pair's terms: row atom (lane li's) against column k; the
+         * column sums are kept with the opposite sign (one LDS add without
+         * a negation) and negated at the flush */
+        auto pair = [&](auto kind_c, double px, double py, double pz, R pri, R pri1i, R psi, R prb, int pni, int k,
+                        R &tij, R &aij, R &aji, bool &plain) {
+            constexpr int KIND = decltype(kind_c)::value;
+            constexpr bool NEAR = KIND == BORN_NEAR, NECK = KIND != BORN_FAR;
+            R tji = R(0);
+            tij = aij = aji = R(0);
+            plain = false;
+            const double dx = px - s_x[k], dy = py - s_y[k], dz = pz - s_z[k];
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            if (r2d <= rp2) {
+                const R r2 = (R)r2d;
+                const R dij1i = gbm<R>::rsqrt_(r2);
+                const R dij = r2 * dij1i;
+                const R rj = s_ri[k], sj = s_sj[k];
+                bool skip_ij, skip_ji, b_ij, b_ji;
+                tij = descreen_dc<R, NEAR>(pri, pri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij, b_ij,
+                                           aij);
+                tji = descreen_dc<R, NEAR>(rj, s_ri1i[k], psi, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ji,
+                                           b_ji, aji);
+                plain = b_ij && b_ji;
+                if (NECK && p.has_neck) {
+                    const int nk = s_nk[k], nij = pni * NECK_N + (nk & 255), nji = (nk >> 8) + pni;
+                    if (dij < prb + s_rb[k] + neckcut) {
+                        R dn;
+                        plain = false;
+                        if (!skip_ij) {
+                            tij += neck_d<R>(dij, dij1i, npos[nij], nval[nij], neckscale, dn);
+                            aij += dn;
+                        }
+                        if (!skip_ji) {
+                            tji += neck_d<R>(dij, dij1i, npos[nji], nval[nji], neckscale, dn);
+                            aji += dn;
+                        }
+                    }
+                }
+                atomicAdd(&s_acc[k], (double)tji);
+            }
+        };
         /* the step loop, compiled for the three visit kinds */
         auto sweep = [&](auto kind_c) {
-        constexpr int KIND = decltype(kind_c)::value;
-        constexpr bool NEAR = KIND == BORN_NEAR, NECK = KIND != BORN_FAR;
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
-            R aij = R(0), aji = R(0);
+            R aij = R(0), aji = R(0), tij;
             bool plain = false; /* both directions case B, no neck term: pass 3 recomputes */
             const bool listed = (mr >> s) & 1ull;
             if (listed) {
-                const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
-                const double r2d = dx * dx + dy * dy + dz * dz;
-                if (r2d <= rp2) {
-                    const R r2 = (R)r2d;
-                    const R dij1i = gbm<R>::rsqrt_(r2);
-                    const R dij = r2 * dij1i;
-                    const R rj = s_ri[k], sj = s_sj[k];
-                    bool skip_ij, skip_ji, b_ij, b_ji;
-                    R tij = descreen_dc<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i, skip_ij,
-                                                 b_ij, aij);
-                    R tji = descreen_dc<R, NEAR>(rj, s_ri1i[k], si, r2, dij, dij1i, rgbmax, rgbmax1i, rgbmax2i,
-                                                 skip_ji, b_ji, aji);
-                    plain = b_ij && b_ji;
-                    if (NECK && p.has_neck) {
-                        const int nk = s_nk[k], nij = niN + (nk & 255), nji = (nk >> 8) + ni;
-                        if (dij < rbi + s_rb[k] + neckcut) {
-                            R dn;
-                            plain = false;
-                            if (!skip_ij) {
-                                tij += neck_d<R>(dij, dij1i, npos[nij], nval[nij], neckscale, dn);
-                                aij += dn;
-                            }
-                            if (!skip_ji) {
-                                tji += neck_d<R>(dij, dij1i, npos[nji], nval[nji], neckscale, dn);
-                                aji += dn;
-                            }
-                        }
-                    }
-                    /* the column sums are kept with the opposite sign (one
-                     * LDS add without a negation) and negated at the flush */
-                    acc -= (double)tij;
-                    atomicAdd(&s_acc[k], (double)tji);
-                }
+                pair(kind_c, xi, yi, zi, ri, ri1i, si, rbi, ni, k, tij, aij, aji, plain);
+                acc -= (double)tij;
             }
             if (store) {
                 const bool need = listed && !plain;
@@ -707,9 +724,63 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
             k = k + 1 == m ? 0 : k + 1;
         }
         };
-        if (kind == BORN_FAR) sweep(std::integral_constant<int, BORN_FAR>{});
-        else if (kind == BORN_NECK) sweep(std::integral_constant<int, BORN_NECK>{});
-        else sweep(std::integral_constant<int, BORN_NEAR>{});
+        /* the listed pairs queued in step order and evaluated 64 at a time,
+         * one per lane (row data from its lane by cross-lane reads, row sums
+         * in LDS); every listed pair has a store entry */
+        auto queued = [&](auto kind_c) {
+            auto run = [&](int h, int n) {
+                const int c = s_q[(h + l) & (2 * TILE - 1)];
+                const int li = c & 63, k = (c >> 6) & 63, po = c >> 12;
+                const double px = __shfl(xi, li), py = __shfl(yi, li), pz = __shfl(zi, li);
+                const R pri = __shfl(ri, li), pri1i = __shfl(ri1i, li), psi = __shfl(si, li), prb = __shfl(rbi, li);
+                const int pni = __shfl(ni, li);
+                if (l < n) {
+                    R tij, aij, aji;
+                    bool plain;
+                    pair(kind_c, px, py, pz, pri, pri1i, psi, prb, pni, k, tij, aij, aji, plain);
+                    atomicAdd(&s_row[li], -(double)tij);
+                    if (store) {
+                        R2 v;
+                        v.x = aij;
+                        v.y = aji;
+                        sp[po] = v;
+                    }
+                }
+            };
+            int k = (l + sb) % m, qh = 0, qn = 0, off = 0;
+            for (int s = sb; s < se; s++) {
+                const bool listed = (mr >> s) & 1ull;
+                const unsigned long long bal = __ballot(listed);
+                if (listed) {
+                    const int lb = lanes_below(bal);
+                    s_q[(qh + qn + lb) & (2 * TILE - 1)] = l | k << 6 | (off + lb) << 12;
+                }
+                if (store) mybal = l == s - sb ? bal : mybal;
+                off += __popcll(bal);
+                qn += __popcll(bal);
+                if (qn >= TILE) {
+                    __syncthreads();
+                    run(qh, TILE);
+                    __syncthreads();
+                    qh = (qh + TILE) & (2 * TILE - 1);
+                    qn -= TILE;
+                }
+                k = k + 1 == m ? 0 : k + 1;
+            }
+            __syncthreads();
+            if (qn) run(qh, qn);
+            __syncthreads();
+            acc = s_row[l];
+        };
+        if (use_queue) {
+            if (kind == BORN_FAR) queued(std::integral_constant<int, BORN_FAR>{});
+            else if (kind == BORN_NECK) queued(std::integral_constant<int, BORN_NECK>{});
+            else queued(std::integral_constant<int, BORN_NEAR>{});
+        } else {
+            if (kind == BORN_FAR) sweep(std::integral_constant<int, BORN_FAR>{});
+            else if (kind == BORN_NECK) sweep(std::integral_constant<int, BORN_NECK>{});
+            else sweep(std::integral_constant<int, BORN_NEAR>{});
+        }
         if (store && l < se - sb) d.sbal[(size_t)t * TILE + sb + l] = mybal;
         __syncthreads();
         if (p.det) {
@@ -1974,6 +2045,10 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
     P.neckscale = o->gbneckscale;
     P.kappa = o->kappa;
     P.epsext = o->epsext;
+    P.qfill[0] = BORN_QFILL;
+    P.qfill[1] = 0.0f;
+    P.qfill[2] = 0.0f;
+    P.pad_ = 0;
 
     DevPtrs &D = E.d;
     D.ntiles = NT;
@@ -2584,6 +2659,15 @@ extern "C" int gbhip_set_split(gbhip_ctx *ctx, const int *split3)
 {
     if (!ctx || !split3) return seterr(GBHIP_EINVAL, "NULL argument");
     return set_split(ctx->E, split3);
+}
+
+extern "C" int gbhip_set_queue_fill(gbhip_ctx *ctx, int pass, double fill)
+{
+    if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (pass != 0) return seterr(GBHIP_EINVAL, "only pass 1 (pass index 0) has a queued form");
+    if (!(fill >= 0.0 && fill <= 2.0)) return seterr(GBHIP_EINVAL, "queue fill must be in [0, 2]");
+    ctx->E.par.qfill[0] = (float)fill;
+    return 0;
 }
 
 extern "C" int gbhip_get_split(gbhip_ctx *ctx, int *split3)

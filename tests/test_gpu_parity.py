@@ -316,6 +316,44 @@ def test_step_split_matches_oracle(gpu, gen_dir, precision):
     orc.close()
 
 
+@pytest.mark.parametrize("gb,cut,rgbmax", [(8, 14.0, 12.0), (5, 20.0, 20.0)])
+def test_queued_pass1_matches_oracle(gpu, gen_dir, gb, cut, rgbmax):
+    """Pass 1 of the mixed kernels evaluates work items whose listed pairs
+    fill little of their lane-steps from a queue of those pairs, 64 at a
+    time: never (fill 0), by default (0.7) and always (2), with the
+    derivative store and without, whole and split, the energies and
+    gradient match the oracle and each other (the same pair terms summed in
+    another order)."""
+    na = gpu
+    parm, arr, x = generated_system(2500, 21, gen_dir)
+    ctx, orc, _ = _compare(na, parm, arr, x, gb=gb, cut=cut, rgbmax=rgbmax, kappa=0.127315, precision=1)
+    r = orc.egb(x)
+    res = {}
+    for store in (1, 0):
+        ctx.set_derivative_store(store)
+        ctx.nblist()
+        for sp in ((1, 1, 1), (4, 2, 2)):
+            ctx.set_split(*sp)
+            for fill in (0.0, 0.7, 2.0):
+                ctx.set_queue_fill(fill)
+                e = ctx.egb()
+                for c in range(3):
+                    assert rel(e[c], r[c]) <= MIX_E, (store, sp, fill, c)
+                assert rel_rms(e[3], r[3]) <= MIX_G, (store, sp, fill)
+                res[(store, sp, fill)] = e
+    # the queued and the swept form are separately compiled FP32 sequences
+    # of the same terms (their last bits may differ), summed in FP64
+    base = res[(1, (1, 1, 1), 0.0)]
+    for key, e in res.items():
+        assert rel(e[0], base[0]) <= 1e-7, key
+        assert rel_rms(e[3], base[3]) <= 1e-6, key
+    for bad in ((0.5, 1), (-1.0, 0), (3.0, 0)):
+        with pytest.raises(na.GBHipError):
+            ctx.set_queue_fill(bad[0], pass_=bad[1])
+    ctx.close()
+    orc.close()
+
+
 def test_default_step_split_by_size(gpu, gen_dir):
     """The default step split per pass, from the tile-pair capacity: 8 in
     every pass below 128 tile pairs, pass 1 at most 4 above, passes 2 and 3
