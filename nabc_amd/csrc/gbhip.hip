@@ -652,7 +652,9 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         /* the tile pair's fill, listed pairs over lane-steps (of the whole
          * tile pair, so that every chunk of it, whatever the split, takes
          * the same form and computes the same FP32 terms) */
-        const bool use_queue = (float)d.tnp[t] < p.qfill[0] * (float)(TILE * m);
+        int npt = __popcll(mr);
+        for (int off = 32; off > 0; off >>= 1) npt += __shfl_xor(npt, off);
+        const bool use_queue = (float)npt < p.qfill[0] * (float)(TILE * m);
         s_row[l] = 0.0;
         __syncthreads();
         double acc = 0.0;
