@@ -51,7 +51,7 @@ int seterr(int code, const char *fmt, ...);
 /* per queue position: the tile pair's oriented tiles, index, non-empty
  * columns and derivative-store offset (one scalar load per visit) */
 struct QMeta {
-    int a, b, t, pad;
+    int a, b, t, mol; /* lane tile, column tile, tile pair, molecule */
     unsigned long long cols;
     long long start;
 };

@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
             mq.a = d.tpo[t].x;
             mq.b = d.tpo[t].y;
             mq.t = t;
-            mq.pad = 0;
+            mq.mol = d.tile_mol[mq.a];
             mq.cols = d.tcol[t];
             mq.start = 0;
             d.qm[q] = mq;
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
-        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
+        const double rp2 = d.rgbpsmax2[mq.mol];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const int ni = d.neck[i], niN = ni * NECK_N;
@@ -885,7 +885,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         if (sb == se) continue;
         const int i = mq.a * TILE + l, j = mq.b * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
-        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
+        const double rp2 = d.rgbpsmax2[mq.mol];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         const bool own = (cols >> l) & 1ull;
@@ -1170,7 +1170,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         if (sb == se) continue;
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
-        const int mol = d.slot_mol[i];
+        const int mol = mq.mol;
         if (mol != cur_mol) {
             if (cur_mol >= 0 && !p.det) {
                 for (int off = 32; off > 0; off >>= 1) {
@@ -1539,7 +1539,7 @@ __global__ __launch_bounds__(64) void k_chain(DevPtrs d, GBPar p)
         const int2 ab = make_int2(mq.a, mq.b);
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
-        const double rp2 = d.rgbpsmax2[d.slot_mol[i]];
+        const double rp2 = d.rgbpsmax2[mq.mol];
         const R ri = ldri0<R>(d, i), si = ldsj<R>(d, i), rbi = ldrb<R>(d, i);
         const R ri1i = R(1) / ri;
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
