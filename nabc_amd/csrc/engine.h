@@ -85,7 +85,6 @@ struct DevPtrs {
     int *tbin;            /* counting sort by column count: 64 counts, 64 offsets */
     int *nnz;             /* number of non-empty tile pairs */
     int *tnp;             /* listed pairs per tile pair */
-    int *tnq;             /* the same in processing order */
     QMeta *qm;            /* per queue position, see QMeta */
     long long *store_need; /* entries the current list needs */
     int *store_ok;        /* 1 when the store holds the current list */

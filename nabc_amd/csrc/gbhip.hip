@@ -338,6 +338,7 @@ __global__ __launch_bounds__(64) void k_binscan(DevPtrs d)
             off += d.tbin[b];
         }
         *d.nnz = off;
+        *d.store_need = 0; /* k_binscatter reserves the store regions from here */
     }
 }
 
@@ -357,54 +358,41 @@ __global__ __launch_bounds__(256) void k_binscatter(DevPtrs d)
         if (threadIdx.x < TILE && h[threadIdx.x])
             base[threadIdx.x] = atomicAdd(&d.tbin[TILE + threadIdx.x], h[threadIdx.x]);
         __syncthreads();
+        /* the tile pair's derivative-store region (one entry per listed
+         * pair): a wavefront's tile pairs take consecutive regions reserved
+         * by one atomic; any disjoint placement serves the passes */
+        const long long np = m ? d.tnp[t] : 0;
+        long long incl = np;
+        for (int off = 1; off < 64; off <<= 1) {
+            const long long v = __shfl_up(incl, off);
+            if ((threadIdx.x & 63) >= off) incl += v;
+        }
+        const long long tot = __shfl(incl, 63);
+        long long rbase = 0;
+        if ((threadIdx.x & 63) == 63 && tot)
+            rbase = (long long)atomicAdd((unsigned long long *)d.store_need, (unsigned long long)tot);
+        rbase = __shfl(rbase, 63);
         if (m) {
             const int q = base[TILE - m] + r;
             d.tord[q] = t;
-            d.tnq[q] = d.tnp[t];
             QMeta mq;
             mq.a = d.tpo[t].x;
             mq.b = d.tpo[t].y;
             mq.t = t;
             mq.mol = d.tile_mol[mq.a];
             mq.cols = d.tcol[t];
-            mq.start = 0;
+            mq.start = rbase + incl - np;
             d.qm[q] = mq;
         }
         __syncthreads();
     }
 }
 
-/*
- * Derivative-store offsets: exclusive prefix sum of the listed-pair counts
- * in processing order (one block), the store size the list needs, and the
- * flag saying whether the store allocated holds it.
- */
-__global__ __launch_bounds__(1024) void k_qscan(DevPtrs d)
+/* whether the store allocated holds the list just built (strictly less:
+ * pass 3 may read one entry past a region's last) */
+__global__ __launch_bounds__(64) void k_storeok(DevPtrs d)
 {
-    __shared__ long long s_part[1024];
-    const int n = *d.nnz, t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = min(n, t * per), e = min(n, b + per);
-    long long sum = 0;
-    for (int q = b; q < e; q++) sum += d.tnq[q];
-    s_part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const long long v = t >= off ? s_part[t - off] : 0;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
-    }
-    long long run = s_part[t] - sum;
-    for (int q = b; q < e; q++) {
-        d.qm[q].start = run;
-        run += d.tnq[q];
-    }
-    if (t == 1023) {
-        *d.store_need = s_part[1023];
-        /* strictly less: pass 3 may read one entry past a list's last */
-        *d.store_ok = d.dstore != nullptr && s_part[1023] < d.store_cap;
-    }
+    if (threadIdx.x == 0) *d.store_ok = d.dstore != nullptr && *d.store_need < d.store_cap;
 }
 
 __global__ __launch_bounds__(64) void k_setflag(int *flag, int v)
@@ -2087,7 +2075,7 @@ int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_op
         (rc = E.dalloc(&D.tpo, (size_t)cap)) || (rc = E.dalloc(&D.tflip, (size_t)cap)) ||
         (rc = E.dalloc(&D.tcol, (size_t)cap)) || (rc = E.dalloc(&D.tord, (size_t)cap)) ||
         (rc = E.dalloc(&D.tbin, 2 * TILE)) || (rc = E.dalloc(&D.nnz, 1)) ||
-        (rc = E.dalloc(&D.tnp, (size_t)cap)) || (rc = E.dalloc(&D.tnq, (size_t)cap)) ||
+        (rc = E.dalloc(&D.tnp, (size_t)cap)) ||
         (rc = E.dalloc(&D.qm, (size_t)cap)) ||
         (rc = E.dalloc(&D.store_need, 1)) || (rc = E.dalloc(&D.store_ok, 1)) ||
         (rc = E.dalloc(&D.mask, (size_t)cap * TILE)) || (rc = E.dalloc(&D.excl, (size_t)cap * TILE)) ||
@@ -2131,7 +2119,7 @@ int engine_gather(Engine &E, const double *x_dev)
     return 0;
 }
 
-/* the store is large enough for every tile pair full (k_qscan's test,
+/* the store is large enough for every tile pair full (k_storeok's test,
  * need < cap, then holds for any list) */
 static bool store_covers_all(const Engine &E)
 {
@@ -2152,7 +2140,7 @@ int engine_nblist(Engine &E)
     hipLaunchKernelGGL(k_bincount, dim3(nsb), dim3(256), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_binscan, dim3(1), dim3(64), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_binscatter, dim3(nsb), dim3(256), 0, E.stream, E.d);
-    hipLaunchKernelGGL(k_qscan, dim3(1), dim3(1024), 0, E.stream, E.d);
+    hipLaunchKernelGGL(k_storeok, dim3(1), dim3(64), 0, E.stream, E.d);
     if (E.natoms > 0)
         hipLaunchKernelGGL(k_excl, dim3((E.natoms + 255) / 256), dim3(256), 0, E.stream, E.d);
     hipLaunchKernelGGL(k_steporder, dim3(E.pair_blocks), dim3(64), 0, E.stream, E.d);
@@ -2160,7 +2148,7 @@ int engine_nblist(Engine &E)
     LAUNCHCHK();
     E.have_list = true;
     /* a store sized for every tile pair full holds any list; otherwise the
-     * host does not know until a synchronous build checks (k_qscan has
+     * host does not know until a synchronous build checks (k_storeok has
      * set the device flag either way) */
     E.store_holds_list = store_covers_all(E);
     return 0;
@@ -2252,7 +2240,7 @@ int engine_store_reserve(Engine &E, bool upper)
     int rc = engine_d2h(E, &need, E.d.store_need, sizeof(need));
     if (rc) return rc;
     rc = store_reserve(E, upper, need);
-    /* the same test k_qscan (or k_setflag after a regrowth) applied on the
+    /* the same test k_storeok (or k_setflag after a regrowth) applied on the
      * device */
     E.store_holds_list = E.d.dstore && need < E.d.store_cap;
     return rc;
@@ -2855,7 +2843,7 @@ static size_t static_build_bytes(size_t natom, size_t ntypes, size_t nexcl)
     add(cap * sizeof(int)); add(NT * sizeof(int)); add(NT * sizeof(int)); add(sizeof(int)); /* cand rowcnt rowstart ntp */
     add(cap * sizeof(int2)); add(cap * sizeof(int2)); add(cap * sizeof(int));          /* tp tpo tflip */
     add(cap * sizeof(unsigned long long)); add(cap * sizeof(int)); add(2 * TILE * sizeof(int)); /* tcol tord tbin */
-    add(sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(QMeta)); /* nnz tnp tnq qm */
+    add(sizeof(int)); add(cap * sizeof(int)); add(cap * sizeof(QMeta)); /* nnz tnp qm */
     add(sizeof(long long)); add(sizeof(int));                                          /* store_need store_ok */
     for (int k = 0; k < 3; k++) add(cap * TILE * sizeof(unsigned long long)); /* mask excl sbal */
     add(sizeof(unsigned long long)); add(3 * NT * sizeof(double)); add(3 * NT * sizeof(double)); /* npairs bblo bbhi */
