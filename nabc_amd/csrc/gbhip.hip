@@ -1372,9 +1372,11 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
     const R2 *st = (const R2 *)d.dstore;
-    /* the chunks of pass 1's split: pass 1 packed each chunk's entries from
-     * the chunk's first step (SPL is chosen from pass 1's split) */
-    const int sh = SPL ? p.split_sh[0] : 0, nit = nnz << sh;
+    /* pass 1 packed the entries of each of its chunks (split 1 << sh1)
+     * from that chunk's first step; this pass's chunks (split 1 << sh) may
+     * be coarser or finer */
+    const int sh = SPL ? p.split_sh[2] : 0, nit = nnz << sh;
+    const int sh1 = p.split_sh[0];
     QMeta nxt;
     if (snake_item(0) < nit) nxt = d.qm[snake_item(0) >> sh];
     for (int r = 0, q = snake_item(0); q < nit; q = snake_item(++r)) {
@@ -1395,7 +1397,24 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         /* lane l: the store mask of step sb + l (which lanes have an entry) */
         const unsigned long long mybal = l < se - sb ? d.sbal[(size_t)t * TILE + sb + l] : 0ull;
-        const R2 *sp = st + mq.start + (sb ? entries_before(mr, sb) : 0);
+        /* pass 1's chunk c1 holding step sb starts at step p1s; its entries
+         * start at the listed pairs before p1s, and those of steps [p1s, sb)
+         * precede sb's; nb1: the next pass-1 chunk's first step */
+        int c1 = 0, nb1 = m;
+        /* c1: the last pass-1 chunk starting at or before step ps (chunks
+         * of a split larger than m can be empty and share a start) */
+        auto chunk_at = [&](int ps) {
+            while (c1 + 1 < (1 << sh1) && ((m * (c1 + 1)) >> sh1) <= ps) c1++;
+            nb1 = c1 + 1 < (1 << sh1) ? (m * (c1 + 1)) >> sh1 : m;
+        };
+        chunk_at(sb);
+        const int p1s = (m * c1) >> sh1;
+        const R2 *sp = st + mq.start;
+        if (p1s > 0 || sb > 0) {
+            int skip = l < sb - p1s ? __popcll(d.sbal[(size_t)t * TILE + p1s + l]) : 0;
+            for (int off = 32; off > 0; off >>= 1) skip += __shfl_xor(skip, off);
+            sp += (p1s ? entries_before(mr, p1s) : 0) + skip;
+        }
         const bool own = (cols >> l) & 1ull;
         const int pos = __popcll(cols & ((1ull << l) - 1ull));
         __syncthreads();
@@ -1419,6 +1438,10 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                  * first position, in bounds: the store holds more than the
                  * list needs), so no branch separates a load from its use
                  * and the loads stay in flight */
+                if (sb + u == nb1) {
+                    sp = st + mq.start + entries_before(mr, nb1);
+                    chunk_at(nb1);
+                }
                 const unsigned long long bal = readlane64(mybal, u);
                 const bool e = (bal >> l) & 1ull;
                 win[u] = sp[e ? lanes_below(bal) : 0];
@@ -1437,6 +1460,10 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                     const R2 a = win[u];
                     const bool on = act[u], stored = sto[u];
                     if (s + CHAIN_DEPTH < se) {
+                        if (s + CHAIN_DEPTH == nb1) {
+                            sp = st + mq.start + entries_before(mr, nb1);
+                            chunk_at(nb1);
+                        }
                         const unsigned long long bal = readlane64(mybal, s + CHAIN_DEPTH - sb);
                         const bool e = (bal >> l) & 1ull;
                         win[u] = sp[e ? lanes_below(bal) : 0];
@@ -1915,8 +1942,8 @@ static int default_split(const Engine &E, int pass)
 
 /* evaluation grids until autotuned (measured at 25k atoms, 25 A, where the
  * tile-pair capacity is 2.6 times the 28,713 non-empty tile pairs): passes
- * 1 and 3 fastest with one workgroup per work item up to 128 per CU (pass
- * 3 from the store takes pass 1's chunks), pass 2 near 64 per CU */
+ * 1 and 3 fastest with one workgroup per work item up to 128 per CU, pass
+ * 2 near 64 per CU */
 static void default_grids(Engine &E)
 {
     long long items[3];
@@ -1924,7 +1951,7 @@ static void default_grids(Engine &E)
     const long long wide = std::min<long long>((long long)E.ncu * 128, MAX_LAUNCH);
     E.pass_blocks[0] = (int)std::min<long long>(items[0], wide);
     E.pass_blocks[1] = (int)std::min<long long>(items[1], (long long)E.ncu * 64);
-    E.pass_blocks[2] = (int)std::min<long long>(std::max(items[0], items[2]), wide);
+    E.pass_blocks[2] = (int)std::min<long long>(items[2], wide);
 }
 
 int engine_build(Engine &E, int nmol, const gbhip_topology *tops, const gbhip_options *o,
@@ -2319,13 +2346,13 @@ int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
      * would exit at once) is not launched */
     const bool recompute = !engine_store_holds_list(E);
     if (E.precision == GBHIP_FP64) {
-        hipLaunchKernelGGL((sp1 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
+        hipLaunchKernelGGL((sp3 ? k_chain_st<double, true> : k_chain_st<double, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
         if (recompute)
             hipLaunchKernelGGL((sp3 ? k_chain<double, true> : k_chain<double, false>), dim3(E.pass_blocks[2]),
                                dim3(64), 0, E.stream, E.d, E.par);
     } else {
-        hipLaunchKernelGGL((sp1 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
+        hipLaunchKernelGGL((sp3 ? k_chain_st<float, true> : k_chain_st<float, false>), dim3(E.pass_blocks[2]),
                            dim3(64), 0, E.stream, E.d, E.par);
         if (recompute)
             hipLaunchKernelGGL((sp3 ? k_chain<float, true> : k_chain<float, false>), dim3(E.pass_blocks[2]),
