@@ -1200,8 +1200,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
         /* the column sweep, instantiated for the LJ table in LDS and in
          * global memory and chosen per visit (a per-pair choice compiles to
          * one generic-address load, slower than either) */
-        auto sweep = [&](auto in_lds) {
+        auto sweep = [&](auto in_lds, auto scr) {
         constexpr bool TAB = decltype(in_lds)::value;
+        constexpr bool SCR = decltype(scr)::value; /* Debye-Hueckel screening (kappa != 0) */
         auto lj = [&](int kk) -> double2 {
             if constexpr (TAB) return s_tab[kk - lj0];
             else return gtab[kk];
@@ -1226,7 +1227,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     const bool act = (u == 0 || two) && ((mr >> b) & 1ull);
                     const bool coul = act && !((e >> b) & 1ull);
                     pair_terms<R>(P[u], act, coul, xi, yi, zi, qi, ri, ci, s_x[kq], s_y[kq], s_z[kq], s_q[kq],
-                                  s_r[kq], s_ir[kq], lj(row + s_lj[kq]), epsinv, kks, screened);
+                                  s_r[kq], s_ir[kq], lj(row + s_lj[kq]), epsinv, kks, SCR);
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
@@ -1256,7 +1257,7 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
                     const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
                     const R fgbi = gbm<R>::rsqrt_(r2 + rb2 * efac);
                     R expmkf, dielfac, fgbk;
-                    if (screened) {
+                    if (SCR) {
                         fgbk = kks * gbm<R>::rcp(fgbi);
                         expmkf = gbm<R>::exp_(fgbk) * epsinv;
                     } else {
@@ -1300,8 +1301,14 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
             }
         }
         };
-        if (tab_lds) sweep(std::true_type{});
-        else sweep(std::false_type{});
+        /* instantiated per LJ table location and per screening, so that
+         * neither is a per-pair select */
+        auto sweep_scr = [&](auto in_lds) {
+            if (screened) sweep(in_lds, std::true_type{});
+            else sweep(in_lds, std::false_type{});
+        };
+        if (tab_lds) sweep_scr(std::true_type{});
+        else sweep_scr(std::false_type{});
         __syncthreads();
         if (p.det) {
             /* the visit's energies per tile pair, its sums per side */
