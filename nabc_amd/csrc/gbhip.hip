@@ -816,7 +816,7 @@ template <typename R> struct BornTiles {
 };
 
 /* the queued directions at positions [h, h + n) of the ring, one per lane */
-template <typename R>
+template <typename R, bool NEAR = true, bool NECK = true>
 __device__ __forceinline__ void born_queue_run(BornQueue<R> &Q, int h, int n, const BornTiles<R> &T, const R *npos,
                                                const R *nval, double *s_row, double *s_col, const GBPar &p)
 {
@@ -832,8 +832,8 @@ __device__ __forceinline__ void born_queue_run(BornQueue<R> &Q, int h, int n, co
     const R r2 = Q.r2[e], dij1i = Q.dij1i[e], dij = r2 * dij1i;
     bool skip;
     R dv;
-    R v = descreen_d<R>(ri, ri1i, sj, r2, dij, dij1i, (R)p.rgbmax, (R)p.rgbmax1i, (R)p.rgbmax2i, skip, dv);
-    if (p.has_neck && !skip && dij < T.r_rb[lane] + T.rb[col] + (R)NECK_CUT) {
+    R v = descreen_d<R, NEAR>(ri, ri1i, sj, r2, dij, dij1i, (R)p.rgbmax, (R)p.rgbmax1i, (R)p.rgbmax2i, skip, dv);
+    if (NECK && p.has_neck && !skip && dij < T.r_rb[lane] + T.rb[col] + (R)NECK_CUT) {
         const int nidx = ji ? T.nk[col] * NECK_N + T.r_nk[lane] : T.r_nk[lane] * NECK_N + T.nk[col];
         R dn;
         v += neck_d<R>(dij, dij1i, npos[nidx], nval[nidx], (R)p.neckscale, dn);
@@ -886,17 +886,28 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
         T.r_sj[l] = si; T.r_rb[l] = rbi;
         T.r_nk[l] = d.neck[i];
         s_row[l] = 0.0;
+        double xj = 0.0, yj = 0.0, zj = 0.0;
+        R sjj = R(0), rbj = R(0);
         if (own) {
-            s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
+            xj = d.xs[j]; yj = d.ys[j]; zj = d.zs[j];
+            s_x[pos] = xj; s_y[pos] = yj; s_z[pos] = zj;
             const R rj = ldri0<R>(d, j);
+            sjj = ldsj<R>(d, j); rbj = ldrb<R>(d, j);
             T.ri[pos] = rj; T.ri1i[pos] = sizeof(R) == sizeof(double) ? gbm<R>::rcp(rj) : R(1) / rj;
-            T.sj[pos] = ldsj<R>(d, j); T.rb[pos] = ldrb<R>(d, j);
+            T.sj[pos] = sjj; T.rb[pos] = rbj;
             T.nk[pos] = d.neck[j];
             s_acc[pos] = 0.0;
         }
+        /* the visit kinds of k_born: no near cases and/or no neck term where
+         * the pair distances rule them out */
+        const int kind = born_visit_kind(mr != 0ull, own, xi, yi, zi, (float)si, (float)rbi, xj, yj, zj,
+                                         (float)sjj, (float)rbj, p.has_neck != 0);
         __syncthreads();
         double acc = 0.0;
         unsigned long long mybal = 0ull; /* lane l: the store mask of step sb + l */
+        auto sweep = [&](auto kind_c) {
+        constexpr int KIND = decltype(kind_c)::value;
+        constexpr bool NEAR = KIND == BORN_NEAR, NECK = KIND != BORN_FAR;
         int qh = 0, qn = 0; /* ring head and length, the same in every lane */
         int k = (l + sb) % m;
         for (int s = sb; s < se; s++) {
@@ -911,9 +922,9 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
                     dij1i = gbm<R>::rsqrt_(r2);
                     const R dij = r2 * dij1i;
                     const R sj = T.sj[k];
-                    const DsCase cij = ds_case<R>(ri, sj, dij, rgbmax);
-                    const DsCase cji = ds_case<R>(T.ri[k], si, dij, rgbmax);
-                    const bool neck = p.has_neck && dij < rbi + T.rb[k] + (R)NECK_CUT;
+                    const DsCase cij = ds_case<R, NEAR>(ri, sj, dij, rgbmax);
+                    const DsCase cji = ds_case<R, NEAR>(T.ri[k], si, dij, rgbmax);
+                    const bool neck = NECK && p.has_neck && dij < rbi + T.rb[k] + (R)NECK_CUT;
                     defer_ij = !cij.skip && (cij.a || cij.c || cij.d || cij.e || neck);
                     defer_ji = !cji.skip && (cji.a || cji.c || cji.d || cji.e || neck);
                     plain = cij.b && cji.b && !defer_ij && !defer_ji;
@@ -967,7 +978,7 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
                 qn += __popcll(bal);
                 if (qn >= TILE) {
                     __syncthreads();
-                    born_queue_run<R>(Q, qh, TILE, T, npos, nval, s_row, s_acc, p);
+                    born_queue_run<R, NEAR, NECK>(Q, qh, TILE, T, npos, nval, s_row, s_acc, p);
                     __syncthreads();
                     qh = (qh + TILE) & (BQ_CAP - 1);
                     qn -= TILE;
@@ -975,10 +986,14 @@ __global__ __launch_bounds__(64) void k_born_q(DevPtrs d, GBPar p)
             }
             k = k + 1 == m ? 0 : k + 1;
         }
+        __syncthreads();
+        born_queue_run<R, NEAR, NECK>(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
+        __syncthreads();
+        };
+        if (kind == BORN_FAR) sweep(std::integral_constant<int, BORN_FAR>{});
+        else if (kind == BORN_NECK) sweep(std::integral_constant<int, BORN_NECK>{});
+        else sweep(std::integral_constant<int, BORN_NEAR>{});
         if (store && l < se - sb) d.sbal[(size_t)t * TILE + sb + l] = mybal;
-        __syncthreads();
-        born_queue_run<R>(Q, qh, qn, T, npos, nval, s_row, s_acc, p);
-        __syncthreads();
         acc += s_row[l];
         if (p.det) {
             const int fl = d.tflip[t];
