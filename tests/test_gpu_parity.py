@@ -354,6 +354,58 @@ def test_queued_pass1_matches_oracle(gpu, gen_dir, gb, cut, rgbmax):
     orc.close()
 
 
+def _two_clusters(n_each, gap, seed):
+    """Two clusters of n_each atoms (random in spheres of 5 A, no two atoms
+    closer than 1.6 A), the second shifted along x so that the closest
+    pair across them is `gap` A apart."""
+    rng = np.random.default_rng(seed)
+
+    def cluster():
+        pts = []
+        while len(pts) < n_each:
+            p = rng.uniform(-5.0, 5.0, 3)
+            if np.linalg.norm(p) <= 5.0 and all(np.linalg.norm(p - q) >= 1.6 for q in pts):
+                pts.append(p)
+        return np.array(pts)
+
+    a, b = cluster(), cluster()
+    # shift b along x until the closest cross pair is at `gap`
+    lo, hi = 0.0, 40.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        bb = b + np.array([mid, 0.0, 0.0])
+        dm = np.sqrt(((a[:, None, :] - bb[None, :, :]) ** 2).sum(-1)).min()
+        lo, hi = (mid, hi) if dm < gap else (lo, mid)
+    return np.concatenate([a, b + np.array([hi, 0.0, 0.0])]).ravel()
+
+
+@pytest.mark.parametrize("gb,rgbmax", [(8, 999.0), (8, 12.0), (5, 999.0)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_visit_kinds_at_their_thresholds(gpu, gen_dir, gb, rgbmax, precision):
+    """Pass 1 compiles each tile pair's sweep without the near descreening
+    cases (dij <= 4 sj) and/or the neck term (dij < rbi + rbj + 6.8 A) when
+    a lower bound of its pair distances rules them out.  Two 64-atom
+    clusters (one tile each) are placed with their closest pair at gaps
+    from 3 to 12 A in quarter steps, across every threshold the
+    classification uses; every placement matches the oracle."""
+    na = gpu
+    parm, arr, _ = generated_system(128, 77, gen_dir)
+    for k, gap in enumerate(np.arange(3.0, 12.01, 0.25)):
+        x = _two_clusters(64, gap, 1000 + k)
+        ctx, orc, _ = _compare(na, parm, arr, x, gb=gb, cut=999.0, rgbmax=rgbmax, kappa=0.1,
+                               precision=precision, check_list=False)
+        if precision == 1:
+            # the same kinds in the queued form of pass 1
+            ctx.set_queue_fill(2.0)
+            e = ctx.egb()
+            r = orc.egb(x)
+            for c in range(3):
+                assert rel(e[c], r[c]) <= MIX_E, (gap, c)
+            assert rel_rms(e[3], r[3]) <= MIX_G, gap
+        ctx.close()
+        orc.close()
+
+
 def test_default_step_split_by_size(gpu, gen_dir):
     """The default step split per pass, from the tile-pair capacity: 8 in
     every pass below 128 tile pairs, pass 1 at most 4 above, passes 2 and 3
