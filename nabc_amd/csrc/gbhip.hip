@@ -465,14 +465,18 @@ template <> struct Vec2<double> { typedef double2 T; };
 template <> struct Vec2<float> { typedef float2 T; };
 
 /*
- * Derivative store: pass 1 writes, for every listed pair of every non-empty
- * tile pair, the two pass-3 derivative numerators (datmp of i by j, datmp
- * of j by i; zero when the pair is beyond rgbmax + Fsmax), and pass 3 reads
- * them back instead of recomputing the descreening case split, logarithms
- * and reciprocals.  Tile pairs are stored in processing order from
- * its queue offset; within a tile pair, step by step, the entries of the lanes
- * holding a listed pair at that step, packed in lane order (position =
- * entries of earlier steps + listed lanes below this one at this step).
+ * Derivative store: pass 1 writes, for a listed pair, the two pass-3
+ * derivative numerators (datmp of i by j, datmp of j by i; zero when the
+ * pair is beyond rgbmax + Fsmax), and pass 3 reads them back instead of
+ * recomputing the descreening case split, logarithms and reciprocals.  A
+ * pair whose two directions are both the Taylor case without a neck term
+ * gets no entry (pass 3 recomputes those few FMAs), except in a queued
+ * tile pair, where every listed pair has one.  Each tile pair owns a
+ * region as large as its listed pairs (reserved at list build); a work
+ * item packs its entries from the region's start plus the listed pairs of
+ * the steps before its first, step by step, the entries of one step in
+ * lane order (ballot and lane prefix count), and records per step the
+ * 64-bit mask of the lanes that wrote one (sbal).
  */
 __device__ __forceinline__ int lanes_below(unsigned long long bal)
 {
@@ -574,7 +578,9 @@ __device__ __forceinline__ int born_visit_kind(bool row, bool col, double xi, do
  * j (added to i's sum) and of j by i (added to j's sum).  The r^2 test
  * against (rgbmax+Fsmax)^2 and the FP64 sums are kept in both variants.
  * When the derivative store is large enough for the current list, the
- * pass-3 numerators of both directions are written as well.
+ * pass-3 numerators of both directions are written as well (see the store
+ * above).  Each visit runs one of three compiled sweeps by its kind
+ * (born_visit_kind), or, for a tile pair of low fill, the queued form.
  */
 template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_born(DevPtrs d, GBPar p)
@@ -1380,8 +1386,9 @@ __global__ __launch_bounds__(64) void k_pairs(DevPtrs d, GBPar p)
  * grad_i -= F, grad_j += F, with w = sumdeijda * dR/dsum (0 if frozen).
  * Two kernels, both launched, exactly one of which does the work, as the
  * device flag of the current list says: k_chain_st reads datmp from the
- * derivative store written by pass 1 (memory-bound, few registers, high
- * occupancy); k_chain recomputes it (the store could not hold the list).
+ * derivative store written by pass 1 where an entry exists and recomputes
+ * the Taylor-case ones; k_chain recomputes every datmp (the store could not
+ * hold the list).
  */
 template <typename R, bool SPL>
 __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
