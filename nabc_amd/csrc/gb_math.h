@@ -236,16 +236,17 @@ __device__ __forceinline__ R descreen_d(R ri, R ri1i, R sj, R r2, R dij, R dij1i
 
 /*
  * The case-B (Taylor) derivative numerator alone, for a pair direction
- * known to be in case B without a neck term: the same expression
- * descreen_dc evaluates for it, from dij2i = 1/dij^2.
+ * known to be in case B without a neck term: descreen_dc's expression
+ * tmpsd sj dij2i^2 dumbd with tmpsd = sj^2 dij2i, regrouped as
+ * sj^3 dij2i^3 dumbd(sj^2 dij2i) so that a caller can hold sj^2 and sj^3
+ * per atom and dij2i^3 per pair (dij2i = 1/dij^2).
  */
 template <typename R>
-__device__ __forceinline__ R taylor_dv(R sj, R dij2i)
+__device__ __forceinline__ R taylor_dv3(R sj2, R sj3, R dij2i, R dij6i)
 {
-    const R sj2 = sj * sj;
     const R tmpsd = sj2 * dij2i;
     const R dumbd = R(GB_TE) + tmpsd * (R(GB_TF) + tmpsd * (R(GB_TG) + tmpsd * (R(GB_TH) + tmpsd * R(GB_THH))));
-    return tmpsd * sj * dij2i * dij2i * dumbd;
+    return sj3 * dij6i * dumbd;
 }
 
 template <typename R>

@@ -1396,7 +1396,7 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
     typedef typename Vec2<R>::T R2;
     __shared__ double s_x[TILE], s_y[TILE], s_z[TILE];
     __shared__ double s_fx[TILE], s_fy[TILE], s_fz[TILE];
-    __shared__ R s_w[TILE], s_s[TILE];
+    __shared__ R s_w[TILE], s_s2[TILE], s_s3[TILE]; /* column sj^2, sj^3 */
     if (!*d.store_ok) return;
     const int l = threadIdx.x;
     const int nnz = *d.nnz;
@@ -1422,7 +1422,7 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         const int i = ab.x * TILE + l, j = ab.y * TILE + l;
         const double xi = d.xs[i], yi = d.ys[i], zi = d.zs[i];
         const R wi = (R)(d.sumdeijda[i] * d.dfac[i]);
-        const R si = ldsj<R>(d, i);
+        const R si = ldsj<R>(d, i), si2 = si * si, si3 = si2 * si;
         const unsigned long long mr = d.mask[(size_t)t * TILE + l];
         /* lane l: the store mask of step sb + l (which lanes have an entry) */
         const unsigned long long mybal = l < se - sb ? d.sbal[(size_t)t * TILE + sb + l] : 0ull;
@@ -1450,7 +1450,9 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
         if (own) {
             s_x[pos] = d.xs[j]; s_y[pos] = d.ys[j]; s_z[pos] = d.zs[j];
             s_w[pos] = (R)(d.sumdeijda[j] * d.dfac[j]);
-            s_s[pos] = ldsj<R>(d, j);
+            const R sjc = ldsj<R>(d, j);
+            s_s2[pos] = sjc * sjc;
+            s_s3[pos] = sjc * sjc * sjc;
             s_fx[pos] = 0.0; s_fy[pos] = 0.0; s_fz[pos] = 0.0;
         }
         __syncthreads();
@@ -1510,8 +1512,9 @@ __global__ __launch_bounds__(64) void k_chain_st(DevPtrs d, GBPar p)
                             const R r2 = (R)(dx * dx + dy * dy + dz * dz);
                             const R dij1i = gbm<R>::rsqrt_(r2);
                             const R dij2i = dij1i * dij1i;
-                            aij = taylor_dv<R>(s_s[k], dij2i);
-                            aji = taylor_dv<R>(si, dij2i);
+                            const R dij6i = dij2i * dij2i * dij2i;
+                            aij = taylor_dv3<R>(s_s2[k], s_s3[k], dij2i, dij6i);
+                            aji = taylor_dv3<R>(si2, si3, dij2i, dij6i);
                         }
                         const double f = (double)(aij * wi + aji * s_w[k]);
                         const double gxv = f * dx, gyv = f * dy, gzv = f * dz;
