@@ -52,7 +52,7 @@ def kernel_name(pass_, real, split):
     sp = "true" if split[k] > 1 else "false"
     if pass_ == "born" and real == "double":
         # FP64 pass 1 is the queued variant
-        return "k_born_q<%s>" % sp
+        return "k_born_q<double, %s>" % sp
     base = ("k_born", "k_pairs", "k_chain_st")[k]
     return "%s<%s, %s>" % (base, real, sp)
 
