@@ -175,6 +175,39 @@ int gbhip_egb(gbhip_ctx *ctx, double *grad, double *epol, double *evdw,
 int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const double *x, double *grad,
                      double *epol, double *evdw, double *eelec);
 
+/*
+ * Hierarchical charge partitioning: egb_hcp() (src/sff/hcp_gb.c:2166-2240)
+ * in place of egb(), as mme34 calls it when mm_options sets hcp
+ * (eff.c:2687-2694).  Atoms are grouped into residues, strands and
+ * complexes; a component whose geometric centre lies farther from an atom
+ * than its level's threshold acts on that atom through point charges:
+ *   hcp 1: one charge at the centre of charge (hcp.c:337-540),
+ *   hcp 2: two, the negative and the positive atoms apart,
+ *   hcp 4: two optimal point charges (HCPO, hcp.c:543-840; mm_options'
+ *          hcp=4 is sff.c:1335-1340's two-charge HCPO).
+ * Radii of components: egb_hcp_rbondi (hcp_gb.c:45-124), set here.  The
+ * Born radii have no rgbmax cut-off and no neck term, and no pair list is
+ * built (eff.c:2560-2561): gbhip_nblist returns 0 entries.  Every
+ * evaluation (gbhip_egb, gbhip_mme, dynamics, conjgrad) then runs the HCP
+ * kernels, in FP64 whatever the context's precision; the reference's
+ * single-precision parts (component radii, HCPO moments and charges) stay
+ * single precision.  hcp = 0 (or h == NULL) returns to the pair-list path
+ * (a list must be built again).  Single-molecule contexts only; not with
+ * the deterministic mode; not while dynamics graphs exist.
+ */
+typedef struct {
+    int hcp;              /* 0 (off), 1, 2 or 4 */
+    double h1, h2, h3;    /* residue, strand, complex thresholds (A); sff.c: 15, 50, 150 */
+    double dhcp;          /* hcp 4: neutral-component charge separation factor; sff.c: 0.25 */
+    int nres;             /* residues: atoms ipres[r] .. ipres[r+1]-1 */
+    const int *ipres;     /*   nres + 1 entries, 0-based, ipres[nres] = natom (prm->Ipres) */
+    int nstrand;          /* strands: residues ipstrand[s]-1 .. ipstrand[s+1]-2 */
+    const int *ipstrand;  /*   1-based first residues (prm->Ipstrand, prm.c:1773-1775) */
+    int ncomplex;         /* complexes: strands ipcomplex[c]-1 .. */
+    const int *ipcomplex; /*   1-based first strands (prm->Ipcomplex, prm.c:1782-1784) */
+} gbhip_hcp;
+int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h);
+
 /* Effective Born radii of the last evaluation, original atom order. */
 int gbhip_get_reff(gbhip_ctx *ctx, double *reff);
 

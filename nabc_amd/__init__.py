@@ -159,6 +159,16 @@ _sig("gbhip_valu_peak", C.c_int, [C.c_int, dp])
 _sig("gbhip_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_batch_set_derivative_store", C.c_int, [vp, C.c_int])
 _sig("gbhip_set_deterministic", C.c_int, [vp, C.c_int])
+
+
+class Hcp(C.Structure):
+    """gbhip_hcp (include/gbhip.h)."""
+    _fields_ = [("hcp", C.c_int), ("h1", C.c_double), ("h2", C.c_double), ("h3", C.c_double),
+                ("dhcp", C.c_double), ("nres", C.c_int), ("ipres", ip), ("nstrand", C.c_int),
+                ("ipstrand", ip), ("ncomplex", C.c_int), ("ipcomplex", ip)]
+
+
+_sig("gbhip_set_hcp", C.c_int, [vp, C.POINTER(Hcp)])
 _sig("gbhip_batch_set_deterministic", C.c_int, [vp, C.c_int])
 _sig("gbhip_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
 _sig("gbhip_batch_store_stats", C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip])
@@ -551,6 +561,18 @@ class GBContext:
     def set_deterministic(self, on):
         """Fixed-order reductions (bit-identical results run to run)."""
         _check(lib.gbhip_set_deterministic(self.h, int(on)))
+
+    def set_hcp(self, hcp, ipres, ipstrand=(1,), ipcomplex=(1,), h1=15.0, h2=50.0, h3=150.0, dhcp=0.25):
+        """Hierarchical charge partitioning (egb_hcp) for every later
+        evaluation: hcp 1, 2 or 4 (0 returns to the pair-list path).  ipres:
+        nres + 1 0-based residue starts ending at natom; ipstrand /
+        ipcomplex: 1-based first residues / first strands."""
+        if hcp == 0:
+            _check(lib.gbhip_set_hcp(self.h, None))
+            return
+        r, s, c = (np.ascontiguousarray(a, dtype=np.int32) for a in (ipres, ipstrand, ipcomplex))
+        o = Hcp(hcp, h1, h2, h3, dhcp, len(r) - 1, _p(r, ip), len(s), _p(s, ip), len(c), _p(c, ip))
+        _check(lib.gbhip_set_hcp(self.h, C.byref(o)))
 
     def set_split(self, born=0, pairs=0, chain=0):
         """Step chunks per tile pair for the three pair passes (1, 2, 4, 8, 16;

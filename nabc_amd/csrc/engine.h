@@ -134,6 +134,43 @@ struct GBPar {
 };
 
 /*
+ * Hierarchical charge partitioning (hcp.hip): atoms in their original
+ * order, grouped into residues, strands and complexes; a component farther
+ * from an atom than its level's threshold acts through nq point charges.
+ * All arrays are device memory of the engine (freed with it).
+ */
+struct HcpDev {
+    int on = 0;        /* 1: evaluations run the HCP kernels */
+    int nq = 1;        /* charges per component (1, or 2 for hcp 2 and 4) */
+    int opca = 0;      /* hcp 4: optimal point charges */
+    int gb = 5;
+    int natom = 0, nres = 0, nstrand = 0, ncomplex = 0;
+    float dhcp = 0.25f;
+    double h1sq = 0, h2sq = 0, h3sq = 0, kappa = 0, epsext = 78.5;
+    const int *ipres = nullptr;   /* nres + 1 first atoms */
+    const int *rfirst = nullptr;  /* nstrand + 1 first residues */
+    const int *sfirst = nullptr;  /* ncomplex + 1 first strands */
+    const double4 *atom = nullptr; /* per atom: charge, rborn - gboffset, scaled radius, rborn */
+    const double4 *obc = nullptr;  /* per atom: alpha, beta, gamma */
+    const int2 *lj = nullptr;     /* per atom: row offset into ljab, column */
+    const int *frz = nullptr;
+    const int *ex_ptr = nullptr, *ex_j = nullptr; /* excluded partners, both directions, ascending */
+    const int *inv = nullptr;     /* atom -> engine slot */
+    const double2 *ljab = nullptr;
+    const double *rc1 = nullptr, *rc2 = nullptr, *rc3 = nullptr; /* component radii */
+    double *xc1 = nullptr, *xc2 = nullptr, *xc3 = nullptr;       /* geometric centres */
+    double *qc1 = nullptr, *qc2 = nullptr, *qc3 = nullptr;       /* (x, y, z, q) per charge */
+    double *rf1 = nullptr, *rf2 = nullptr, *rf3 = nullptr;       /* component Born radii */
+    double *rsum = nullptr;  /* per residue: coordinate sums, charge-group sums (3 + 4 nq) */
+    double *ssum = nullptr;  /* per strand: the same */
+    int *scnt = nullptr;     /* per strand: atoms */
+    float *rmp = nullptr, *smp = nullptr; /* per residue / strand: single-precision moments (10) */
+    double *rw = nullptr;    /* per residue: its terms of the strand averages (6) */
+    double *sw = nullptr;    /* per strand: its averaging sums (4) */
+    double *reff = nullptr, *psi = nullptr, *sd = nullptr; /* per atom */
+};
+
+/*
  * Memory a context can live in without allocating (a serving slot,
  * gbhip_slot): a device region the engine suballocates from (256-byte
  * aligned, released as a whole when the context goes), pinned host memory
@@ -208,6 +245,8 @@ struct Engine {
     /* set when the context lives in a slot: device memory, staging and event
      * are the slot's (nothing is allocated or freed here) */
     SlotMem *slot = nullptr;
+    /* hierarchical charge partitioning in place of the pair-list passes */
+    HcpDev hcp{};
 
     ~Engine()
     {
@@ -408,6 +447,10 @@ bool engine_store_holds_list(const Engine &E);
 #define EV_GATHERED 1 /* the slots already hold x_dev's coordinates, accumulators cleared */
 #define EV_NO_POST 2  /* the caller sums the energy slots and scatters the gradient itself */
 int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags = 0);
+/* energy-slot sums and slot gradients into g_dev (may be NULL): k_post */
+int engine_post(Engine &E, double *g_dev, int nene, int nthreads);
+/* the HCP evaluation (hcp.hip), called by engine_eval when E.hcp.on */
+int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags);
 /* mme34 of E.x_dev into E.g_dev and E.ene14_dev (mm_device.hip) */
 int engine_mme(struct gbhip_ctx *c);
 

@@ -2187,6 +2187,11 @@ static bool store_covers_all(const Engine &E)
 
 int engine_nblist(Engine &E)
 {
+    /* HCP builds no pair list (eff.c:2560-2561) */
+    if (E.hcp.on) {
+        E.have_list = true;
+        return 0;
+    }
     const int NT = E.ntiles;
     int rc = engine_gather(E, E.x_dev);
     if (rc) return rc;
@@ -2295,6 +2300,7 @@ static int store_reserve(Engine &E, bool upper, long long need)
 
 int engine_store_reserve(Engine &E, bool upper)
 {
+    if (E.hcp.on) return 0; /* no pair list, no derivative store */
     long long need = 0;
     int rc = engine_d2h(E, &need, E.d.store_need, sizeof(need));
     if (rc) return rc;
@@ -2332,8 +2338,17 @@ static int set_store_mode(Engine &E, int mode)
  * caller's own kernel has done k_gather's work, EV_NO_POST when it does
  * k_post's (the fused kernels of the mme and dynamics paths).
  */
+/* the energy-slot sums and the slot gradients added into g_dev (k_post) */
+int engine_post(Engine &E, double *g_dev, int nene, int nthreads)
+{
+    hipLaunchKernelGGL(k_post, dim3((nthreads + 255) / 256), dim3(256), 0, E.stream, E.d, E.par, g_dev, nene);
+    LAUNCHCHK();
+    return 0;
+}
+
 int engine_eval(Engine &E, const double *x_dev, double *g_dev, int flags)
 {
+    if (E.hcp.on) return engine_eval_hcp(E, x_dev, g_dev, flags);
     if (!E.have_list) return seterr(GBHIP_ESTATE, "egb called before the pair list was built");
     if (!(flags & EV_GATHERED)) {
         int rc = engine_gather(E, x_dev);
@@ -2645,6 +2660,7 @@ static int set_det(Engine &E, int on)
 extern "C" int gbhip_set_deterministic(gbhip_ctx *ctx, int on)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (on && ctx->E.hcp.on) return seterr(GBHIP_EINVAL, "hcp has no deterministic mode (its forces add with atomics)");
     return set_det(ctx->E, on);
 }
 
@@ -2729,6 +2745,7 @@ extern "C" int gbhip_get_split(gbhip_ctx *ctx, int *split3)
 extern "C" int gbhip_autotune(gbhip_ctx *ctx, int reps, int *chosen3)
 {
     if (!ctx) return seterr(GBHIP_EINVAL, "NULL context");
+    if (ctx->E.hcp.on) return seterr(GBHIP_EINVAL, "an hcp context has no pair-kernel launch shapes to tune");
     return autotune(ctx->E, reps, chosen3);
 }
 
@@ -3176,6 +3193,10 @@ extern "C" int gbhip_npairs(gbhip_ctx *ctx, long long *npairs)
 {
     if (!ctx || !npairs) return seterr(GBHIP_EINVAL, "NULL argument");
     Engine &E = ctx->E;
+    if (E.hcp.on) {
+        *npairs = 0;
+        return 0;
+    }
     unsigned long long v = 0;
     const int rc = engine_d2h(E, &v, E.d.npairs, sizeof(v));
     if (rc) return rc;

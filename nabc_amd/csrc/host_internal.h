@@ -21,6 +21,7 @@ typedef struct {
     double dt, t, tautp, gamma_ln, temp0, boltz2, vlimit, genmass, tempi;
     double wcons, k4d, surften, dradius, deltar, rwater, alphanp, kappanp;
     double min_volume, wdssp;
+    double hcp_h1, hcp_h2, hcp_h3, dhcp;
     int ntpr, nscm, nsnb, nsnp, nchk, nchk2, ntwx, ntpr_md, zerov;
     int gb, gbsa, gb_debug, nr_debug, gb2_debug, gbsa_debug, e_debug;
     int dield, irattle, ips, hcp, blocksize, dim, max_set_size;

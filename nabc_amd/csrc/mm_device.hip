@@ -344,6 +344,7 @@ static std::vector<unsigned char> mme_key(const gbhip_ctx *c)
     put(&E.g_dev, sizeof(E.g_dev));
     put(&E.ene14_dev, sizeof(E.ene14_dev));
     put(&c->B, sizeof(c->B));
+    put(&E.hcp, sizeof(E.hcp));
     const bool held = engine_store_holds_list(E); /* decides whether pass 3's recompute kernel is launched */
     put(&held, sizeof(held));
     return k;

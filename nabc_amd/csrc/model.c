@@ -207,8 +207,8 @@ int nabc_model_build(nabc_model *m, PARMSTRUCT_T *prm, const int *frozen, const 
         nabc_seterr("mme_init_sff: gb=%d is not provided by this library (GB models 1, 2, 5, 7, 8 only)", o->gb);
         return -1;
     }
-    if (o->gbsa || o->hcp || o->ips || o->dim != 3 || o->irattle || o->nscm || o->wdssp != 0.0) {
-        nabc_seterr("mme_init_sff: gbsa, hcp, ips, dim=4, rattle, nscm and wdssp are not provided by this library");
+    if (o->gbsa || o->ips || o->dim != 3 || o->irattle || o->nscm || o->wdssp != 0.0) {
+        nabc_seterr("mme_init_sff: gbsa, ips, dim=4, rattle, nscm and wdssp are not provided by this library");
         return -1;
     }
     if (o->dield != 1 && o->dield != 0) {
@@ -296,6 +296,28 @@ int nabc_model_context_in(const nabc_model *m, const double *x, const nabc_opts_
         nabc_seterr("%s", gbhip_last_error());
         gbhip_ctx_destroy(ctx);
         return -1;
+    }
+    if (o->hcp) {
+        /* egb_hcp over the prmtop's residues, strands and complexes
+         * (eff.c:2687-2694; prm.c:1743-1790) */
+        gbhip_hcp h;
+        memset(&h, 0, sizeof(h));
+        h.hcp = o->hcp;
+        h.h1 = o->hcp_h1;
+        h.h2 = o->hcp_h2;
+        h.h3 = o->hcp_h3;
+        h.dhcp = o->dhcp;
+        h.nres = p->Nres;
+        h.ipres = p->Ipres;
+        h.nstrand = p->Nstrand;
+        h.ipstrand = p->Ipstrand;
+        h.ncomplex = p->Ncomplex;
+        h.ipcomplex = p->Ipcomplex;
+        if (gbhip_set_hcp(ctx, &h) != 0) {
+            nabc_seterr("%s", gbhip_last_error());
+            gbhip_ctx_destroy(ctx);
+            return -1;
+        }
     }
     *out = ctx;
     return 0;

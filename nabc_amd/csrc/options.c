@@ -55,6 +55,7 @@ void mm_reset_options(void)
     o->temp0 = 300.0; o->boltz2 = 9.93595e-4; o->vlimit = 10.0;
     o->genmass = 10.0; o->ntpr_md = 10; o->ntwx = 0; o->zerov = 0;
     o->tempi = 0.0; o->irattle = 0; o->dim = 3;
+    o->hcp_h1 = 15.0; o->hcp_h2 = 50.0; o->hcp_h3 = 150.0; o->dhcp = 0.25;
     o->gbprec = 0; o->md_seed = 71277ULL; o->cgdevice = 1;
 }
 
@@ -72,6 +73,8 @@ static const keydef_t keys[] = {
     {"cut", K_DBL, &O->cut, 0}, {"cutnp", K_DBL, &O->cutnp, 0},
     {"scnb", K_DBL, &O->scnb, 0}, {"scee", K_DBL, &O->scee, 0},
     {"ntpr", K_POSINT, &O->ntpr, 0}, {"hcp", K_SET, &O->hcp, "0124"},
+    {"hcp_h1", K_DBL, &O->hcp_h1, 0}, {"hcp_h2", K_DBL, &O->hcp_h2, 0},
+    {"hcp_h3", K_DBL, &O->hcp_h3, 0}, {"dhcp", K_DBL, &O->dhcp, 0},
     {"nscm", K_NNINT, &O->nscm, 0}, {"nsnb", K_NNINT, &O->nsnb, 0},
     {"nsnp", K_NNINT, &O->nsnp, 0}, {"ntwx", K_NNINT, &O->ntwx, 0},
     {"nchk", K_NNINT, &O->nchk, 0}, {"nchk2", K_NNINT, &O->nchk2, 0},

@@ -54,6 +54,15 @@ class MDOpts(C.Structure):
 _sig("orc_gauss", C.c_double, [C.c_ulonglong, C.c_longlong, C.c_longlong])
 _sig("orc_md", C.c_int, [vp, C.c_int, dp, dp, dp, C.POINTER(MDOpts), dp])
 
+class HcpOpts(C.Structure):
+    _fields_ = [("hcp", C.c_int), ("h1", C.c_double), ("h2", C.c_double), ("h3", C.c_double),
+                ("dhcp", C.c_double), ("nres", C.c_int), ("nstrand", C.c_int), ("ncomplex", C.c_int),
+                ("ipres", ip), ("ipstrand", ip), ("ipcomplex", ip)]
+
+
+_sig("orc_set_hcp", C.c_int, [vp, C.POINTER(HcpOpts)])
+_sig("orc_hcp_components", C.c_int, [vp, C.c_int, dp, dp, dp, dp])
+
 MMEFUNC = C.CFUNCTYPE(C.c_double, dp, dp, ip)
 MME_CB = MMEFUNC(("orc_mme_cb", lib))
 
@@ -123,6 +132,29 @@ class Oracle:
         if rc != 0:
             raise MemoryError("orc_md")
         return x, v, rows[:nsteps]
+
+    def set_hcp(self, hcp, ipres, ipstrand=(1,), ipcomplex=(1,), h1=15.0, h2=50.0, h3=150.0, dhcp=0.25):
+        """egb_hcp in place of egb (hcp 1, 2 or 4; 0 switches back).  ipres:
+        nres + 1 0-based residue starts ending at natom; ipstrand /
+        ipcomplex: 1-based first residues / strands."""
+        if hcp == 0:
+            return lib.orc_set_hcp(self.h, None)
+        self._hcp_arrays = [np.ascontiguousarray(a, dtype=np.int32) for a in (ipres, ipstrand, ipcomplex)]
+        r, s, c = self._hcp_arrays
+        o = HcpOpts(hcp, h1, h2, h3, dhcp, len(r) - 1, len(s), len(c), _p(r, ip), _p(s, ip), _p(c, ip))
+        if lib.orc_set_hcp(self.h, C.byref(o)) != 0:
+            raise ValueError("orc_set_hcp: bad hierarchy")
+        self._hcp_nq = 1 if hcp == 1 else 2
+
+    def hcp_components(self, level):
+        """(centres[n,3], charges[n,nq,4], radii[n,nq], reff[n,nq]) of level 1-3."""
+        m = lib.orc_hcp_components(self.h, level, None, None, None, None)
+        if m < 0:
+            raise ValueError("no HCP state")
+        nq = self._hcp_nq
+        xc, qc, rc, rf = np.zeros((m, 3)), np.zeros((m, nq, 4)), np.zeros((m, nq)), np.zeros((m, nq))
+        lib.orc_hcp_components(self.h, level, _p(xc), _p(qc), _p(rc), _p(rf))
+        return xc, qc, rc, rf
 
     def gbparams(self):
         n = self.n

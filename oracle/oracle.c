@@ -17,6 +17,7 @@
 #endif
 
 #include "oracle.h"
+#include "oracle_internal.h"
 
 /* GB-neck maxima table (gbneck.h:17-147), shared with the product only as
  * published data: the numbers are the same physical constants. */
@@ -24,25 +25,10 @@
 
 #define KSCALE 0.73
 
-struct orc {
-    PARMSTRUCT_T *prm;
-    int n, gb, dield;
-    double cut, rgbmax, kappa, epsext, gboffset, neckscale, fsmax;
-    double *fs, *q, *alpha, *beta, *gamma;
-    int *neck, *frozen;
-    /* pair list: per atom, lower (j<i, ascending) then upper (j>i, ascending) */
-    int *lp, *up, **list;
-    int have_list;
-    /* 1-2/1-3/1-4 exclusion lists per atom (IexclAt, sff.c:1551-1574) */
-    int **iexcl;
-    /* 1-4 list per atom (sff.c:1521-1537) */
-    int **n14;
-    double **scee14, **scnb14;
-};
 
 static int orc_threads = 0;
 void orc_set_threads(int n) { orc_threads = n; }
-static int nthreads(void)
+int orc_nthreads(void)
 {
 #ifdef _OPENMP
     return orc_threads > 0 ? orc_threads : omp_get_max_threads();
@@ -165,6 +151,7 @@ void orc_destroy(orc_t *o)
     free(o->scee14); free(o->scnb14);
     free(o->fs); free(o->q); free(o->alpha); free(o->beta); free(o->gamma);
     free(o->neck); free(o->frozen);
+    orc_hcp_free(o);
     free(o);
 }
 
@@ -198,7 +185,7 @@ long long orc_nblist(orc_t *o, const double *x)
     const double cut2 = o->cut * o->cut;
     long long tot = 0;
     int i;
-#pragma omp parallel for schedule(dynamic, 8) reduction(+ : tot) num_threads(nthreads())
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : tot) num_threads(orc_nthreads())
     for (i = 0; i < n; i++) {
         int j, cnt = 0, lo = 0;
         int *buf = (int *)malloc(sizeof(int) * (n > 1 ? n : 1));
@@ -256,13 +243,14 @@ void orc_list_copy(orc_t *o, int *flat)
 double orc_egb(orc_t *o, const double *x, double *f, double *enb, double *eel,
                double *reff_out, double *sumdeijda_out)
 {
+    if (o->hcp) return orc_egb_hcp(o, x, f, enb, eel, reff_out, sumdeijda_out);
     const int n = o->n, gb = o->gb;
     const PARMSTRUCT_T *prm = o->prm;
     const double rgbmax = o->rgbmax, gboffset = o->gboffset;
     const double rgbmax1i = 1.0 / rgbmax, rgbmax2i = rgbmax1i * rgbmax1i;
     const double rgbmaxpsmax2 = (rgbmax + o->fsmax) * (rgbmax + o->fsmax);
     const double kappa = o->kappa, diel_ext = o->epsext;
-    const int nt = nthreads();
+    const int nt = orc_nthreads();
     double *reff = (double *)calloc(n, sizeof(double));
     double *psi = (double *)calloc(n, sizeof(double));
     double *sumdeijda = (double *)calloc((size_t)n * nt, sizeof(double));
@@ -703,7 +691,8 @@ double orc_mme(orc_t *o, const double *x, double *f, int rebuild, double *ene, d
     double e[13], enb, eel, enb14, eel14, s = 0.0;
     int i;
     memset(e, 0, sizeof(e));
-    if (!o->have_list || rebuild) orc_nblist(o, x);
+    /* HCP uses no pair list (eff.c:2560-2561) */
+    if (!o->hcp && (!o->have_list || rebuild)) orc_nblist(o, x);
     memset(f, 0, sizeof(double) * 3 * n);
     e[3] = o_ebond(p->Nbonh, p->BondHAt1, p->BondHAt2, p->BondHNum, p->Rk, p->Req, x, f) +
            o_ebond(p->Mbona, p->BondAt1, p->BondAt2, p->BondNum, p->Rk, p->Req, x, f);

@@ -14,6 +14,7 @@
  *   orc_mme      mme34() assembly (eff.c:2442-2947) with ebond/eangl/ephi
  *                (eff.c:82-627) and the 1-4 nbond() (eff.c:896-1331)
  *   orc_md       md() (sff.c:2071-2366) over orc_mme
+ *   orc_set_hcp  egb_hcp() and its set-up (hcp_gb.c, hcp.c; oracle_hcp.c)
  */
 #ifndef NABC_AMD_ORACLE_H
 #define NABC_AMD_ORACLE_H
@@ -80,5 +81,35 @@ int orc_md(orc_t *o, int maxstep, double *x, double *v, const double *masses,
 
 /* threads used by the OpenMP loops (0 = library default) */
 void orc_set_threads(int n);
+
+/*
+ * Hierarchical charge partitioning (egb_hcp, hcp_gb.c:2166-2240, with the
+ * set-up of sff.c:1335-1340 and :1577-1692).  hcp as mm_options takes it:
+ * 1 = one charge per component, 2 = two (negative and positive atoms
+ * apart), 4 = two-charge HCPO (optimal point charges, hcp.c:150-840).
+ * h1, h2, h3: residue, strand and complex threshold distances; dhcp the
+ * HCPO neutral-component factor.  Residues are atoms ipres[r] ..
+ * ipres[r+1]-1 (0-based, nres+1 entries, as rdparm leaves prm->Ipres);
+ * strands and complexes start at the 1-based ipstrand[s] / ipcomplex[c]
+ * (prm.c:1743-1790).  Once set, orc_egb evaluates egb_hcp and orc_mme
+ * builds no pair list (eff.c:2560-2561, :2687-2694).  h == NULL or
+ * h->hcp == 0 switches back.  Returns 0, or -1 on a bad hierarchy.
+ */
+typedef struct {
+    int hcp;
+    double h1, h2, h3, dhcp;
+    int nres, nstrand, ncomplex;
+    const int *ipres, *ipstrand, *ipcomplex;
+} orc_hcp_opts;
+int orc_set_hcp(orc_t *o, const orc_hcp_opts *h);
+/*
+ * Component data of the last HCP evaluation, level 1 (residues), 2
+ * (strands) or 3 (complexes); nq = 1 or 2 charges per component.  Any
+ * pointer may be NULL: centres[3 n], charges[4 n nq] (x, y, z, q),
+ * radii[n nq] (egb_hcp_rbondi), reff[n nq] (component Born radii).
+ * Returns the component count, or -1.
+ */
+int orc_hcp_components(orc_t *o, int level, double *centres, double *charges, double *radii,
+                       double *reff);
 
 #endif

@@ -67,3 +67,46 @@ def oracle_pairs(orc, natom):
     key = i * natom + j
     key.sort()
     return np.stack([key // natom, key % natom], axis=1)
+
+
+def gunzip_golden(name, tmpdir):
+    """A gzipped fixture of tests/golden decompressed into tmpdir (once)."""
+    import gzip
+    import shutil
+    dst = os.path.join(str(tmpdir), name[:-3] if name.endswith(".gz") else name)
+    if not os.path.exists(dst):
+        with gzip.open(os.path.join(GOLDEN, name), "rb") as fin, open(dst, "wb") as fout:
+            shutil.copyfileobj(fin, fout)
+    return dst
+
+
+def trx_hcp_system(tmpdir):
+    """test/thcp1.c's thioredoxin (2trx.parm7 / 2trx.rst7, 1653 atoms):
+    (parm, arrays, x, hierarchy) with hierarchy = (ipres, ipstrand,
+    ipcomplex) as the prmtop gives them."""
+    import nabc_amd as na
+    top = gunzip_golden("2trx.parm7.gz", tmpdir)
+    rst = gunzip_golden("2trx.rst7.gz", tmpdir)
+    parm = na.read_parm(top)
+    arrs = na.parm_arrays(parm)
+    x, _ = na.read_rst7(rst, arrs["natom"])
+    return parm, arrs, x, parm_hierarchy(parm)
+
+
+def parm_hierarchy(parm):
+    """(ipres[nres + 1], ipstrand[nstrand], ipcomplex[ncomplex]) of a PARMSTRUCT_T."""
+    p = parm.contents
+    ipres = np.array([p.Ipres[i] for i in range(p.Nres + 1)], np.int32)
+    ips = np.array([p.Ipstrand[i] for i in range(p.Nstrand)], np.int32)
+    ipc = np.array([p.Ipcomplex[i] for i in range(p.Ncomplex)], np.int32)
+    return ipres, ips, ipc
+
+
+def layered_hierarchy(natom, res_atoms=20, strand_res=5, complex_strands=3):
+    """Residues of res_atoms atoms, strands of strand_res residues,
+    complexes of complex_strands strands (the last of each may be short)."""
+    ipres = np.array(list(range(0, natom, res_atoms)) + [natom], np.int32)
+    nres = len(ipres) - 1
+    ips = np.arange(1, nres + 1, strand_res, dtype=np.int32)
+    ipc = np.arange(1, len(ips) + 1, complex_strands, dtype=np.int32)
+    return ipres, ips, ipc

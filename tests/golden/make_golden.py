@@ -27,19 +27,30 @@ golden.json, each with its source file:line.
   asp_nm2  test/nab/asp_nm2.nab, asp_nm2.out.check:1-15: the same minimum's
            lowest 20 frequencies (modes 6-20 kept by the test script), 5
            decimals, compared at 1e-4 (Run.nmode2: dacdif -a 1.e-4).
+  thcp1    test/thcp1.c:20-46, test/2trx.log:21: hierarchical charge
+           partitioning (hcp=1, hcp_h1=15, hcp_h2=hcp_h3=999) GB of
+           thioredoxin (1653 atoms), gb=5, kappa 0.125: the ff: line of the
+           first mme() call (iteration 0, at the input coordinates, where
+           the wcons restraint energy is zero).  Its nonpolar column and
+           frms include the gbsa=1 surface term; bad, vdW, elect and
+           genBorn do not.  2trx.parm7 / 2trx.rst7 are stored gzipped.
 """
+import gzip
 import json
 import os
 import re
 import shutil
 
 REF = "/root/reference/test/nab"
+REF_TEST = "/root/reference/test"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def ff_lines(path):
+def ff_lines(path, first=None):
     out = []
     for k, line in enumerate(open(path), 1):
+        if first is not None and len(out) == first:
+            break
         if line.startswith("ff:"):
             f = line.split()
             out.append(dict(line=k, iter=int(f[1]), total=float(f[2]), bad=float(f[3]),
@@ -127,6 +138,15 @@ def main():
             rows.append(dict(line=k, mode=int(f[0]), freq=float(f[1])))
     g["asp_nm2"] = dict(source="test/nab/asp_nm2.out.check", prmtop="asp.top", pdb="asp.pdb", options=nm,
                         nmode=dict(eigp=20, ntrun=2), tolerance=1e-4, modes=rows)
+    # thcp1: hcp GB (test/thcp1.c:34-46), first ff: line of test/2trx.log
+    for name in ("2trx.parm7", "2trx.rst7"):
+        with open(os.path.join(REF_TEST, name), "rb") as fin, open(os.path.join(HERE, name + ".gz"), "wb") as raw:
+            with gzip.GzipFile(filename="", mode="wb", fileobj=raw, mtime=0) as fout:
+                shutil.copyfileobj(fin, fout)
+    g["thcp1"] = dict(source="test/2trx.log", prmtop="2trx.parm7.gz", rst7="2trx.rst7.gz",
+                      options=dict(gb=5, cut=15.0, rgbmax=15.0, kappa=0.125, epsext=78.5, diel="C",
+                                   hcp=1, hcp_h1=15.0, hcp_h2=999.0, hcp_h3=999.0, gbsa=1, wcons=5.0),
+                      ff=ff_lines(os.path.join(REF_TEST, "2trx.log"), first=1))
     with open(os.path.join(HERE, "golden.json"), "w") as fp:
         json.dump(g, fp, indent=1)
 
