@@ -168,6 +168,11 @@ struct HcpDev {
     double *rw = nullptr;    /* per residue: its terms of the strand averages (6) */
     double *sw = nullptr;    /* per strand: its averaging sums (4) */
     double *reff = nullptr, *psi = nullptr, *sd = nullptr; /* per atom */
+    double *sumi = nullptr;       /* per atom: descreening sum (added by residue range) */
+    const int *res_s = nullptr;   /* per residue: its strand */
+    const int *s_cx = nullptr;    /* per strand: its complex */
+    int ntile = 0;                /* 64-atom tiles of the work items */
+    int nchunk = 1, csize = 1;    /* residue ranges of csize residues */
 };
 
 /*

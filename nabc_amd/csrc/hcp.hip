@@ -82,7 +82,7 @@ __device__ __forceinline__ double wave_sum(double v)
  * (hcp_gb.c:139-160) */
 __device__ __forceinline__ double far_descreen(double r2, double sj)
 {
-    const double dij2i = 1.0 / r2, sj2 = sj * sj, tmpsd = sj2 * dij2i;
+    const double rs = rsqrt(r2), dij2i = rs * rs, sj2 = sj * sj, tmpsd = sj2 * dij2i;
     const double dumbo = HCP_TA + tmpsd * (HCP_TB + tmpsd * (HCP_TC + tmpsd * (HCP_TD + tmpsd * HCP_TDD)));
     return sj * tmpsd * dij2i * dumbo;
 }
@@ -91,7 +91,7 @@ __device__ __forceinline__ double far_descreen(double r2, double sj)
  * rgbmax cut-off or the neck term (hcp_gb.c:345-390) */
 __device__ __forceinline__ double near_descreen(double r2, double ri, double ri1i, double sj)
 {
-    const double dij1i = 1.0 / sqrt(r2), dij = r2 * dij1i, sj2 = sj * sj;
+    const double dij1i = rsqrt(r2), dij = r2 * dij1i, sj2 = sj * sj;
     if (dij > 4.0 * sj) {
         const double dij2i = dij1i * dij1i, tmpsd = sj2 * dij2i;
         const double dumbo = HCP_TA + tmpsd * (HCP_TB + tmpsd * (HCP_TC + tmpsd * (HCP_TD + tmpsd * HCP_TDD)));
@@ -110,7 +110,7 @@ __device__ __forceinline__ double near_descreen(double r2, double ri, double ri1
 /* numerator of dR_i/dd_ij over d_ij (hcp_gb.c:1599-1645, :1712-1764) */
 __device__ __forceinline__ double descreen_deriv(double r2, double ri, double ri1i, double sj)
 {
-    const double dij1i = 1.0 / sqrt(r2), dij2i = dij1i * dij1i, dij = r2 * dij1i, sj2 = sj * sj;
+    const double dij1i = rsqrt(r2), dij2i = dij1i * dij1i, dij = r2 * dij1i, sj2 = sj * sj;
     if (dij > 4.0 * sj) {
         const double tmpsd = sj2 * dij2i;
         const double dumbo = HCP_TE + tmpsd * (HCP_TF + tmpsd * (HCP_TG + tmpsd * (HCP_TH + tmpsd * HCP_THH)));
@@ -131,15 +131,25 @@ __device__ __forceinline__ double descreen_deriv(double r2, double ri, double ri
 
 /* GB (+ Coulomb) of atom i with one charge qj of Born radius rj at r^2:
  * energies, i's sumdeijda term; returns the pair force factor
- * (hcp_gb.c:1369-1436, :1473-1520) */
+ * (hcp_gb.c:1369-1436, :1473-1520).  f_GB from one reciprocal square root
+ * (1/f and f = (r^2 + RiRj e) / f), the screening exponential only with
+ * salt (KAPPA), inv_eps = 1 / epsext. */
+template <bool KAPPA>
 __device__ __forceinline__ double gb_pair(double qi, double ri, double qj, double rj, double r2, double kappa,
-                                          double epsext, double &epol, double &elec, double &sd, bool coulomb)
+                                          double inv_eps, double &epol, double &elec, double &sd, bool coulomb)
 {
     const double qiqj = qi * qj, rb2 = ri * rj;
     const double efac = exp(-r2 / (4.0 * rb2));
-    const double fgbi = 1.0 / sqrt(r2 + rb2 * efac);
-    const double fgbk = -kappa * HCP_KSCALE / fgbi;
-    const double expmkf = exp(fgbk) / epsext, dielfac = 1.0 - expmkf;
+    const double fsq = r2 + rb2 * efac;
+    const double fgbi = rsqrt(fsq);
+    double expmkf, fgbk = 0.0;
+    if (KAPPA) {
+        fgbk = -kappa * HCP_KSCALE * (fsq * fgbi);
+        expmkf = exp(fgbk) * inv_eps;
+    } else {
+        expmkf = inv_eps;
+    }
+    const double dielfac = 1.0 - expmkf;
     epol += -qiqj * dielfac * fgbi;
     const double temp4 = fgbi * fgbi * fgbi;
     const double temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
@@ -147,7 +157,7 @@ __device__ __forceinline__ double gb_pair(double qi, double ri, double qj, doubl
     const double temp5 = 0.5 * efac * temp6 * (rb2 + 0.25 * r2);
     sd += ri * temp5;
     if (coulomb) {
-        const double rinv = 1. / sqrt(r2), r2inv = rinv * rinv, eel = qiqj * rinv;
+        const double rinv = rsqrt(r2), r2inv = rinv * rinv, eel = qiqj * rinv;
         elec += eel;
         de -= eel * r2inv;
     }
@@ -371,6 +381,7 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_res(HcpDev H, const double 
     for (int a = a0; a < a1; a++) {
         const double xa = x[3 * (size_t)a], ya = x[3 * (size_t)a + 1], za = x[3 * (size_t)a + 2];
         const double q = H.atom[a].x;
+        H.sumi[a] = 0.0; /* the descreening sums k_hcp_born adds up */
         s[0] += xa;
         s[1] += ya;
         s[2] += za;
@@ -465,33 +476,45 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_up(HcpDev H)
 /* ------------------------------------------------- the hierarchy walk */
 
 /*
- * One atom per lane.  far(level, comp) runs for lanes to which component
- * comp of that level (0 residue, 1 strand, 2 complex) is far; member(a)
- * runs for every lane at each atom a of a residue that is near to some
- * lane, with `on` telling whether it is near to this one (and a != i).
- * All lanes reach every member() call together (the loops are uniform),
- * so member() may use cross-lane operations.
+ * One work item: the 64 atoms of a tile (one per lane) against the residues
+ * r0 .. r1-1 of a residue range (which may span strands and complexes).
+ * far(level, comp) runs for lanes to which component comp of that level
+ * (0 residue, 1 strand, 2 complex) is far -- a far strand or complex by
+ * the range holding its first residue only, so each atom meets it once;
+ * member(a, on) runs for every lane at each atom a of a residue that is
+ * near to some lane, `on` telling whether it is near to this one (and
+ * a != i) -- or, given as near_res(r, in1), once per such residue.  The
+ * loops are wavefront-uniform: all lanes reach every member() or
+ * near_res() call together, so these may use cross-lane operations and
+ * the workgroup barrier (one wavefront per workgroup).
  */
-template <typename Far, typename Member>
-__device__ __forceinline__ void hcp_walk(const HcpDev &H, bool valid, int i, double xi, double yi, double zi, Far far,
-                                         Member member)
+template <typename Far, typename NearRes>
+__device__ __forceinline__ void hcp_walk_res(const HcpDev &H, int r0, int r1, bool valid, double xi, double yi,
+                                             double zi, Far far, NearRes near_res)
 {
-    for (int c = 0; c < H.ncomplex; c++) {
+    int r = r0;
+    while (r < r1) {
+        const int s = H.res_s[r], c = H.s_cx[s];
+        const int sr0 = H.rfirst[s], sr1 = H.rfirst[s + 1];
+        const int cr0 = H.rfirst[H.sfirst[c]], cr1 = H.rfirst[H.sfirst[c + 1]];
         const bool f3 = valid && hcp_d2(xi, yi, zi, H.xc3 + 3 * c) > H.h3sq;
-        if (wave_any(f3)) {
+        if (r == cr0 && wave_any(f3)) {
             if (f3) far(2, c);
         }
         const bool in3 = valid && !f3;
-        if (!wave_any(in3)) continue;
-        for (int s = H.sfirst[c]; s < H.sfirst[c + 1]; s++) {
-            const double ds = hcp_d2(xi, yi, zi, H.xc2 + 3 * s);
-            const bool f2 = in3 && ds > H.h2sq;
-            if (wave_any(f2)) {
-                if (f2) far(1, s);
-            }
-            const bool in2 = in3 && !(ds > H.h2sq);
-            if (!wave_any(in2)) continue;
-            for (int r = H.rfirst[s]; r < H.rfirst[s + 1]; r++) {
+        if (!wave_any(in3)) {
+            r = min(r1, cr1);
+            continue;
+        }
+        const double ds = hcp_d2(xi, yi, zi, H.xc2 + 3 * s);
+        const bool f2 = in3 && ds > H.h2sq;
+        if (r == sr0 && wave_any(f2)) {
+            if (f2) far(1, s);
+        }
+        const bool in2 = in3 && !(ds > H.h2sq);
+        const int rend = min(r1, sr1);
+        if (wave_any(in2)) {
+            for (; r < rend; r++) {
                 const double dr = hcp_d2(xi, yi, zi, H.xc1 + 3 * r);
                 const bool f1 = in2 && dr > H.h1sq;
                 if (wave_any(f1)) {
@@ -499,10 +522,39 @@ __device__ __forceinline__ void hcp_walk(const HcpDev &H, bool valid, int i, dou
                 }
                 const bool in1 = in2 && !(dr > H.h1sq);
                 if (!wave_any(in1)) continue;
-                for (int a = H.ipres[r]; a < H.ipres[r + 1]; a++) member(a, in1 && a != i);
+                near_res(r, in1);
             }
         }
+        r = rend;
     }
+}
+
+template <typename Far, typename Member>
+__device__ __forceinline__ void hcp_walk(const HcpDev &H, int r0, int r1, bool valid, int i, double xi, double yi,
+                                         double zi, Far far, Member member)
+{
+    hcp_walk_res(H, r0, r1, valid, xi, yi, zi, far, [&](int r, bool in1) {
+        for (int a = H.ipres[r]; a < H.ipres[r + 1]; a++) member(a, in1 && a != i);
+    });
+}
+
+/* the work item of this wavefront: its tile's atom for this lane and its
+ * residue range */
+struct HcpItem {
+    int i, r0, r1;
+    bool valid;
+};
+
+__device__ __forceinline__ HcpItem hcp_item(const HcpDev &H)
+{
+    const int b = blockIdx.x, tile = b / H.nchunk, k = b - tile * H.nchunk;
+    HcpItem it;
+    const int t = tile * 64 + (int)threadIdx.x;
+    it.valid = t < H.natom;
+    it.i = it.valid ? t : H.natom - 1;
+    it.r0 = k * H.csize;
+    it.r1 = min(H.nres, it.r0 + H.csize);
+    return it;
 }
 
 __device__ __forceinline__ const double *lvl_q(const HcpDev &H, int l)
@@ -518,19 +570,23 @@ __device__ __forceinline__ const double *lvl_reff(const HcpDev &H, int l)
     return l == 0 ? H.rf1 : (l == 1 ? H.rf2 : H.rf3);
 }
 
-/* hcp_gb.c:179-430: Born radius of each atom */
-__global__ __launch_bounds__(HCP_THREADS) void k_hcp_born(HcpDev H, const double *__restrict__ x, DevPtrs d)
+__device__ __forceinline__ void ene_add(const DevPtrs &d, int term, double v)
 {
-    const int t = blockIdx.x * HCP_THREADS + threadIdx.x;
-    const bool valid = t < H.natom;
-    const int i = valid ? t : H.natom - 1;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&d.eslot[(size_t)term * ENE_SLOTS + (blockIdx.x & (ENE_SLOTS - 1))], v);
+}
+
+/* hcp_gb.c:179-430: descreening sums of each atom over one residue range */
+__global__ __launch_bounds__(64) void k_hcp_born(HcpDev H, const double *__restrict__ x)
+{
+    const HcpItem it = hcp_item(H);
+    const int i = it.i;
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
-    const double4 ai = H.atom[i];
-    const double ri = ai.y, ri1i = 1. / ri;
+    const double ri = H.atom[i].y, ri1i = 1. / ri;
     const int nq = H.nq;
     double sumi = 0.0;
     hcp_walk(
-        H, valid, i, xi, yi, zi,
+        H, it.r0, it.r1, it.valid, i, xi, yi, zi,
         [&](int l, int comp) {
             const double *qc = lvl_q(H, l) + 4 * (size_t)nq * comp;
             const double *rc = lvl_r(H, l) + (size_t)nq * comp;
@@ -539,7 +595,21 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_born(HcpDev H, const double
         [&](int a, bool on) {
             if (on) sumi -= near_descreen(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z);
         });
-    if (!valid) return;
+    if (it.valid && sumi != 0.0) atomicAdd(&H.sumi[i], sumi);
+}
+
+/*
+ * per atom: the Born radius from its descreening sum (hcp_gb.c:395-425),
+ * and the diagonal GB term with its sumdeijda part (hcp_gb.c:1877-1888),
+ * which start the sums the energy kernel adds to
+ */
+__global__ __launch_bounds__(HCP_THREADS) void k_hcp_radii(HcpDev H, DevPtrs d)
+{
+    const int t = blockIdx.x * HCP_THREADS + threadIdx.x;
+    const bool valid = t < H.natom;
+    const int i = valid ? t : H.natom - 1;
+    const double4 ai = H.atom[i];
+    const double ri = ai.y, ri1i = 1. / ri, sumi = H.sumi[i];
     double reff, psi = 0.0;
     if (H.gb == 1) {
         reff = 1.0 / (ri1i + sumi);
@@ -549,11 +619,23 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_born(HcpDev H, const double
         psi = -ri * sumi;
         reff = 1.0 / (ri1i - tanh((g.x - g.y * psi + g.z * psi * psi) * psi) / ai.w);
     }
-    H.reff[i] = reff;
-    H.psi[i] = psi;
-    const int sl = H.inv[i];
-    d.reff[sl] = reff;
-    d.psi[sl] = psi;
+    double ep = 0.0, sd = 0.0;
+    if (valid && !H.frz[i]) {
+        const double qi = ai.x, kappa = H.kappa;
+        const double expmkf = exp(-HCP_KSCALE * kappa * reff) / H.epsext, dielfac = 1.0 - expmkf;
+        const double qi2h = 0.5 * qi * qi, qid2h = qi2h * dielfac;
+        ep = -qid2h * 2.0 / reff;
+        sd = qid2h - HCP_KSCALE * kappa * qi2h * expmkf * reff;
+    }
+    if (valid) {
+        H.reff[i] = reff;
+        H.psi[i] = psi;
+        H.sd[i] = sd;
+        const int sl = H.inv[i];
+        d.reff[sl] = reff;
+        d.psi[sl] = psi;
+    }
+    ene_add(d, 0, 0.5 * ep);
 }
 
 /* ------------------------------------------- component Born radii */
@@ -679,33 +761,22 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_reff_up(HcpDev H)
 
 /* ------------------------------------------------ energies and forces */
 
-__device__ __forceinline__ void ene_add(const DevPtrs &d, int term, double v)
+/* hcp_gb.c:1843-1976 (egb_hcp_nbond, step 1) over one residue range:
+ * energies, sumdeijda and the pair forces on each atom of the tile */
+template <bool KAPPA>
+__global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__restrict__ x, DevPtrs d)
 {
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&d.eslot[(size_t)term * ENE_SLOTS + (blockIdx.x & (ENE_SLOTS - 1))], v);
-}
-
-/* hcp_gb.c:1843-1976 (egb_hcp_nbond, step 1) */
-__global__ __launch_bounds__(HCP_THREADS) void k_hcp_energy(HcpDev H, const double *__restrict__ x, DevPtrs d)
-{
-    const int t = blockIdx.x * HCP_THREADS + threadIdx.x;
-    const bool valid = t < H.natom;
-    const int i = valid ? t : H.natom - 1;
+    const HcpItem it = hcp_item(H);
+    const int i = it.i;
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
-    const double ri = H.reff[i], qi = H.atom[i].x, kappa = H.kappa, epsext = H.epsext;
+    const double ri = H.reff[i], qi = H.atom[i].x, kappa = H.kappa, inv_eps = 1.0 / H.epsext;
     const int2 lji = H.lj[i];
     const int nq = H.nq;
     int ep = H.ex_ptr[i];
     const int ee = H.ex_ptr[i + 1];
-    double sd = 0.0, ep_gb = 0.0, el = 0.0, ev = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-    if (valid && !H.frz[i]) {
-        const double expmkf = exp(-HCP_KSCALE * kappa * ri) / epsext, dielfac = 1.0 - expmkf;
-        const double qi2h = 0.5 * qi * qi, qid2h = qi2h * dielfac;
-        ep_gb += -qid2h * 2.0 / ri;
-        sd += qid2h - HCP_KSCALE * kappa * qi2h * expmkf * ri;
-    }
+    double sd = 0.0, eg = 0.0, el = 0.0, ev = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
     hcp_walk(
-        H, valid, i, xi, yi, zi,
+        H, it.r0, it.r1, it.valid, i, xi, yi, zi,
         [&](int l, int comp) {
             const double *qc = lvl_q(H, l) + 4 * (size_t)nq * comp;
             const double *rf = lvl_reff(H, l) + (size_t)nq * comp;
@@ -714,7 +785,7 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_energy(HcpDev H, const doub
                 if (fabs(qv[3]) > HCP_MIN_Q) {
                     const double xij = xi - qv[0], yij = yi - qv[1], zij = zi - qv[2];
                     const double r2 = xij * xij + yij * yij + zij * zij;
-                    const double de = gb_pair(qi, ri, qv[3], rf[q], r2, kappa, epsext, ep_gb, el, sd, true);
+                    const double de = gb_pair<KAPPA>(qi, ri, qv[3], rf[q], r2, kappa, inv_eps, eg, el, sd, true);
                     dx += de * xij;
                     dy += de * yij;
                     dz += de * zij;
@@ -723,14 +794,16 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_energy(HcpDev H, const doub
         },
         [&](int a, bool on) {
             if (!on) return;
+            /* the walk meets atoms in increasing order: a pointer into the
+             * ascending excluded-partner list finds them */
             while (ep < ee && H.ex_j[ep] < a) ep++;
             const bool excl = ep < ee && H.ex_j[ep] == a;
             const double xij = xi - x[3 * (size_t)a], yij = yi - x[3 * (size_t)a + 1], zij = zi - x[3 * (size_t)a + 2];
             const double r2 = xij * xij + yij * yij + zij * zij;
-            double de = gb_pair(qi, ri, H.atom[a].x, H.reff[a], r2, kappa, epsext, ep_gb, el, sd, !excl);
+            double de = gb_pair<KAPPA>(qi, ri, H.atom[a].x, H.reff[a], r2, kappa, inv_eps, eg, el, sd, !excl);
             if (!excl) {
                 const double2 ab = H.ljab[lji.x + H.lj[a].y];
-                const double rinv = 1. / sqrt(r2), r2inv = rinv * rinv, r6inv = r2inv * r2inv * r2inv;
+                const double rinv = rsqrt(r2), r2inv = rinv * rinv, r6inv = r2inv * r2inv * r2inv;
                 const double f6 = ab.y * r6inv, f12 = ab.x * r6inv * r6inv;
                 ev += f12 - f6;
                 de -= (12. * f12 - 6. * f6) * r2inv;
@@ -739,29 +812,40 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_energy(HcpDev H, const doub
             dy += de * yij;
             dz += de * zij;
         });
-    if (valid) {
-        H.sd[i] = sd;
+    if (it.valid) {
+        if (sd != 0.0) atomicAdd(&H.sd[i], sd);
         const int sl = H.inv[i];
-        d.sumdeijda[sl] = sd;
-        d.gx[sl] += dx;
-        d.gy[sl] += dy;
-        d.gz[sl] += dz;
+        if (dx != 0.0 || dy != 0.0 || dz != 0.0) {
+            atomicAdd(&d.gx[sl], dx);
+            atomicAdd(&d.gy[sl], dy);
+            atomicAdd(&d.gz[sl], dz);
+        }
     }
-    ene_add(d, 0, valid ? 0.5 * ep_gb : 0.0);
-    ene_add(d, 1, valid ? 0.5 * ev : 0.0);
-    ene_add(d, 2, valid ? 0.5 * el : 0.0);
+    ene_add(d, 0, it.valid ? 0.5 * eg : 0.0);
+    ene_add(d, 1, it.valid ? 0.5 * ev : 0.0);
+    ene_add(d, 2, it.valid ? 0.5 * el : 0.0);
 }
 
-/* hcp_gb.c:1978-2097 (egb_hcp_nbond, step 2) */
-__global__ __launch_bounds__(HCP_THREADS) void k_hcp_deriv(HcpDev H, const double *__restrict__ x, DevPtrs d)
+/*
+ * hcp_gb.c:1978-2097 (egb_hcp_nbond, step 2) over one residue range.  A
+ * near residue's atoms are staged in LDS, 64 at a time, and the lanes step
+ * through them in rotated order (lane l takes member (l + t) mod m at step
+ * t), so that each step's partner-side terms go to different LDS sums;
+ * each member's sum is added to its gradient with one atomic per residue
+ * visit.
+ */
+__global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__restrict__ x, DevPtrs d)
 {
-    const int t = blockIdx.x * HCP_THREADS + threadIdx.x;
-    const int i = t < H.natom ? t : H.natom - 1;
-    const bool valid = t < H.natom && !H.frz[i];
+    __shared__ double s_pos[4][64]; /* x, y, z, scaled radius of the staged members */
+    __shared__ double s_acc[3][64];
+    const HcpItem it = hcp_item(H);
+    const int i = it.i, lane = threadIdx.x;
+    const bool valid = it.valid && !H.frz[i];
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
     const double4 ai = H.atom[i];
     const double ri = ai.y, ri1i = 1. / ri;
     double sumda = H.sd[i];
+    if (it.valid && it.r0 == 0) d.sumdeijda[H.inv[i]] = sumda;
     if (H.gb > 1) {
         const double4 g = H.obc[i];
         const double ps = H.psi[i];
@@ -770,8 +854,8 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_deriv(HcpDev H, const doubl
     }
     const int nq = H.nq;
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    hcp_walk(
-        H, valid, i, xi, yi, zi,
+    hcp_walk_res(
+        H, it.r0, it.r1, valid, xi, yi, zi,
         [&](int l, int comp) {
             const double *qc = lvl_q(H, l) + 4 * (size_t)nq * comp;
             const double *rc = lvl_r(H, l) + (size_t)nq * comp;
@@ -785,31 +869,52 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_deriv(HcpDev H, const doubl
                 dz += zij * datmp;
             }
         },
-        [&](int a, bool on) {
-            double cx = 0.0, cy = 0.0, cz = 0.0;
-            if (on) {
-                const double xij = xi - x[3 * (size_t)a], yij = yi - x[3 * (size_t)a + 1],
-                             zij = zi - x[3 * (size_t)a + 2];
-                const double r2 = xij * xij + yij * yij + zij * zij;
-                const double datmp = descreen_deriv(r2, ri, ri1i, H.atom[a].z) * sumda;
-                cx = xij * datmp;
-                cy = yij * datmp;
-                cz = zij * datmp;
-                dx += cx;
-                dy += cy;
-                dz += cz;
-            }
-            cx = wave_sum(cx);
-            cy = wave_sum(cy);
-            cz = wave_sum(cz);
-            if ((threadIdx.x & 63) == 0) {
-                const int sa = H.inv[a];
-                atomicAdd(&d.gx[sa], cx);
-                atomicAdd(&d.gy[sa], cy);
-                atomicAdd(&d.gz[sa], cz);
+        [&](int r, bool in1) {
+            for (int base = H.ipres[r]; base < H.ipres[r + 1]; base += 64) {
+                const int m = min(64, H.ipres[r + 1] - base);
+                __syncthreads();
+                if (lane < m) {
+                    const int a = base + lane;
+                    s_pos[0][lane] = x[3 * (size_t)a];
+                    s_pos[1][lane] = x[3 * (size_t)a + 1];
+                    s_pos[2][lane] = x[3 * (size_t)a + 2];
+                    s_pos[3][lane] = H.atom[a].z;
+                    s_acc[0][lane] = 0.0;
+                    s_acc[1][lane] = 0.0;
+                    s_acc[2][lane] = 0.0;
+                }
+                __syncthreads();
+                if (in1) {
+                    int k = lane % m;
+                    for (int t = 0; t < m; t++) {
+                        if (base + k != i) {
+                            const double xij = xi - s_pos[0][k], yij = yi - s_pos[1][k], zij = zi - s_pos[2][k];
+                            const double r2 = xij * xij + yij * yij + zij * zij;
+                            const double datmp = descreen_deriv(r2, ri, ri1i, s_pos[3][k]) * sumda;
+                            const double cx = xij * datmp, cy = yij * datmp, cz = zij * datmp;
+                            dx += cx;
+                            dy += cy;
+                            dz += cz;
+                            atomicAdd(&s_acc[0][k], cx);
+                            atomicAdd(&s_acc[1][k], cy);
+                            atomicAdd(&s_acc[2][k], cz);
+                        }
+                        k = k + 1 == m ? 0 : k + 1;
+                    }
+                }
+                __syncthreads();
+                if (lane < m) {
+                    const int sa = H.inv[base + lane];
+                    const double ax = s_acc[0][lane], ay = s_acc[1][lane], az = s_acc[2][lane];
+                    if (ax != 0.0 || ay != 0.0 || az != 0.0) {
+                        atomicAdd(&d.gx[sa], ax);
+                        atomicAdd(&d.gy[sa], ay);
+                        atomicAdd(&d.gz[sa], az);
+                    }
+                }
             }
         });
-    if (valid) {
+    if (valid && (dx != 0.0 || dy != 0.0 || dz != 0.0)) {
         const int sl = H.inv[i];
         atomicAdd(&d.gx[sl], -dx);
         atomicAdd(&d.gy[sl], -dy);
@@ -840,7 +945,9 @@ int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags)
         hipLaunchKernelGGL(k_hcp_res<false>, dim3(gres), dim3(HCP_THREADS), 0, E.stream, H, x_dev);
         hipLaunchKernelGGL(k_hcp_up<false>, dim3(1), dim3(HCP_THREADS), 0, E.stream, H);
     }
-    hipLaunchKernelGGL(k_hcp_born, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, x_dev, E.d);
+    const unsigned items = (unsigned)H.ntile * (unsigned)H.nchunk;
+    hipLaunchKernelGGL(k_hcp_born, dim3(items), dim3(64), 0, E.stream, H, x_dev);
+    hipLaunchKernelGGL(k_hcp_radii, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
     if (H.opca) {
         hipLaunchKernelGGL(k_hcp_reff_res<true>, dim3(gres), dim3(HCP_THREADS), 0, E.stream, H, x_dev);
@@ -850,9 +957,12 @@ int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags)
         hipLaunchKernelGGL(k_hcp_reff_up<false>, dim3(1), dim3(HCP_THREADS), 0, E.stream, H);
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
-    hipLaunchKernelGGL(k_hcp_energy, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, x_dev, E.d);
+    if (H.kappa != 0.0)
+        hipLaunchKernelGGL(k_hcp_energy<true>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    else
+        hipLaunchKernelGGL(k_hcp_energy<false>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
-    hipLaunchKernelGGL(k_hcp_deriv, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, x_dev, E.d);
+    hipLaunchKernelGGL(k_hcp_deriv, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (!(flags & EV_NO_POST)) {
         const int nene = 3 * E.nmol, nt = std::max(64 * nene, g_dev ? E.nslots : 0);
@@ -986,6 +1096,20 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
     const int nq = h->hcp == 1 ? 1 : 2;
     std::vector<double> r1((size_t)h->nres * nq), r2((size_t)h->nstrand * nq), r3((size_t)h->ncomplex * nq);
     hcp_radii(ipres, rfirst, sfirst, sj_a, q_a, nq, r1, r2, r3);
+    std::vector<int> res_s(h->nres), s_cx(h->nstrand);
+    for (int s = 0; s < h->nstrand; s++)
+        for (int r = rfirst[s]; r < rfirst[s + 1]; r++) res_s[r] = s;
+    for (int c = 0; c < h->ncomplex; c++)
+        for (int s = sfirst[c]; s < sfirst[c + 1]; s++) s_cx[s] = c;
+    /* work items: 64-atom tiles x ranges of csize residues, enough of them
+     * for about 16 wavefronts per SIMD (4 SIMDs per CU) */
+    const int ntile = (n + 63) / 64;
+    const long long target = 64LL * std::max(1, E.ncu);
+    const long long want_chunks = std::max(1LL, (target + ntile - 1) / ntile);
+    int csize = (int)std::max(4LL, ((long long)h->nres + want_chunks - 1) / want_chunks);
+    csize = std::min(csize, h->nres);
+    const int nchunk = (h->nres + csize - 1) / csize;
+    if ((long long)ntile * nchunk > 0x7fffffffLL) return seterr(GBHIP_EINVAL, "too many hcp work items");
 
     HcpDev H;
     H.nq = nq;
@@ -1002,6 +1126,9 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
     H.kappa = E.par.kappa;
     H.epsext = E.par.epsext;
     H.ljab = E.d.ljab;
+    H.ntile = ntile;
+    H.nchunk = nchunk;
+    H.csize = csize;
     const size_t w = 3 + 4 * (size_t)nq;
     std::lock_guard<std::recursive_mutex> lk(device_alloc_lock());
     if ((rc = E.upload(&H.ipres, ipres)) || (rc = E.upload(&H.rfirst, rfirst)) || (rc = E.upload(&H.sfirst, sfirst)) ||
@@ -1017,7 +1144,8 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
         (rc = E.dalloc(&H.scnt, (size_t)h->nstrand)) || (rc = E.dalloc(&H.rmp, (size_t)MP_N * h->nres)) ||
         (rc = E.dalloc(&H.smp, (size_t)MP_N * h->nstrand)) || (rc = E.dalloc(&H.rw, 6 * (size_t)h->nres)) ||
         (rc = E.dalloc(&H.sw, 4 * (size_t)h->nstrand)) || (rc = E.dalloc(&H.reff, (size_t)n)) ||
-        (rc = E.dalloc(&H.psi, (size_t)n)) || (rc = E.dalloc(&H.sd, (size_t)n)))
+        (rc = E.dalloc(&H.psi, (size_t)n)) || (rc = E.dalloc(&H.sd, (size_t)n)) ||
+        (rc = E.dalloc(&H.sumi, (size_t)n)) || (rc = E.upload(&H.res_s, res_s)) || (rc = E.upload(&H.s_cx, s_cx)))
         return rc;
     if ((rc = E.upload_done())) return rc;
     H.on = 1;
