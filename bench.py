@@ -516,6 +516,145 @@ def bench_hessian(args):
                       "rows": rows}))
 
 
+# hierarchical charge partitioning workload: a generated polymer in 20-atom
+# residues, strands of 50 residues, complexes of 10 strands, the sff.c
+# default thresholds, gb=5, 0.1 M salt
+HCP_NATOM, HCP_SEED = 100000, 7
+HCP_H = (15.0, 50.0, 150.0)
+HCP_KAPPA = 0.10395
+# algorithmic flops per unit of the hierarchical walk, counted from the
+# reference expressions (hcp_gb.c) as the pair path's are: a component
+# distance check; per far charge and per near ordered atom pair in the
+# Born pass, the energy pass (GB + Coulomb, + LJ for atom pairs) and the
+# chain-rule pass
+HCP_FLOPS = dict(check=8, born_far=23, born_near=24, energy_far=56, energy_near=68, deriv_far=34, deriv_near=37)
+
+
+def hcp_layers(natom, res_atoms=20, strand_res=50, complex_strands=10):
+    import numpy as np
+    ipres = np.array(list(range(0, natom, res_atoms)) + [natom], np.int32)
+    ips = np.arange(1, len(ipres), strand_res, dtype=np.int32)
+    ipc = np.arange(1, len(ips) + 1, complex_strands, dtype=np.int32)
+    return ipres, ips, ipc
+
+
+def hcp_walk_counts(x, ipres, ips, ipc, h):
+    """Units of one evaluation's hierarchical walk (as each atom walks it in
+    hcp_gb.c): component distance checks, far components, near ordered
+    atom pairs."""
+    import numpy as np
+    xyz = x.reshape(-1, 3)
+    n, nres = len(xyz), len(ipres) - 1
+    rfirst = np.append(ips - 1, nres)
+    sfirst = np.append(ipc - 1, len(ips))
+    natres = np.diff(ipres)
+    res_of = np.repeat(np.arange(nres), natres)
+    cres = np.add.reduceat(xyz, ipres[:-1], axis=0) / natres[:, None]
+    cstr = np.array([xyz[ipres[rfirst[s]]:ipres[rfirst[s + 1]]].mean(0) for s in range(len(ips))])
+    ccpx = np.array([xyz[ipres[rfirst[sfirst[c]]]:ipres[rfirst[sfirst[c + 1]]]].mean(0) for c in range(len(ipc))])
+    s_of_r = np.repeat(np.arange(len(ips)), np.diff(rfirst))
+    c_of_s = np.repeat(np.arange(len(ipc)), np.diff(sfirst))
+    checks = far = near = 0
+    for i0 in range(0, n, 2000):
+        xi = xyz[i0:i0 + 2000]
+        d3 = ((xi[:, None, :] - ccpx[None]) ** 2).sum(-1) > h[2] ** 2
+        checks += d3.size
+        far += int(d3.sum())
+        in3 = ~d3[:, c_of_s]
+        d2 = ((xi[:, None, :] - cstr[None]) ** 2).sum(-1) > h[1] ** 2
+        checks += int(in3.sum())
+        far += int((in3 & d2).sum())
+        in2 = (in3 & ~d2)[:, s_of_r]
+        d1 = ((xi[:, None, :] - cres[None]) ** 2).sum(-1) > h[0] ** 2
+        checks += int(in2.sum())
+        far += int((in2 & d1).sum())
+        in1 = in2 & ~d1
+        near += int((in1 * natres[None, :]).sum()) - int(in1[np.arange(len(xi)), res_of[i0:i0 + 2000]].sum())
+    return checks, far, near
+
+
+def bench_hcp(args):
+    """Hierarchical charge partitioning (egb_hcp) at 100,000 atoms (the
+    system size the reference offers it for): evaluations/s of hcp 1, 2 and
+    4 (HIP events around each pass, after warm-up), the roofline of the
+    energy kernel (algorithmic FP64 flops of the walk per launch over its
+    duration), and the oracle's egb_hcp on the host cores."""
+    import numpy as np
+    import nabc_amd as na
+    parm, arr, x = na.build_system(HCP_NATOM, HCP_SEED)
+    ipres, ips, ipc = hcp_layers(HCP_NATOM)
+    steps = max(5, min(args.steps, 100))
+    rows = {}
+    for hcp in (1, 2, 4):
+        ctx = na.GBContext(arr, x, gb=5, cut=999.0, rgbmax=999.0, kappa=HCP_KAPPA)
+        ctx.set_hcp(hcp, ipres, ips, ipc, h1=HCP_H[0], h2=HCP_H[1], h3=HCP_H[2])
+        ctx.nblist()
+        for _ in range(3):
+            ctx.egb_device()
+        ctx.sync()
+        ctx.timing_start(steps)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.egb_device()
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / steps
+        ms, ne = ctx.timing_stop()
+        e = ctx.egb()
+        ctx.close()
+        if hcp == 1:
+            grad1 = e[3]
+        rows[hcp] = dict(evals_per_s=round(1.0 / wall, 2), ms_per_eval=round(1e3 * wall, 4),
+                         pass_ms=dict(zip(("charges_born", "component_radii", "energy", "deriv"),
+                                          [round(v / ne, 4) for v in ms])),
+                         energies=dict(epol=e[0], evdw=e[1], eelec=e[2]))
+        print("# hcp %d: %s" % (hcp, json.dumps(rows[hcp])), file=sys.stderr, flush=True)
+    checks, far, near = hcp_walk_counts(x, ipres, ips, ipc, HCP_H)
+    F = HCP_FLOPS
+    w_energy = checks * F["check"] + far * F["energy_far"] + near * F["energy_near"]
+    w_eval = (3 * checks * F["check"] + far * (F["born_far"] + F["energy_far"] + F["deriv_far"]) +
+              near * (F["born_near"] + F["energy_near"] + F["deriv_near"]))
+    t_energy = rows[1]["pass_ms"]["energy"] * 1e-3
+    achieved = w_energy / t_energy / 1e12
+    line = {"metric": "HCP GB energy+grad evals/s at 100k atoms", "value": rows[1]["evals_per_s"],
+            "unit": "evals/s", "n_gpus": 1, "steps": steps, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic (seeded 100,000-atom polymer, nabc_amd/csrc/gen.c)",
+            "config": {"workload": "egb_hcp, 100,000 atoms in 5,000 20-atom residues, 100 strands, 10 complexes, "
+                                   "thresholds 15/50/150 A, gb=5, kappa 0.10395",
+                       "natom": HCP_NATOM, "seed": HCP_SEED},
+            "hcp": {str(k): v for k, v in rows.items()},
+            "walk": {"checks": checks, "far_components": far, "near_pairs": near,
+                     "flops_per_eval_hcp1": w_eval, "flops_per_unit": F},
+            "roofline": {"bound": "valu", "kernel": "k_hcp_energy<true>", "kernel_ms": round(1e3 * t_energy, 4),
+                         "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                         "frac": round(achieved / PEAK["fp64"], 4),
+                         "eval_achieved": round(w_eval / (rows[1]["ms_per_eval"] * 1e-3) / 1e12, 3)}}
+    if not args.no_cpu:
+        import oracle
+        visible = len(os.sched_getaffinity(0))
+        threads = visible
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        if omp.isdigit() and 0 < int(omp) < visible:
+            threads = int(omp)
+        oracle.set_threads(threads)
+        o = oracle.Oracle(parm, HCP_NATOM, gb=5, cut=999.0, rgbmax=999.0, kappa=HCP_KAPPA)
+        o.set_hcp(1, ipres, ips, ipc, h1=HCP_H[0], h2=HCP_H[1], h3=HCP_H[2])
+        n, t = 0, 0.0
+        while t < 15.0 and n < 20:
+            t0 = time.perf_counter()
+            ref = o.egb(x)
+            t += time.perf_counter() - t0
+            n += 1
+        o.close()
+        line["cpu_baseline"] = {"value": round(n / t, 4), "unit": "evals/s", "cores": threads, "kind": "port",
+                                "sample": "%d oracle egb_hcp (hcp 1) evaluations of the same system, OpenMP, "
+                                          "%d threads" % (n, threads)}
+        ge = float(np.sqrt(np.mean((grad1 - ref[3]) ** 2) / np.mean(ref[3] ** 2)))
+        line["error_vs_oracle_hcp1"] = {
+            k: abs(rows[1]["energies"][k] - ref[j]) / abs(ref[j]) for j, k in enumerate(("epol", "evdw", "eelec"))}
+        line["error_vs_oracle_hcp1"]["grad_rel_rms"] = ge
+    print(json.dumps(line))
+
+
 def bench_sweep(args):
     """Size sweep of the headline workload (igb=8, cut = rgbmax = 25 A,
     kappa 0.127315, list rebuilt every 20 evaluations): evaluations/s and
@@ -571,7 +710,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     ap.add_argument("--config", default="c3",
-                    choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg", "hessian"],
+                    choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg", "hessian", "hcp"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
                          "errors against the oracle at C1/C2/C3; sweep: evaluation rate at 5k-100k atoms")
@@ -608,6 +747,8 @@ def main():
         return bench_cg(args)
     if args.config == "hessian":
         return bench_hessian(args)
+    if args.config == "hcp":
+        return bench_hcp(args)
 
     import numpy as np
     import nabc_amd as na

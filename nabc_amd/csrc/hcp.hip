@@ -18,22 +18,26 @@
  *                  unnormalised sums the levels above add up
  *   k_hcp_up       strands, then complexes (one workgroup), in the
  *                  reference's summation order
- *   k_hcp_born     per atom: Born radius (egb_hcp_atom_reff)
+ *   k_hcp_born     descreening sums (egb_hcp_atom_reff)
+ *   k_hcp_radii    per atom: Born radius, diagonal GB term
  *   k_hcp_reff_res per residue: component Born radii (egb_hcp_reff /
  *                  egb_hcpo_reff_avg at the residue level)
  *   k_hcp_reff_up  strands and complexes
- *   k_hcp_energy   per atom: GB, Coulomb and LJ energies, sumdeijda and
- *                  the pair forces on the atom (egb_hcp_nbond step 1)
- *   k_hcp_deriv    per atom: the chain rule through the Born radii
- *                  (step 2); forces on the partner atoms are summed over
- *                  the wavefront and added with one atomic per atom
+ *   k_hcp_energy   GB, Coulomb and LJ energies, sumdeijda and the pair
+ *                  forces on each atom (egb_hcp_nbond step 1)
+ *   k_hcp_deriv    the chain rule through the Born radii (step 2); forces
+ *                  on member atoms summed in LDS, one atomic per atom and
+ *                  residue visit
  * then the engine's own energy-slot and gradient kernels.  Everything is
  * FP64 except what the reference computes in single precision (component
  * radii, the HCPO moments and point charges), which is single precision
  * here too, compiled without contraction so each operation rounds as it
  * does there.
  *
- * Each per-atom kernel runs one atom per lane; the hierarchy loops are
+ * The walking kernels (born, energy, deriv) take work items of one
+ * wavefront: the 64 atoms of a tile (one per lane, original order) against
+ * one range of consecutive residues, about 512 work items per CU, their
+ * partial sums added with atomics.  Within an item the hierarchy loops are
  * wavefront-uniform (every lane steps through the same complexes, strands,
  * residues and atoms) and a lane's branch (far charges or the component's
  * members) runs only where some lane needs it.
@@ -1101,10 +1105,13 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
         for (int r = rfirst[s]; r < rfirst[s + 1]; r++) res_s[r] = s;
     for (int c = 0; c < h->ncomplex; c++)
         for (int s = sfirst[c]; s < sfirst[c + 1]; s++) s_cx[s] = c;
-    /* work items: 64-atom tiles x ranges of csize residues, enough of them
-     * for about 16 wavefronts per SIMD (4 SIMDs per CU) */
+    /* work items: 64-atom tiles x ranges of csize residues, about 512 per CU
+     * (ranges of nearby and of distant residues differ several-fold in
+     * work; 100k atoms, per evaluation: 7.66 / 7.22 / 6.89 / 6.40 / 6.22 /
+     * 6.91 / 8.01 ms at 32 / 64 / 128 / 256 / 512 / 1024 / 2048 per CU, the
+     * last ones paying for the per-range atomics) */
     const int ntile = (n + 63) / 64;
-    const long long target = 64LL * std::max(1, E.ncu);
+    const long long target = 512LL * std::max(1, E.ncu);
     const long long want_chunks = std::max(1LL, (target + ntile - 1) / ntile);
     int csize = (int)std::max(4LL, ((long long)h->nres + want_chunks - 1) / want_chunks);
     csize = std::min(csize, h->nres);
