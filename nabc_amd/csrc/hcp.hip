@@ -340,7 +340,7 @@ __device__ static float spread(const double *pts, int n, const float rcog[3])
 }
 
 /* two optimal point charges of a component with the given moments and
- * centre; offsets limited to 0.8 / 0.9 of its spread (gb > 0, hcp.c:576) */
+ * centre; offsets limited to 0.8 / 0.9 of its spread (gb > 0, hcp.c:578-581) */
 __device__ static void opca_component(const float *mp, const double *centre, const double *pts, int npts,
                                       float coef2, double *qv)
 {
