@@ -576,58 +576,69 @@ def hcp_walk_counts(x, ipres, ips, ipc, h):
 def bench_hcp(args):
     """Hierarchical charge partitioning (egb_hcp) at 100,000 atoms (the
     system size the reference offers it for): evaluations/s of hcp 1, 2 and
-    4 (HIP events around each pass, after warm-up), the roofline of the
-    energy kernel (algorithmic FP64 flops of the walk per launch over its
-    duration), and the oracle's egb_hcp on the host cores."""
+    4 in the mixed and the FP64 variant (HIP events around each pass, after
+    warm-up), the roofline of the energy kernel (algorithmic flops of the
+    walk per launch over its duration), errors against the oracle, and the
+    oracle's egb_hcp on the host cores."""
     import numpy as np
     import nabc_amd as na
     parm, arr, x = na.build_system(HCP_NATOM, HCP_SEED)
     ipres, ips, ipc = hcp_layers(HCP_NATOM)
     steps = max(5, min(args.steps, 100))
-    rows = {}
-    for hcp in (1, 2, 4):
-        ctx = na.GBContext(arr, x, gb=5, cut=999.0, rgbmax=999.0, kappa=HCP_KAPPA)
-        ctx.set_hcp(hcp, ipres, ips, ipc, h1=HCP_H[0], h2=HCP_H[1], h3=HCP_H[2])
-        ctx.nblist()
-        for _ in range(3):
-            ctx.egb_device()
-        ctx.sync()
-        ctx.timing_start(steps)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ctx.egb_device()
-        ctx.sync()
-        wall = (time.perf_counter() - t0) / steps
-        ms, ne = ctx.timing_stop()
-        e = ctx.egb()
-        ctx.close()
-        if hcp == 1:
-            grad1 = e[3]
-        rows[hcp] = dict(evals_per_s=round(1.0 / wall, 2), ms_per_eval=round(1e3 * wall, 4),
-                         pass_ms=dict(zip(("charges_born", "component_radii", "energy", "deriv"),
-                                          [round(v / ne, 4) for v in ms])),
-                         energies=dict(epol=e[0], evdw=e[1], eelec=e[2]))
-        print("# hcp %d: %s" % (hcp, json.dumps(rows[hcp])), file=sys.stderr, flush=True)
+    runs, grads = {}, {}
+    for pname, prec in (("mixed", na.GBHIP_MIXED), ("fp64", na.GBHIP_FP64)):
+        rows = {}
+        for hcp in (1, 2, 4):
+            ctx = na.GBContext(arr, x, gb=5, cut=999.0, rgbmax=999.0, kappa=HCP_KAPPA, precision=prec)
+            ctx.set_hcp(hcp, ipres, ips, ipc, h1=HCP_H[0], h2=HCP_H[1], h3=HCP_H[2])
+            ctx.nblist()
+            for _ in range(3):
+                ctx.egb_device()
+            ctx.sync()
+            ctx.timing_start(steps)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.egb_device()
+            ctx.sync()
+            wall = (time.perf_counter() - t0) / steps
+            ms, ne = ctx.timing_stop()
+            e = ctx.egb()
+            ctx.close()
+            if hcp == 1:
+                grads[pname] = e[3]
+            rows[hcp] = dict(evals_per_s=round(1.0 / wall, 2), ms_per_eval=round(1e3 * wall, 4),
+                             pass_ms=dict(zip(("charges_born", "component_radii", "energy", "deriv"),
+                                              [round(v / ne, 4) for v in ms])),
+                             energies=dict(epol=e[0], evdw=e[1], eelec=e[2]))
+            print("# %s hcp %d: %s" % (pname, hcp, json.dumps(rows[hcp])), file=sys.stderr, flush=True)
+        runs[pname] = rows
     checks, far, near = hcp_walk_counts(x, ipres, ips, ipc, HCP_H)
     F = HCP_FLOPS
     w_energy = checks * F["check"] + far * F["energy_far"] + near * F["energy_near"]
     w_eval = (3 * checks * F["check"] + far * (F["born_far"] + F["energy_far"] + F["deriv_far"]) +
               near * (F["born_near"] + F["energy_near"] + F["deriv_near"]))
-    t_energy = rows[1]["pass_ms"]["energy"] * 1e-3
-    achieved = w_energy / t_energy / 1e12
-    line = {"metric": "HCP GB energy+grad evals/s at 100k atoms", "value": rows[1]["evals_per_s"],
-            "unit": "evals/s", "n_gpus": 1, "steps": steps, "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic (seeded 100,000-atom polymer, nabc_amd/csrc/gen.c)",
-            "config": {"workload": "egb_hcp, 100,000 atoms in 5,000 20-atom residues, 100 strands, 10 complexes, "
-                                   "thresholds 15/50/150 A, gb=5, kappa 0.10395",
+
+    def roof(pname, kernel):
+        t = runs[pname][1]["pass_ms"]["energy"] * 1e-3
+        a = w_energy / t / 1e12
+        return {"bound": "valu", "kernel": kernel, "kernel_ms": round(1e3 * t, 4), "achieved": round(a, 3),
+                "peak": PEAK[pname], "unit": "TFLOP/s", "frac": round(a / PEAK[pname], 4),
+                "eval_achieved": round(w_eval / (runs[pname][1]["ms_per_eval"] * 1e-3) / 1e12, 3)}
+
+    m1, f1 = runs["mixed"][1], runs["fp64"][1]
+    line = {"metric": "HCP GB energy+grad evals/s at 100k atoms", "value": m1["evals_per_s"],
+            "unit": "evals/s", "n_gpus": 1, "steps": steps, "higher_is_better": True,
+            "dtype": "f32-pair/f64-accumulate", "data": "synthetic (seeded 100,000-atom polymer, nabc_amd/csrc/gen.c)",
+            "config": {"workload": "egb_hcp hcp 1, 100,000 atoms in 5,000 20-atom residues, 100 strands, "
+                                   "10 complexes, thresholds 15/50/150 A, gb=5, kappa 0.10395",
                        "natom": HCP_NATOM, "seed": HCP_SEED},
-            "hcp": {str(k): v for k, v in rows.items()},
+            "mixed": {str(k): v for k, v in runs["mixed"].items()},
+            "fp64": {str(k): v for k, v in runs["fp64"].items()},
+            "mixed_vs_fp64": round(m1["evals_per_s"] / f1["evals_per_s"], 3),
             "walk": {"checks": checks, "far_components": far, "near_pairs": near,
                      "flops_per_eval_hcp1": w_eval, "flops_per_unit": F},
-            "roofline": {"bound": "valu", "kernel": "k_hcp_energy<true>", "kernel_ms": round(1e3 * t_energy, 4),
-                         "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                         "frac": round(achieved / PEAK["fp64"], 4),
-                         "eval_achieved": round(w_eval / (rows[1]["ms_per_eval"] * 1e-3) / 1e12, 3)}}
+            "roofline": roof("mixed", "k_hcp_energy<float, true>"),
+            "roofline_fp64": roof("fp64", "k_hcp_energy<double, true>")}
     if not args.no_cpu:
         import oracle
         visible = len(os.sched_getaffinity(0))
@@ -648,10 +659,11 @@ def bench_hcp(args):
         line["cpu_baseline"] = {"value": round(n / t, 4), "unit": "evals/s", "cores": threads, "kind": "port",
                                 "sample": "%d oracle egb_hcp (hcp 1) evaluations of the same system, OpenMP, "
                                           "%d threads" % (n, threads)}
-        ge = float(np.sqrt(np.mean((grad1 - ref[3]) ** 2) / np.mean(ref[3] ** 2)))
-        line["error_vs_oracle_hcp1"] = {
-            k: abs(rows[1]["energies"][k] - ref[j]) / abs(ref[j]) for j, k in enumerate(("epol", "evdw", "eelec"))}
-        line["error_vs_oracle_hcp1"]["grad_rel_rms"] = ge
+        for pname in ("mixed", "fp64"):
+            err = {k: abs(runs[pname][1]["energies"][k] - ref[j]) / abs(ref[j])
+                   for j, k in enumerate(("epol", "evdw", "eelec"))}
+            err["grad_rel_rms"] = float(np.sqrt(np.mean((grads[pname] - ref[3]) ** 2) / np.mean(ref[3] ** 2)))
+            line["error_vs_oracle_hcp1_" + pname] = err
     print(json.dumps(line))
 
 

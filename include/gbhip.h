@@ -189,9 +189,10 @@ int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const 
  * Born radii have no rgbmax cut-off and no neck term, and no pair list is
  * built (eff.c:2560-2561): gbhip_nblist returns 0 entries.  Every
  * evaluation (gbhip_egb, gbhip_mme, dynamics, conjgrad) then runs the HCP
- * kernels, in FP64 whatever the context's precision; the reference's
- * single-precision parts (component radii, HCPO moments and charges) stay
- * single precision.  hcp = 0 (or h == NULL) returns to the pair-list path
+ * kernels in the context's precision (GBHIP_MIXED: descreening and GB
+ * terms in FP32, Coulomb, LJ and every sum in FP64, as the pair path); the
+ * reference's single-precision parts (component radii, HCPO moments and
+ * charges) stay single precision in both.  hcp = 0 (or h == NULL) returns to the pair-list path
  * (a list must be built again).  Single-molecule contexts only; not with
  * the deterministic mode; not while dynamics graphs exist.
  */

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "gb_math.h"
 
 #define HCP_KSCALE 0.73
 #define HCP_MIN_Q 0.00001
@@ -82,86 +83,113 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+/*
+ * Pair arithmetic, R = double (every operation FP64) or float (the mixed
+ * variant: the descreening and GB terms in FP32 with the hardware
+ * transcendentals, as the pair kernels' mixed variant computes them;
+ * squared distances from FP64 coordinates, Coulomb and LJ in FP64, every
+ * sum the caller keeps in FP64).  gbm<R> (gb_math.h) supplies exp, log,
+ * reciprocal and reciprocal square root.
+ */
+
 /* far component: the Taylor term of the descreening integral only
  * (hcp_gb.c:139-160) */
-__device__ __forceinline__ double far_descreen(double r2, double sj)
+template <typename R> __device__ __forceinline__ double far_descreen(double r2d, double sjd)
 {
-    const double rs = rsqrt(r2), dij2i = rs * rs, sj2 = sj * sj, tmpsd = sj2 * dij2i;
-    const double dumbo = HCP_TA + tmpsd * (HCP_TB + tmpsd * (HCP_TC + tmpsd * (HCP_TD + tmpsd * HCP_TDD)));
-    return sj * tmpsd * dij2i * dumbo;
+    const R r2 = (R)r2d, sj = (R)sjd;
+    const R rs = gbm<R>::rsqrt_(r2), dij2i = rs * rs, sj2 = sj * sj, tmpsd = sj2 * dij2i;
+    const R dumbo = R(HCP_TA) + tmpsd * (R(HCP_TB) + tmpsd * (R(HCP_TC) + tmpsd * (R(HCP_TD) + tmpsd * R(HCP_TDD))));
+    return (double)(sj * tmpsd * dij2i * dumbo);
 }
 
 /* a member atom's descreening of atom i: the analytic cases without the
  * rgbmax cut-off or the neck term (hcp_gb.c:345-390) */
-__device__ __forceinline__ double near_descreen(double r2, double ri, double ri1i, double sj)
+template <typename R> __device__ __forceinline__ double near_descreen(double r2d, R ri, R ri1i, double sjd)
 {
-    const double dij1i = rsqrt(r2), dij = r2 * dij1i, sj2 = sj * sj;
-    if (dij > 4.0 * sj) {
-        const double dij2i = dij1i * dij1i, tmpsd = sj2 * dij2i;
-        const double dumbo = HCP_TA + tmpsd * (HCP_TB + tmpsd * (HCP_TC + tmpsd * (HCP_TD + tmpsd * HCP_TDD)));
-        return sj * tmpsd * dij2i * dumbo;
+    const R r2 = (R)r2d, sj = (R)sjd;
+    const R dij1i = gbm<R>::rsqrt_(r2), dij = r2 * dij1i, sj2 = sj * sj;
+    R v = R(0);
+    if (dij > R(4) * sj) {
+        const R dij2i = dij1i * dij1i, tmpsd = sj2 * dij2i;
+        const R dumbo =
+            R(HCP_TA) + tmpsd * (R(HCP_TB) + tmpsd * (R(HCP_TC) + tmpsd * (R(HCP_TD) + tmpsd * R(HCP_TDD))));
+        v = sj * tmpsd * dij2i * dumbo;
     } else if (dij > ri + sj) {
-        return 0.5 * (sj / (r2 - sj2) + 0.5 * dij1i * log((dij - sj) / (dij + sj)));
+        v = R(0.5) * (sj * gbm<R>::rcp(r2 - sj2) + R(0.5) * dij1i * gbm<R>::log_((dij - sj) * gbm<R>::rcp(dij + sj)));
     } else if (dij > fabs(ri - sj)) {
-        const double theta = 0.5 * ri1i * dij1i * (r2 + ri * ri - sj2), uij = 1. / (dij + sj);
-        return 0.25 * (ri1i * (2. - theta) - uij + dij1i * log(ri * uij));
+        const R theta = R(0.5) * ri1i * dij1i * (r2 + ri * ri - sj2), uij = gbm<R>::rcp(dij + sj);
+        v = R(0.25) * (ri1i * (R(2) - theta) - uij + dij1i * gbm<R>::log_(ri * uij));
     } else if (ri < sj) {
-        return 0.5 * (sj / (r2 - sj2) + 2. * ri1i + 0.5 * dij1i * log((sj - dij) / (sj + dij)));
+        v = R(0.5) * (sj * gbm<R>::rcp(r2 - sj2) + R(2) * ri1i +
+                      R(0.5) * dij1i * gbm<R>::log_((sj - dij) * gbm<R>::rcp(sj + dij)));
     }
-    return 0.0;
+    return (double)v;
 }
 
 /* numerator of dR_i/dd_ij over d_ij (hcp_gb.c:1599-1645, :1712-1764) */
-__device__ __forceinline__ double descreen_deriv(double r2, double ri, double ri1i, double sj)
+template <typename R> __device__ __forceinline__ double descreen_deriv(double r2d, R ri, R ri1i, double sjd)
 {
-    const double dij1i = rsqrt(r2), dij2i = dij1i * dij1i, dij = r2 * dij1i, sj2 = sj * sj;
-    if (dij > 4.0 * sj) {
-        const double tmpsd = sj2 * dij2i;
-        const double dumbo = HCP_TE + tmpsd * (HCP_TF + tmpsd * (HCP_TG + tmpsd * (HCP_TH + tmpsd * HCP_THH)));
-        return tmpsd * sj * dij2i * dij2i * dumbo;
+    const R r2 = (R)r2d, sj = (R)sjd;
+    const R dij1i = gbm<R>::rsqrt_(r2), dij2i = dij1i * dij1i, dij = r2 * dij1i, sj2 = sj * sj;
+    R v = R(0);
+    if (dij > R(4) * sj) {
+        const R tmpsd = sj2 * dij2i;
+        const R dumbo =
+            R(HCP_TE) + tmpsd * (R(HCP_TF) + tmpsd * (R(HCP_TG) + tmpsd * (R(HCP_TH) + tmpsd * R(HCP_THH))));
+        v = tmpsd * sj * dij2i * dij2i * dumbo;
     } else if (dij > ri + sj) {
-        const double t1 = 1. / (r2 - sj2);
-        return t1 * sj * (-0.5 * dij2i + t1) + 0.25 * dij1i * dij2i * log((dij - sj) / (dij + sj));
+        const R t1 = gbm<R>::rcp(r2 - sj2);
+        v = t1 * sj * (R(-0.5) * dij2i + t1) +
+            R(0.25) * dij1i * dij2i * gbm<R>::log_((dij - sj) * gbm<R>::rcp(dij + sj));
     } else if (dij > fabs(ri - sj)) {
-        const double t1 = 1. / (dij + sj), dij3i = dij2i * dij1i;
-        return -0.25 * (-0.5 * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i + dij1i * t1 * (t1 - dij1i) -
-                        dij3i * log(ri * t1));
+        const R t1 = gbm<R>::rcp(dij + sj), dij3i = dij2i * dij1i;
+        v = R(-0.25) * (R(-0.5) * (r2 - ri * ri + sj2) * dij3i * ri1i * ri1i + dij1i * t1 * (t1 - dij1i) -
+                        dij3i * gbm<R>::log_(ri * t1));
     } else if (ri < sj) {
-        const double t1 = 1. / (r2 - sj2);
-        return -0.5 * (sj * dij2i * t1 - 2. * sj * t1 * t1 - 0.5 * dij2i * dij1i * log((sj - dij) / (sj + dij)));
+        const R t1 = gbm<R>::rcp(r2 - sj2);
+        v = R(-0.5) * (sj * dij2i * t1 - R(2) * sj * t1 * t1 -
+                       R(0.5) * dij2i * dij1i * gbm<R>::log_((sj - dij) * gbm<R>::rcp(sj + dij)));
     }
-    return 0.;
+    return (double)v;
 }
+
+/* FP64 1/r for Coulomb and LJ: the hardware estimate refined by Newton
+ * steps (FP64) or the FP32 estimate refined by one FP64 step (mixed) */
+template <typename R> __device__ __forceinline__ double hcp_rinv(double r2);
+template <> __device__ __forceinline__ double hcp_rinv<double>(double r2) { return gbm<double>::rsqrt_(r2); }
+template <> __device__ __forceinline__ double hcp_rinv<float>(double r2) { return rsqrt_pair<float>(r2); }
 
 /* GB (+ Coulomb) of atom i with one charge qj of Born radius rj at r^2:
  * energies, i's sumdeijda term; returns the pair force factor
  * (hcp_gb.c:1369-1436, :1473-1520).  f_GB from one reciprocal square root
  * (1/f and f = (r^2 + RiRj e) / f), the screening exponential only with
- * salt (KAPPA), inv_eps = 1 / epsext. */
-template <bool KAPPA>
-__device__ __forceinline__ double gb_pair(double qi, double ri, double qj, double rj, double r2, double kappa,
-                                          double inv_eps, double &epol, double &elec, double &sd, bool coulomb)
+ * salt (KAPPA); kks = -0.73 kappa, inv_eps = 1 / epsext. */
+template <typename R, bool KAPPA>
+__device__ __forceinline__ double gb_pair(double qi, R ri, double qj, R rj, double r2d, R kks, R inv_eps,
+                                          double &epol, double &elec, double &sd, bool coulomb)
 {
-    const double qiqj = qi * qj, rb2 = ri * rj;
-    const double efac = exp(-r2 / (4.0 * rb2));
-    const double fsq = r2 + rb2 * efac;
-    const double fgbi = rsqrt(fsq);
-    double expmkf, fgbk = 0.0;
+    const R r2 = (R)r2d;
+    const double qiqj_d = qi * qj;
+    const R qiqj = (R)qiqj_d, rb2 = ri * rj;
+    const R efac = gbm<R>::exp_(-r2 * gbm<R>::rcp(R(4) * rb2));
+    const R fsq = r2 + rb2 * efac;
+    const R fgbi = gbm<R>::rsqrt_(fsq);
+    R expmkf, fgbk = R(0);
     if (KAPPA) {
-        fgbk = -kappa * HCP_KSCALE * (fsq * fgbi);
-        expmkf = exp(fgbk) * inv_eps;
+        fgbk = kks * (fsq * fgbi);
+        expmkf = gbm<R>::exp_(fgbk) * inv_eps;
     } else {
         expmkf = inv_eps;
     }
-    const double dielfac = 1.0 - expmkf;
-    epol += -qiqj * dielfac * fgbi;
-    const double temp4 = fgbi * fgbi * fgbi;
-    const double temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
-    double de = temp6 * (1.0 - 0.25 * efac);
-    const double temp5 = 0.5 * efac * temp6 * (rb2 + 0.25 * r2);
-    sd += ri * temp5;
+    const R dielfac = R(1) - expmkf;
+    epol += (double)(-qiqj * dielfac * fgbi);
+    const R temp4 = fgbi * fgbi * fgbi;
+    const R temp6 = qiqj * temp4 * (dielfac + fgbk * expmkf);
+    double de = (double)(temp6 * (R(1) - R(0.25) * efac));
+    const R temp5 = R(0.5) * efac * temp6 * (rb2 + R(0.25) * r2);
+    sd += (double)(ri * temp5);
     if (coulomb) {
-        const double rinv = rsqrt(r2), r2inv = rinv * rinv, eel = qiqj * rinv;
+        const double rinv = hcp_rinv<R>(r2d), r2inv = rinv * rinv, eel = qiqj_d * rinv;
         elec += eel;
         de -= eel * r2inv;
     }
@@ -581,12 +609,13 @@ __device__ __forceinline__ void ene_add(const DevPtrs &d, int term, double v)
 }
 
 /* hcp_gb.c:179-430: descreening sums of each atom over one residue range */
+template <typename R>
 __global__ __launch_bounds__(64) void k_hcp_born(HcpDev H, const double *__restrict__ x)
 {
     const HcpItem it = hcp_item(H);
     const int i = it.i;
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
-    const double ri = H.atom[i].y, ri1i = 1. / ri;
+    const R ri = (R)H.atom[i].y, ri1i = R(1) / ri;
     const int nq = H.nq;
     double sumi = 0.0;
     hcp_walk(
@@ -594,10 +623,10 @@ __global__ __launch_bounds__(64) void k_hcp_born(HcpDev H, const double *__restr
         [&](int l, int comp) {
             const double *qc = lvl_q(H, l) + 4 * (size_t)nq * comp;
             const double *rc = lvl_r(H, l) + (size_t)nq * comp;
-            for (int q = 0; q < nq; q++) sumi -= far_descreen(hcp_d2(xi, yi, zi, qc + 4 * q), rc[q]);
+            for (int q = 0; q < nq; q++) sumi -= far_descreen<R>(hcp_d2(xi, yi, zi, qc + 4 * q), rc[q]);
         },
         [&](int a, bool on) {
-            if (on) sumi -= near_descreen(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z);
+            if (on) sumi -= near_descreen<R>(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z);
         });
     if (it.valid && sumi != 0.0) atomicAdd(&H.sumi[i], sumi);
 }
@@ -767,13 +796,14 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_reff_up(HcpDev H)
 
 /* hcp_gb.c:1843-1976 (egb_hcp_nbond, step 1) over one residue range:
  * energies, sumdeijda and the pair forces on each atom of the tile */
-template <bool KAPPA>
+template <typename R, bool KAPPA>
 __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__restrict__ x, DevPtrs d)
 {
     const HcpItem it = hcp_item(H);
     const int i = it.i;
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
-    const double ri = H.reff[i], qi = H.atom[i].x, kappa = H.kappa, inv_eps = 1.0 / H.epsext;
+    const double qi = H.atom[i].x;
+    const R ri = (R)H.reff[i], kks = (R)(-HCP_KSCALE * H.kappa), inv_eps = (R)(1.0 / H.epsext);
     const int2 lji = H.lj[i];
     const int nq = H.nq;
     int ep = H.ex_ptr[i];
@@ -789,7 +819,7 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
                 if (fabs(qv[3]) > HCP_MIN_Q) {
                     const double xij = xi - qv[0], yij = yi - qv[1], zij = zi - qv[2];
                     const double r2 = xij * xij + yij * yij + zij * zij;
-                    const double de = gb_pair<KAPPA>(qi, ri, qv[3], rf[q], r2, kappa, inv_eps, eg, el, sd, true);
+                    const double de = gb_pair<R, KAPPA>(qi, ri, qv[3], (R)rf[q], r2, kks, inv_eps, eg, el, sd, true);
                     dx += de * xij;
                     dy += de * yij;
                     dz += de * zij;
@@ -804,10 +834,10 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
             const bool excl = ep < ee && H.ex_j[ep] == a;
             const double xij = xi - x[3 * (size_t)a], yij = yi - x[3 * (size_t)a + 1], zij = zi - x[3 * (size_t)a + 2];
             const double r2 = xij * xij + yij * yij + zij * zij;
-            double de = gb_pair<KAPPA>(qi, ri, H.atom[a].x, H.reff[a], r2, kappa, inv_eps, eg, el, sd, !excl);
+            double de = gb_pair<R, KAPPA>(qi, ri, H.atom[a].x, (R)H.reff[a], r2, kks, inv_eps, eg, el, sd, !excl);
             if (!excl) {
                 const double2 ab = H.ljab[lji.x + H.lj[a].y];
-                const double rinv = rsqrt(r2), r2inv = rinv * rinv, r6inv = r2inv * r2inv * r2inv;
+                const double rinv = hcp_rinv<R>(r2), r2inv = rinv * rinv, r6inv = r2inv * r2inv * r2inv;
                 const double f6 = ab.y * r6inv, f12 = ab.x * r6inv * r6inv;
                 ev += f12 - f6;
                 de -= (12. * f12 - 6. * f6) * r2inv;
@@ -838,6 +868,7 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
  * each member's sum is added to its gradient with one atomic per residue
  * visit.
  */
+template <typename R>
 __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__restrict__ x, DevPtrs d)
 {
     __shared__ double s_pos[4][64]; /* x, y, z, scaled radius of the staged members */
@@ -847,14 +878,14 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
     const bool valid = it.valid && !H.frz[i];
     const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
     const double4 ai = H.atom[i];
-    const double ri = ai.y, ri1i = 1. / ri;
+    const R ri = (R)ai.y, ri1i = R(1) / ri;
     double sumda = H.sd[i];
     if (it.valid && it.r0 == 0) d.sumdeijda[H.inv[i]] = sumda;
     if (H.gb > 1) {
         const double4 g = H.obc[i];
         const double ps = H.psi[i];
         const double thi = tanh((g.x - g.y * ps + g.z * ps * ps) * ps);
-        sumda *= (g.x - 2.0 * g.y * ps + 3.0 * g.z * ps * ps) * (1.0 - thi * thi) * ri / ai.w;
+        sumda *= (g.x - 2.0 * g.y * ps + 3.0 * g.z * ps * ps) * (1.0 - thi * thi) * ai.y / ai.w;
     }
     const int nq = H.nq;
     double dx = 0.0, dy = 0.0, dz = 0.0;
@@ -867,7 +898,7 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
                 const double *qv = qc + 4 * q;
                 const double xij = xi - qv[0], yij = yi - qv[1], zij = zi - qv[2];
                 const double r2 = xij * xij + yij * yij + zij * zij;
-                const double datmp = descreen_deriv(r2, ri, ri1i, rc[q]) * (sumda * 2);
+                const double datmp = descreen_deriv<R>(r2, ri, ri1i, rc[q]) * (sumda * 2);
                 dx += xij * datmp;
                 dy += yij * datmp;
                 dz += zij * datmp;
@@ -894,7 +925,7 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
                         if (base + k != i) {
                             const double xij = xi - s_pos[0][k], yij = yi - s_pos[1][k], zij = zi - s_pos[2][k];
                             const double r2 = xij * xij + yij * yij + zij * zij;
-                            const double datmp = descreen_deriv(r2, ri, ri1i, s_pos[3][k]) * sumda;
+                            const double datmp = descreen_deriv<R>(r2, ri, ri1i, s_pos[3][k]) * sumda;
                             const double cx = xij * datmp, cy = yij * datmp, cz = zij * datmp;
                             dx += cx;
                             dy += cy;
@@ -950,7 +981,9 @@ int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags)
         hipLaunchKernelGGL(k_hcp_up<false>, dim3(1), dim3(HCP_THREADS), 0, E.stream, H);
     }
     const unsigned items = (unsigned)H.ntile * (unsigned)H.nchunk;
-    hipLaunchKernelGGL(k_hcp_born, dim3(items), dim3(64), 0, E.stream, H, x_dev);
+    const bool mixed = E.precision == GBHIP_MIXED;
+    if (mixed) hipLaunchKernelGGL(k_hcp_born<float>, dim3(items), dim3(64), 0, E.stream, H, x_dev);
+    else hipLaunchKernelGGL(k_hcp_born<double>, dim3(items), dim3(64), 0, E.stream, H, x_dev);
     hipLaunchKernelGGL(k_hcp_radii, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[1], E.stream));
     if (H.opca) {
@@ -961,12 +994,17 @@ int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags)
         hipLaunchKernelGGL(k_hcp_reff_up<false>, dim3(1), dim3(HCP_THREADS), 0, E.stream, H);
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], E.stream));
-    if (H.kappa != 0.0)
-        hipLaunchKernelGGL(k_hcp_energy<true>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    if (mixed && H.kappa != 0.0)
+        hipLaunchKernelGGL((k_hcp_energy<float, true>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    else if (mixed)
+        hipLaunchKernelGGL((k_hcp_energy<float, false>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    else if (H.kappa != 0.0)
+        hipLaunchKernelGGL((k_hcp_energy<double, true>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     else
-        hipLaunchKernelGGL(k_hcp_energy<false>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+        hipLaunchKernelGGL((k_hcp_energy<double, false>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
-    hipLaunchKernelGGL(k_hcp_deriv, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    if (mixed) hipLaunchKernelGGL(k_hcp_deriv<float>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    else hipLaunchKernelGGL(k_hcp_deriv<double>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (!(flags & EV_NO_POST)) {
         const int nene = 3 * E.nmol, nt = std::max(64 * nene, g_dev ? E.nslots : 0);
