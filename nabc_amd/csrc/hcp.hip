@@ -28,11 +28,13 @@
  *   k_hcp_deriv    the chain rule through the Born radii (step 2); forces
  *                  on member atoms summed in LDS, one atomic per atom and
  *                  residue visit
- * then the engine's own energy-slot and gradient kernels.  Everything is
- * FP64 except what the reference computes in single precision (component
- * radii, the HCPO moments and point charges), which is single precision
- * here too, compiled without contraction so each operation rounds as it
- * does there.
+ * then the engine's own energy-slot and gradient kernels.  The walking
+ * kernels take the context's precision: FP64, or the mixed variant with
+ * the descreening and GB terms in FP32 (Coulomb, LJ, coordinate
+ * differences and every sum FP64).  What the reference computes in single
+ * precision (component radii, the HCPO moments and point charges) is
+ * single precision here in both, compiled without contraction so each
+ * operation rounds as it does there.
  *
  * The walking kernels (born, energy, deriv) take work items of one
  * wavefront: the 64 atoms of a tile (one per lane, original order) against
