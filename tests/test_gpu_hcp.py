@@ -17,6 +17,9 @@
   - the mixed variant (descreening and GB terms in FP32, Coulomb, LJ and
     every sum in FP64) on thioredoxin and the layered system: energies to
     1e-6 relative, gradient to 1e-5 relative RMS, Born radii to 1e-5;
+  - edge hierarchies (one residue for the whole molecule, one per atom),
+    refused hierarchies and options (GBHIP_EINVAL), and the return to the
+    pair-list path when HCP is switched off;
   - dynamics and minimisation through the host API with hcp on: 40 Langevin
     steps (one hipGraph each) against the oracle's md() (coordinates and
     velocities to 1e-8), and 15 conjgrad iterations with the device-resident
@@ -197,3 +200,46 @@ def test_cg_follows_oracle(gpu, trx):
     assert rd == rr, (rd, rr)
     assert abs(fd - fr) <= 1e-9 * abs(fr), (fd, fr)
     assert np.max(np.abs(xd - xr)) <= 1e-7, np.max(np.abs(xd - xr))
+
+
+def test_edge_hierarchies_and_errors(gpu):
+    """One residue holding the whole molecule, one residue per atom, a
+    single strand and complex; invalid hierarchies and options are refused
+    with GBHIP_EINVAL; switching HCP off returns to the pair-list path."""
+    na = gpu
+    p, a, x = na.build_system(300, 61)
+    n = a["natom"]
+    cases = [(np.array([0, n], np.int32), (20.0, 50.0, 150.0)),
+             (np.arange(n + 1, dtype=np.int32), (6.0, 50.0, 150.0))]
+    for ipres, h in cases:
+        for hcp in (1, 2):
+            print("edge hierarchy: nres=%d hcp=%d" % (len(ipres) - 1, hcp), flush=True)
+            _compare(na, a, x, 5, hcp, (ipres, np.array([1], np.int32), np.array([1], np.int32)), h, kappa=0.1,
+                     parm=p)
+    ctx = na.GBContext(a, x, gb=5, cut=999.0, rgbmax=999.0)
+    good = layered_hierarchy(n)
+    bad = [(3, good), (1, (np.array([1, n], np.int32), good[1], good[2])),           # hcp 3; not from atom 0
+           (1, (np.array([0, n - 1], np.int32), good[1], good[2])),                    # not ending at natom
+           (1, (np.array([0, 200, 100, n], np.int32), good[1], good[2])),              # decreasing
+           (1, (good[0], np.array([2], np.int32), good[2])),                           # strand not from 1
+           (1, (good[0], np.array([1, 1], np.int32), good[2])),                        # strands not increasing
+           (1, (good[0], good[1], np.array([1, 99], np.int32)))]                       # complex beyond strands
+    for hcp, hier in bad:
+        with pytest.raises(na.GBHipError) as ei:
+            ctx.set_hcp(hcp, *hier)
+        assert "code -1" in str(ei.value), ei.value
+    ctx.set_hcp(1, *good, h1=12.0)
+    with pytest.raises(na.GBHipError):
+        ctx.set_deterministic(1)
+    with pytest.raises(na.GBHipError):
+        ctx.autotune(1)
+    e_hcp = ctx.egb()
+    ctx.set_hcp(0, *good)
+    ctx.nblist()
+    e_pair = ctx.egb()
+    o = oracle.Oracle(p, n, gb=5, cut=999.0, rgbmax=999.0)
+    o.nblist(x)
+    ref = o.egb(x)
+    assert rel(e_pair[0], ref[0]) <= 1e-9 and rel_rms(e_pair[3], ref[3]) <= 1e-9
+    assert abs(e_hcp[0] - e_pair[0]) > 1e-6 * abs(ref[0])  # the HCP result was a different one
+    ctx.close()
