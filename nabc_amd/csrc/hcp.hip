@@ -563,13 +563,98 @@ __device__ __forceinline__ void hcp_walk_res(const HcpDev &H, int r0, int r1, bo
     }
 }
 
+/* member(a) for each member a != i of each residue near to this lane, in a
+ * wavefront-uniform loop over the members (lanes without the residue idle) */
 template <typename Far, typename Member>
 __device__ __forceinline__ void hcp_walk(const HcpDev &H, int r0, int r1, bool valid, int i, double xi, double yi,
                                          double zi, Far far, Member member)
 {
     hcp_walk_res(H, r0, r1, valid, xi, yi, zi, far, [&](int r, bool in1) {
-        for (int a = H.ipres[r]; a < H.ipres[r + 1]; a++) member(a, in1 && a != i);
+        if (in1)
+            for (int a = H.ipres[r]; a < H.ipres[r + 1]; a++)
+                if (a != i) member(a);
     });
+}
+
+/*
+ * The same walk with the near residues taken per lane: while the far terms
+ * are evaluated, each lane marks its near residues in a 64-bit mask (bit k
+ * = residue base + k); every 64 residues, and at the end, the lanes drain
+ * their own masks independently, atom after atom in increasing order,
+ * member(a) for each member a != i of each marked residue, all lanes busy
+ * until the longest list ends.  A tile's lanes disagree about most of the
+ * residues near them (a residue is near to some of the 64 atoms and far
+ * from the rest), which leaves a wavefront-uniform atom loop with half its
+ * lanes idle; here only the far charges of such residues run on part of
+ * the wavefront.  member() must not use cross-lane operations.  Used by the
+ * energy pass (FP64 2.99 -> 2.86 ms at 100k atoms, mixed unchanged); the
+ * Born pass, whose member terms are short, was slower with it (mixed 0.78
+ * -> 0.83 ms: the per-lane gathers of member data cost more than the idle
+ * lanes).
+ */
+template <typename Far, typename Member>
+__device__ __forceinline__ void hcp_walk_lanes(const HcpDev &H, int r0, int r1, bool valid, int i, double xi,
+                                               double yi, double zi, Far far, Member member)
+{
+    unsigned long long nm = 0ull;
+    int base = r0;
+    auto drain = [&]() {
+        if (!wave_any(nm != 0ull)) return;
+        int a = 0, aend = 0;
+        for (;;) {
+            while (a == aend && nm != 0ull) {
+                const int k = __ffsll((long long)nm) - 1;
+                nm &= nm - 1ull;
+                a = H.ipres[base + k];
+                aend = H.ipres[base + k + 1];
+            }
+            const bool on = a < aend;
+            if (!wave_any(on)) break;
+            if (on) {
+                if (a != i) member(a);
+                a++;
+            }
+        }
+        nm = 0ull;
+    };
+    int r = r0;
+    while (r < r1) {
+        const int s = H.res_s[r], c = H.s_cx[s];
+        const int sr0 = H.rfirst[s], sr1 = H.rfirst[s + 1];
+        const int cr0 = H.rfirst[H.sfirst[c]], cr1 = H.rfirst[H.sfirst[c + 1]];
+        const bool f3 = valid && hcp_d2(xi, yi, zi, H.xc3 + 3 * c) > H.h3sq;
+        if (r == cr0 && wave_any(f3)) {
+            if (f3) far(2, c);
+        }
+        const bool in3 = valid && !f3;
+        if (!wave_any(in3)) {
+            r = min(r1, cr1);
+            continue;
+        }
+        const double ds = hcp_d2(xi, yi, zi, H.xc2 + 3 * s);
+        const bool f2 = in3 && ds > H.h2sq;
+        if (r == sr0 && wave_any(f2)) {
+            if (f2) far(1, s);
+        }
+        const bool in2 = in3 && !(ds > H.h2sq);
+        const int rend = min(r1, sr1);
+        if (wave_any(in2)) {
+            for (; r < rend; r++) {
+                if (r - base >= 64) {
+                    drain();
+                    base = r;
+                }
+                const double dr = hcp_d2(xi, yi, zi, H.xc1 + 3 * r);
+                const bool f1 = in2 && dr > H.h1sq;
+                if (wave_any(f1)) {
+                    if (f1) far(0, r);
+                }
+                if (in2 && !(dr > H.h1sq)) nm |= 1ull << (r - base);
+            }
+        }
+        r = rend;
+    }
+    drain();
 }
 
 /* the work item of this wavefront: its tile's atom for this lane and its
@@ -627,9 +712,7 @@ __global__ __launch_bounds__(64) void k_hcp_born(HcpDev H, const double *__restr
             const double *rc = lvl_r(H, l) + (size_t)nq * comp;
             for (int q = 0; q < nq; q++) sumi -= far_descreen<R>(hcp_d2(xi, yi, zi, qc + 4 * q), rc[q]);
         },
-        [&](int a, bool on) {
-            if (on) sumi -= near_descreen<R>(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z);
-        });
+        [&](int a) { sumi -= near_descreen<R>(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z); });
     if (it.valid && sumi != 0.0) atomicAdd(&H.sumi[i], sumi);
 }
 
@@ -811,7 +894,7 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
     int ep = H.ex_ptr[i];
     const int ee = H.ex_ptr[i + 1];
     double sd = 0.0, eg = 0.0, el = 0.0, ev = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-    hcp_walk(
+    hcp_walk_lanes(
         H, it.r0, it.r1, it.valid, i, xi, yi, zi,
         [&](int l, int comp) {
             const double *qc = lvl_q(H, l) + 4 * (size_t)nq * comp;
@@ -828,10 +911,9 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
                 }
             }
         },
-        [&](int a, bool on) {
-            if (!on) return;
-            /* the walk meets atoms in increasing order: a pointer into the
-             * ascending excluded-partner list finds them */
+        [&](int a) {
+            /* each lane meets its atoms in increasing order: a pointer into
+             * the ascending excluded-partner list finds them */
             while (ep < ee && H.ex_j[ep] < a) ep++;
             const bool excl = ep < ee && H.ex_j[ep] == a;
             const double xij = xi - x[3 * (size_t)a], yij = yi - x[3 * (size_t)a + 1], zij = zi - x[3 * (size_t)a + 2];
