@@ -194,7 +194,11 @@ int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const 
  * reference's single-precision parts (component radii, HCPO moments and
  * charges) stay single precision in both.  hcp = 0 (or h == NULL) returns to the pair-list path
  * (a list must be built again).  Single-molecule contexts only; not with
- * the deterministic mode; not while dynamics graphs exist.
+ * the deterministic mode; not while dynamics graphs exist.  Each call with
+ * hcp != 0 allocates the hierarchy's device arrays anew (about 120 bytes
+ * per atom plus the exclusions, from the slot's region for a context made
+ * in a slot); those of earlier calls stay allocated until the context is
+ * destroyed.
  */
 typedef struct {
     int hcp;              /* 0 (off), 1, 2 or 4 */
