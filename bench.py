@@ -667,6 +667,50 @@ def bench_hcp(args):
     print(json.dumps(line))
 
 
+def bench_hcp_sweep(args):
+    """Size sweep of the egb_hcp workload (hcp 1, 20-atom residues, strands
+    of 50 residues, complexes of 10 strands, thresholds 15/50/150 A, gb=5,
+    kappa 0.10395) at 25k-200k atoms: evaluations/s per precision, HIP-event
+    pass times, the walk's units and the energy kernel's rate."""
+    import numpy as np
+    import nabc_amd as na
+    F = HCP_FLOPS
+    rows = []
+    for natom in (25000, 50000, 100000, 200000):
+        parm, arr, x = na.build_system(natom, HCP_SEED)
+        ipres, ips, ipc = hcp_layers(natom)
+        print("# %d atoms: counting the walk's units on the host" % natom, file=sys.stderr, flush=True)
+        checks, far, near = hcp_walk_counts(x, ipres, ips, ipc, HCP_H)
+        w_energy = checks * F["check"] + far * F["energy_far"] + near * F["energy_near"]
+        row = {"natom": natom, "residues": len(ipres) - 1, "strands": len(ips), "complexes": len(ipc),
+               "walk": {"checks": checks, "far_components": far, "near_pairs": near}}
+        steps = max(10, min(100, int(4e6 / natom)))
+        for pname, prec in (("mixed", na.GBHIP_MIXED), ("fp64", na.GBHIP_FP64)):
+            ctx = na.GBContext(arr, x, gb=5, cut=999.0, rgbmax=999.0, kappa=HCP_KAPPA, precision=prec)
+            ctx.set_hcp(1, ipres, ips, ipc, h1=HCP_H[0], h2=HCP_H[1], h3=HCP_H[2])
+            ctx.nblist()
+            for _ in range(3):
+                ctx.egb_device()
+            ctx.sync()
+            ctx.timing_start(steps)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.egb_device()
+            ctx.sync()
+            wall = (time.perf_counter() - t0) / steps
+            ms, ne = ctx.timing_stop()
+            ctx.close()
+            pm = [v / ne for v in ms]
+            row[pname] = {"evals_per_s": round(1.0 / wall, 2), "ms_per_eval": round(1e3 * wall, 4),
+                          "pass_ms": [round(v, 4) for v in pm],
+                          "energy_tflops": round(w_energy / (pm[2] * 1e-3) / 1e12, 3)}
+        print("# %s" % json.dumps(row), file=sys.stderr, flush=True)
+        rows.append(row)
+    print(json.dumps({"metric": "HCP GB energy+grad evals/s by system size", "unit": "evals/s",
+                      "pass_order": ["charges_born", "component_radii", "energy", "deriv"],
+                      "data": "synthetic (seeded polymers, nabc_amd/csrc/gen.c)", "rows": rows}))
+
+
 def bench_sweep(args):
     """Size sweep of the headline workload (igb=8, cut = rgbmax = 25 A,
     kappa 0.127315, list rebuilt every 20 evaluations): evaluations/s and
@@ -722,7 +766,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates HBM byte counters)")
     ap.add_argument("--config", default="c3",
-                    choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg", "hessian", "hcp"],
+                    choices=["c3", "batch", "md", "serve", "accuracy", "sweep", "cg", "hessian", "hcp", "hcp-sweep"],
                     help="c3: 25k-atom evaluation rate (default); batch: 1,024-molecule scoring; "
                          "md: dynamics (see --md); serve: request loop; accuracy: per-term mme() "
                          "errors against the oracle at C1/C2/C3; sweep: evaluation rate at 5k-100k atoms")
@@ -755,6 +799,8 @@ def main():
         return bench_accuracy(args)
     if args.config == "sweep":
         return bench_sweep(args)
+    if args.config == "hcp-sweep":
+        return bench_hcp_sweep(args)
     if args.config == "cg":
         return bench_cg(args)
     if args.config == "hessian":

@@ -196,9 +196,10 @@ int gbhip_egb_static(const gbhip_topology *top, const gbhip_options *opt, const 
  * (a list must be built again).  Single-molecule contexts only; not with
  * the deterministic mode; not while dynamics graphs exist.  Each call with
  * hcp != 0 allocates the hierarchy's device arrays anew (about 120 bytes
- * per atom plus the exclusions, from the slot's region for a context made
- * in a slot); those of earlier calls stay allocated until the context is
- * destroyed.
+ * per atom plus the exclusions, and 2 KiB per work item for partial sums,
+ * up to about 135 MB on a 256-CU device; from the slot's region for
+ * a context made in a slot); those of earlier calls stay allocated until
+ * the context is destroyed.
  */
 typedef struct {
     int hcp;              /* 0 (off), 1, 2 or 4 */

@@ -168,7 +168,10 @@ struct HcpDev {
     double *rw = nullptr;    /* per residue: its terms of the strand averages (6) */
     double *sw = nullptr;    /* per strand: its averaging sums (4) */
     double *reff = nullptr, *psi = nullptr, *sd = nullptr; /* per atom */
-    double *sumi = nullptr;       /* per atom: descreening sum (added by residue range) */
+    /* per work item, its lanes' partial sums, [component][range][tile * 64 + lane]
+     * (4 components: Born sum; energy sd, x, y, z; chain rule x, y, z), added
+     * up per atom in a fixed order instead of by atomics */
+    double *part = nullptr;
     const int *res_s = nullptr;   /* per residue: its strand */
     const int *s_cx = nullptr;    /* per strand: its complex */
     int ntile = 0;                /* 64-atom tiles of the work items */

@@ -38,8 +38,13 @@
  *
  * The walking kernels (born, energy, deriv) take work items of one
  * wavefront: the 64 atoms of a tile (one per lane, original order) against
- * one range of consecutive residues, about 512 work items per CU, their
- * partial sums added with atomics.  Within an item the hierarchy loops are
+ * one range of consecutive residues, about 256 work items per CU, range
+ * after range over the tiles.  An item stores its lanes' sums (Born sums,
+ * sumdeijda, forces on the lane's atom) in a buffer of its own, and a pass
+ * over the atoms adds them up in range order: atomics from every item of
+ * a tile to its 64 atoms had cost about as much as the walk itself at 25k
+ * atoms.  The forces on member atoms in the chain rule still go through
+ * LDS sums and atomics.  Within an item the hierarchy loops are
  * wavefront-uniform (every lane steps through the same complexes, strands,
  * residues and atoms) and a lane's branch (far charges or the component's
  * members) runs only where some lane needs it.
@@ -415,7 +420,6 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_res(HcpDev H, const double 
     for (int a = a0; a < a1; a++) {
         const double xa = x[3 * (size_t)a], ya = x[3 * (size_t)a + 1], za = x[3 * (size_t)a + 2];
         const double q = H.atom[a].x;
-        H.sumi[a] = 0.0; /* the descreening sums k_hcp_born adds up */
         s[0] += xa;
         s[1] += ya;
         s[2] += za;
@@ -661,19 +665,28 @@ __device__ __forceinline__ void hcp_walk_lanes(const HcpDev &H, int r0, int r1, 
  * residue range */
 struct HcpItem {
     int i, r0, r1;
+    int k, t; /* residue range, and tile * 64 + lane */
     bool valid;
 };
 
 __device__ __forceinline__ HcpItem hcp_item(const HcpDev &H)
 {
-    const int b = blockIdx.x, tile = b / H.nchunk, k = b - tile * H.nchunk;
+    const int b = blockIdx.x, k = b / H.ntile, tile = b - k * H.ntile;
     HcpItem it;
     const int t = tile * 64 + (int)threadIdx.x;
+    it.k = k;
+    it.t = t;
     it.valid = t < H.natom;
     it.i = it.valid ? t : H.natom - 1;
     it.r0 = k * H.csize;
     it.r1 = min(H.nres, it.r0 + H.csize);
     return it;
+}
+
+/* this lane's partial sum c of its item */
+__device__ __forceinline__ double &hcp_part(const HcpDev &H, const HcpItem &it, int c)
+{
+    return H.part[((size_t)c * H.nchunk + it.k) * ((size_t)H.ntile * 64) + it.t];
 }
 
 __device__ __forceinline__ const double *lvl_q(const HcpDev &H, int l)
@@ -713,7 +726,7 @@ __global__ __launch_bounds__(64) void k_hcp_born(HcpDev H, const double *__restr
             for (int q = 0; q < nq; q++) sumi -= far_descreen<R>(hcp_d2(xi, yi, zi, qc + 4 * q), rc[q]);
         },
         [&](int a) { sumi -= near_descreen<R>(hcp_d2(xi, yi, zi, x + 3 * (size_t)a), ri, ri1i, H.atom[a].z); });
-    if (it.valid && sumi != 0.0) atomicAdd(&H.sumi[i], sumi);
+    if (it.valid) hcp_part(H, it, 0) = sumi;
 }
 
 /*
@@ -727,7 +740,10 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_radii(HcpDev H, DevPtrs d)
     const bool valid = t < H.natom;
     const int i = valid ? t : H.natom - 1;
     const double4 ai = H.atom[i];
-    const double ri = ai.y, ri1i = 1. / ri, sumi = H.sumi[i];
+    const size_t np = (size_t)H.ntile * 64;
+    double sumi = 0.0;
+    for (int k = 0; k < H.nchunk; k++) sumi += H.part[(size_t)k * np + i];
+    const double ri = ai.y, ri1i = 1. / ri;
     double reff, psi = 0.0;
     if (H.gb == 1) {
         reff = 1.0 / (ri1i + sumi);
@@ -931,13 +947,10 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
             dz += de * zij;
         });
     if (it.valid) {
-        if (sd != 0.0) atomicAdd(&H.sd[i], sd);
-        const int sl = H.inv[i];
-        if (dx != 0.0 || dy != 0.0 || dz != 0.0) {
-            atomicAdd(&d.gx[sl], dx);
-            atomicAdd(&d.gy[sl], dy);
-            atomicAdd(&d.gz[sl], dz);
-        }
+        hcp_part(H, it, 0) = sd;
+        hcp_part(H, it, 1) = dx;
+        hcp_part(H, it, 2) = dy;
+        hcp_part(H, it, 3) = dz;
     }
     ene_add(d, 0, it.valid ? 0.5 * eg : 0.0);
     ene_add(d, 1, it.valid ? 0.5 * ev : 0.0);
@@ -1033,12 +1046,33 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
                 }
             }
         });
-    if (valid && (dx != 0.0 || dy != 0.0 || dz != 0.0)) {
-        const int sl = H.inv[i];
-        atomicAdd(&d.gx[sl], -dx);
-        atomicAdd(&d.gy[sl], -dy);
-        atomicAdd(&d.gz[sl], -dz);
+    if (it.valid) {
+        hcp_part(H, it, 0) = -dx;
+        hcp_part(H, it, 1) = -dy;
+        hcp_part(H, it, 2) = -dz;
     }
+}
+
+/* per atom: the partial sums of the energy (SD: sd, then the force) or
+ * chain-rule pass over its tile's residue ranges, in range order, added to
+ * sd and (one atomic per component) to the gradient */
+template <bool SD>
+__global__ __launch_bounds__(HCP_THREADS) void k_hcp_sum(HcpDev H, DevPtrs d)
+{
+    const int a = blockIdx.x * HCP_THREADS + threadIdx.x;
+    if (a >= H.natom) return;
+    const size_t np = (size_t)H.ntile * 64, cs = (size_t)H.nchunk * np;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < H.nchunk; k++) {
+        const double *p = H.part + (size_t)k * np + a;
+        for (int c = 0; c < (SD ? 4 : 3); c++) s[c] += p[c * cs];
+    }
+    const double *g = SD ? s + 1 : s;
+    if (SD) H.sd[a] += s[0];
+    const int sl = H.inv[a];
+    if (g[0] != 0.0) atomicAdd(&d.gx[sl], g[0]);
+    if (g[1] != 0.0) atomicAdd(&d.gy[sl], g[1]);
+    if (g[2] != 0.0) atomicAdd(&d.gz[sl], g[2]);
 }
 
 /* ------------------------------------------------------------ host side */
@@ -1086,9 +1120,12 @@ int engine_eval_hcp(Engine &E, const double *x_dev, double *g_dev, int flags)
         hipLaunchKernelGGL((k_hcp_energy<double, true>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     else
         hipLaunchKernelGGL((k_hcp_energy<double, false>), dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    /* the energy pass is timed alone; its sums count with the chain rule's */
     if (ev) HIPCHK(hipEventRecord(ev[3], E.stream));
+    hipLaunchKernelGGL(k_hcp_sum<true>, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, E.d);
     if (mixed) hipLaunchKernelGGL(k_hcp_deriv<float>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
     else hipLaunchKernelGGL(k_hcp_deriv<double>, dim3(items), dim3(64), 0, E.stream, H, x_dev, E.d);
+    hipLaunchKernelGGL(k_hcp_sum<false>, dim3(gat), dim3(HCP_THREADS), 0, E.stream, H, E.d);
     if (ev) HIPCHK(hipEventRecord(ev[4], E.stream));
     if (!(flags & EV_NO_POST)) {
         const int nene = 3 * E.nmol, nt = std::max(64 * nene, g_dev ? E.nslots : 0);
@@ -1227,13 +1264,15 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
         for (int r = rfirst[s]; r < rfirst[s + 1]; r++) res_s[r] = s;
     for (int c = 0; c < h->ncomplex; c++)
         for (int s = sfirst[c]; s < sfirst[c + 1]; s++) s_cx[s] = c;
-    /* work items: 64-atom tiles x ranges of csize residues, about 512 per CU
+    /* work items: 64-atom tiles x ranges of csize residues, about 256 per CU
      * (ranges of nearby and of distant residues differ several-fold in
-     * work; 100k atoms, per evaluation: 7.66 / 7.22 / 6.89 / 6.40 / 6.22 /
-     * 6.91 / 8.01 ms at 32 / 64 / 128 / 256 / 512 / 1024 / 2048 per CU, the
-     * last ones paying for the per-range atomics) */
+     * work).  Mixed hcp 1, ms per evaluation at 25k / 100k / 200k atoms for
+     * 128 / 192 / 256 / 384 / 512 / 1024 items per CU: 1.06 / 3.84 / 7.99,
+     * 1.09 / 3.80 / 7.88, 1.13 / 3.72 / 7.86, 1.25 / 3.77 / 7.78,
+     * 1.34 / 3.79 / 7.76, 1.34 / 4.24 / 7.90 (the partial-sum buffer and
+     * its sums grow with the item count) */
     const int ntile = (n + 63) / 64;
-    const long long target = 512LL * std::max(1, E.ncu);
+    const long long target = 256LL * std::max(1, E.ncu);
     const long long want_chunks = std::max(1LL, (target + ntile - 1) / ntile);
     int csize = (int)std::max(4LL, ((long long)h->nres + want_chunks - 1) / want_chunks);
     csize = std::min(csize, h->nres);
@@ -1274,7 +1313,7 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
         (rc = E.dalloc(&H.smp, (size_t)MP_N * h->nstrand)) || (rc = E.dalloc(&H.rw, 6 * (size_t)h->nres)) ||
         (rc = E.dalloc(&H.sw, 4 * (size_t)h->nstrand)) || (rc = E.dalloc(&H.reff, (size_t)n)) ||
         (rc = E.dalloc(&H.psi, (size_t)n)) || (rc = E.dalloc(&H.sd, (size_t)n)) ||
-        (rc = E.dalloc(&H.sumi, (size_t)n)) || (rc = E.upload(&H.res_s, res_s)) || (rc = E.upload(&H.s_cx, s_cx)))
+        (rc = E.dalloc(&H.part, 4 * (size_t)nchunk * ntile * 64)) || (rc = E.upload(&H.res_s, res_s)) || (rc = E.upload(&H.s_cx, s_cx)))
         return rc;
     if ((rc = E.upload_done())) return rc;
     H.on = 1;
