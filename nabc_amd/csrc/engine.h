@@ -172,6 +172,7 @@ struct HcpDev {
      * (4 components: Born sum; energy sd, x, y, z; chain rule x, y, z), added
      * up per atom in a fixed order instead of by atomics */
     double *part = nullptr;
+    double *gm = nullptr; /* chain rule: forces on member atoms, original order, x | y | z */
     const int *res_s = nullptr;   /* per residue: its strand */
     const int *s_cx = nullptr;    /* per strand: its complex */
     int ntile = 0;                /* 64-atom tiles of the work items */

@@ -27,7 +27,9 @@
  *                  forces on each atom (egb_hcp_nbond step 1)
  *   k_hcp_deriv    the chain rule through the Born radii (step 2); forces
  *                  on member atoms summed in LDS, one atomic per atom and
- *                  residue visit
+ *                  residue visit into an array in atom order
+ *   k_hcp_sum      per atom: the items' partial sums (and those member
+ *                  forces) into sd and the gradient
  * then the engine's own energy-slot and gradient kernels.  The walking
  * kernels take the context's precision: FP64, or the mixed variant with
  * the descreening and GB terms in FP32 (Coulomb, LJ, coordinate
@@ -765,6 +767,9 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_radii(HcpDev H, DevPtrs d)
         H.reff[i] = reff;
         H.psi[i] = psi;
         H.sd[i] = sd;
+        H.gm[i] = 0.0; /* the chain rule's member forces start here */
+        H.gm[H.natom + i] = 0.0;
+        H.gm[2 * (size_t)H.natom + i] = 0.0;
         const int sl = H.inv[i];
         d.reff[sl] = reff;
         d.psi[sl] = psi;
@@ -962,8 +967,10 @@ __global__ __launch_bounds__(64) void k_hcp_energy(HcpDev H, const double *__res
  * near residue's atoms are staged in LDS, 64 at a time, and the lanes step
  * through them in rotated order (lane l takes member (l + t) mod m at step
  * t), so that each step's partner-side terms go to different LDS sums;
- * each member's sum is added to its gradient with one atomic per residue
- * visit.
+ * each member's sum is added with one atomic per residue visit to a
+ * per-atom array in original order (consecutive lanes, consecutive
+ * addresses; through the slot map they had been 64 scattered addresses
+ * per wavefront), which k_hcp_sum adds to the gradient.
  */
 template <typename R>
 __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__restrict__ x, DevPtrs d)
@@ -1036,12 +1043,13 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
                 }
                 __syncthreads();
                 if (lane < m) {
-                    const int sa = H.inv[base + lane];
+                    /* original order: consecutive lanes, consecutive addresses */
+                    const int a = base + lane;
                     const double ax = s_acc[0][lane], ay = s_acc[1][lane], az = s_acc[2][lane];
                     if (ax != 0.0 || ay != 0.0 || az != 0.0) {
-                        atomicAdd(&d.gx[sa], ax);
-                        atomicAdd(&d.gy[sa], ay);
-                        atomicAdd(&d.gz[sa], az);
+                        atomicAdd(&H.gm[a], ax);
+                        atomicAdd(&H.gm[H.natom + a], ay);
+                        atomicAdd(&H.gm[2 * (size_t)H.natom + a], az);
                     }
                 }
             }
@@ -1054,8 +1062,9 @@ __global__ __launch_bounds__(64) void k_hcp_deriv(HcpDev H, const double *__rest
 }
 
 /* per atom: the partial sums of the energy (SD: sd, then the force) or
- * chain-rule pass over its tile's residue ranges, in range order, added to
- * sd and (one atomic per component) to the gradient */
+ * chain-rule pass (with the member forces) over its tile's residue ranges,
+ * in range order, added to sd and (one atomic per component) to the
+ * gradient */
 template <bool SD>
 __global__ __launch_bounds__(HCP_THREADS) void k_hcp_sum(HcpDev H, DevPtrs d)
 {
@@ -1067,6 +1076,8 @@ __global__ __launch_bounds__(HCP_THREADS) void k_hcp_sum(HcpDev H, DevPtrs d)
         const double *p = H.part + (size_t)k * np + a;
         for (int c = 0; c < (SD ? 4 : 3); c++) s[c] += p[c * cs];
     }
+    if (!SD)
+        for (int c = 0; c < 3; c++) s[c] += H.gm[(size_t)c * H.natom + a];
     const double *g = SD ? s + 1 : s;
     if (SD) H.sd[a] += s[0];
     const int sl = H.inv[a];
@@ -1313,7 +1324,7 @@ extern "C" int gbhip_set_hcp(gbhip_ctx *ctx, const gbhip_hcp *h)
         (rc = E.dalloc(&H.smp, (size_t)MP_N * h->nstrand)) || (rc = E.dalloc(&H.rw, 6 * (size_t)h->nres)) ||
         (rc = E.dalloc(&H.sw, 4 * (size_t)h->nstrand)) || (rc = E.dalloc(&H.reff, (size_t)n)) ||
         (rc = E.dalloc(&H.psi, (size_t)n)) || (rc = E.dalloc(&H.sd, (size_t)n)) ||
-        (rc = E.dalloc(&H.part, 4 * (size_t)nchunk * ntile * 64)) || (rc = E.upload(&H.res_s, res_s)) || (rc = E.upload(&H.s_cx, s_cx)))
+        (rc = E.dalloc(&H.part, 4 * (size_t)nchunk * ntile * 64)) || (rc = E.dalloc(&H.gm, 3 * (size_t)n)) || (rc = E.upload(&H.res_s, res_s)) || (rc = E.upload(&H.s_cx, s_cx)))
         return rc;
     if ((rc = E.upload_done())) return rc;
     H.on = 1;
